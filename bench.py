@@ -1,0 +1,172 @@
+#!/usr/bin/env python
+"""Benchmark: ns/day of the 95,000-atom water + ions NVE step on one MI355X.
+
+Workload: cubic box L = 99 A, 31,000 rigid 3-site waters + 1000 Na+ +
+1000 Cl-, cutoff 10 A, skin 2 A, PME 96^3 order 4, dt = 2 fs, FP32.  A step
+is one full MD step through the HIP library: neighbor-list check (and
+rebuild when an atom moved more than skin/2), PME excluded-pair correction,
+LJ + PME direct pair forces, PME reciprocal (spread, FFT convolution,
+gather), leap-frog and SETTLE.  Inputs are synthetic (seeded lattice water,
+random orientations, Maxwell-Boltzmann velocities) and resident on the GPU
+before the timed region.
+
+Prints one JSON line (see DESIGN.md for the fields).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "ns/day @96k-atom water+ions; learned-vs-plain speedup at ≤1% recip RMSE"
+FP32_PEAK_TFLOPS = 157.3          # MI355X FP32 vector, dual-issue packed
+FP32_PEAK_TFLOPS_UNPACKED = 78.6  # without packed FP32 instructions
+HBM_PEAK_GBS = 8000.0
+TIMERS = ("lj_pairs", "nl_search", "pme_spread", "pme_gather", "pme_fft_z_r2c", "pme_fft_y",
+          "pme_fft_x_conv", "pme_ifft_y", "pme_fft_z_c2r")
+
+
+def pair_kernel_flops(n_in, n_list):
+    """Algorithmic FP32 FLOP of one force-only pair-kernel launch: 45 per
+    in-cutoff pair (distance, minimum image, LJ and erfc-Coulomb force) and
+    17 per list entry beyond the cutoff (distance and minimum image)."""
+    return 45.0 * n_in + 17.0 * (n_list - n_in)
+
+
+def cpu_baseline(s, gpu_rebuild_interval, budget_s=20.0):
+    """The oracle's baseline build (the reference CPU path restated with its
+    OpenMP + atomic structure, -O3 -ffast-math) on the same coordinates:
+    force evaluations of the full system, list builds amortised over the
+    rebuild interval the GPU run observed."""
+    from oracle import oracle as O
+    lib = O.lib(fast=True)
+    threads = lib.orc_num_threads()
+    beta = O.get_beta(s.cutoff, s.tolerance)
+    t0 = time.perf_counter()
+    c, ser, _ = O.nl_build(s, fast=True)
+    t_nl = time.perf_counter() - t0
+    pm = O.PME(s, s.fft, beta, fast=True)
+    n = 0
+    t_force = 0.0
+    start = time.perf_counter()
+    while True:
+        t0 = time.perf_counter()
+        O.lj_direct(s, c, ser, beta, need_energy=False, need_virial=False, fast=True)
+        pm.reciprocal(need_energy=False, need_virial=False)
+        t_force += time.perf_counter() - t0
+        n += 1
+        if time.perf_counter() - start > budget_s or n >= 50:
+            break
+    per_step = t_force / n + t_nl / max(gpu_rebuild_interval, 1.0)
+    ns_day = 86.4 * s.dt_ps / per_step
+    return dict(value=ns_day, unit="ns/day", cores=threads, kind="port",
+                sample=f"{n} force evaluations (LJ + PME direct, PME reciprocal with the "
+                       f"oracle's own FFT) of the 95,000-atom system plus one list build "
+                       f"({t_nl:.2f} s) amortised over {gpu_rebuild_interval:.1f} steps; "
+                       f"{t_force / n * 1e3:.0f} ms per force evaluation",
+                ms_per_step=per_step * 1e3)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--config", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the hot path is "
+              "measured on one GPU's share; see DESIGN.md for how it would shard.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from sponge_amd import abi
+    from sponge_amd import system as S
+
+    s = S.config(args.config)
+    md = abi.MDSystem(s)
+    nl_h, lj_h, pm_h = md.handles()
+    lib = abi.load()
+    torch.cuda.synchronize()
+
+    md.run(args.warmup)
+    torch.cuda.synchronize()
+    rb0 = C_rebuilds(lib, nl_h)
+
+    abi.reset_kernel_timing()
+    abi.set_kernel_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    md.run(args.steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    abi.set_kernel_timing(False)
+    rb1 = C_rebuilds(lib, nl_h)
+    kernel = {k: abi.kernel_timing(k) for k in TIMERS}
+
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    ns_day = 86.4 * s.dt_ps / (ms_per_step * 1e-3)
+    rebuilds = rb1 - rb0
+    rebuild_interval = args.steps / max(rebuilds, 1)
+
+    # pair statistics of the current list, counted by the oracle
+    from oracle import oracle as O
+    N = s.n_atoms
+    M = 1200
+    counts = torch.zeros(N, dtype=torch.int32, device="cuda")
+    serial = torch.zeros(N * M, dtype=torch.int32, device="cuda")
+    rc = lib.sponge_nl_export_half_list(nl_h, abi._ptr(counts), abi._ptr(serial), M)
+    assert rc == 0
+    crd, _, _ = md.state()
+    torch.cuda.synchronize()
+    n_in, n_list = O.pair_counts(s, counts.cpu().numpy(), serial.view(N, M).cpu().numpy(), crd=crd)
+
+    lj_ms, lj_n = kernel["lj_pairs"]
+    lj_avg_s = lj_ms * 1e-3 / max(lj_n, 1)
+    flops = pair_kernel_flops(n_in, n_list)
+    achieved = flops / lj_avg_s / 1e12
+    roofline = dict(bound="valu", achieved=achieved, peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
+                    frac=achieved / FP32_PEAK_TFLOPS, traffic=None,
+                    kernel="lj_pairs", flop_per_launch=flops, pairs_in_cutoff=n_in,
+                    list_entries=n_list, avg_launch_us=lj_avg_s * 1e6,
+                    frac_of_unpacked_peak=achieved / FP32_PEAK_TFLOPS_UNPACKED)
+
+    out = dict(metric=METRIC, value=ns_day, unit="ns/day", n_gpus=1, steps=args.steps,
+               warmup=args.warmup, ms_per_step=ms_per_step, higher_is_better=True,
+               scaling="weak", vs_baseline=None, dtype="f32", data="synthetic",
+               config=dict(workload="water_ions_95k NVE step (31000 rigid TIP3P-like waters + "
+                                    "1000 Na+ + 1000 Cl-, L=99 A, rc=10 A, skin=2 A, PME 96^3 "
+                                    "order 4, dt=2 fs, SETTLE)",
+                           atoms=N, box_A=float(s.box[0]), cutoff_A=s.cutoff, skin_A=s.skin,
+                           pme_grid=list(s.fft), dt_fs=s.dt_ps * 1e3, parallelism="1 GPU"),
+               roofline=roofline,
+               force_evals_per_s=1e3 / ms_per_step,
+               list_rebuild_interval_steps=rebuild_interval,
+               kernel_avg_us={k: (v[0] * 1e3 / v[1] if v[1] else None) for k, v in kernel.items()})
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(s, rebuild_interval, args.cpu_budget)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+def C_rebuilds(lib, nl_h):
+    import ctypes
+    n = ctypes.c_longlong()
+    rc = lib.sponge_nl_rebuild_count(nl_h, ctypes.byref(n))
+    assert rc == 0
+    return n.value
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,313 @@
+/*
+ * sponge_amd.h -- C ABI of the MI355X (gfx950) nonbonded hot path:
+ * cell-grid neighbor list, LJ + PME direct-space pair forces with exclusions,
+ * and the PME reciprocal part (order-4 B-spline spread, 3D FFT convolution,
+ * force gather).
+ *
+ * Every entry point mirrors one method of the reference's modules and keeps
+ * its argument meaning:
+ *   NEIGHBOR_LIST              SPONGE/neighbor_list/neighbor_list.h:7-127
+ *   LENNARD_JONES_INFORMATION  SPONGE/Lennard_Jones_force/Lennard_Jones_force.h:87-173
+ *   Particle_Mesh              SPONGE/PM_force/PM_force.h:152-209
+ *
+ * Conventions (those of the reference):
+ *   - VECTOR arrays are float[3*n] AoS (x, y, z); LTMatrix3 is float[6] =
+ *     {a11, a21, a22, a31, a32, a33}; the box is passed as the two LTMatrix3 of
+ *     the reference's Boundary (cell and rcell), in HOST memory.
+ *   - Per-atom arrays (crd, frc, charge, energies, virials, exclusion CSR,
+ *     atom_local) are DEVICE pointers owned by the caller.
+ *   - Force, energy and virial outputs are ACCUMULATED (+=) into caller-zeroed
+ *     arrays; energies only when need_energy, virials only when need_virial.
+ *   - Every call is asynchronous on the stream given at *_initial (NULL =
+ *     the default stream) unless it says it synchronises.
+ *   - Return value: 0 on success; a HIP runtime error code (< 1000) when a
+ *     runtime call fails; otherwise a SPONGE error code (1001..1010,
+ *     SPONGE/utils/control/error.hpp:5-29).  Nothing calls exit().
+ */
+#ifndef SPONGE_AMD_H
+#define SPONGE_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum sponge_amd_status {
+    SPONGE_AMD_SUCCESS = 0,
+    SPONGE_AMD_ERROR_NOT_IMPLEMENTED = 1001,
+    SPONGE_AMD_ERROR_BAD_FILE_FORMAT = 1002,
+    SPONGE_AMD_ERROR_CONFLICTING_COMMAND = 1003,
+    SPONGE_AMD_ERROR_MISSING_COMMAND = 1004,
+    SPONGE_AMD_ERROR_TYPE_ERROR_COMMAND = 1005,
+    SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND = 1006,
+    SPONGE_AMD_ERROR_SIMULATION_BREAKDOWN = 1007,
+    SPONGE_AMD_ERROR_MALLOC_FAILED = 1008,
+    SPONGE_AMD_ERROR_OVERFLOW = 1009,
+    SPONGE_AMD_ERROR_OPEN_FILE_FAILED = 1010
+};
+
+/* Update modes of NEIGHBOR_LIST::Update (neighbor_list.h:112-116). */
+enum sponge_amd_nl_update { SPONGE_AMD_NL_CONDITIONAL = 0, SPONGE_AMD_NL_FORCED = 1 };
+
+/* Overflow flags reported by sponge_nl_check_overflow. */
+enum sponge_amd_nl_flag {
+    SPONGE_AMD_NL_GRID_OVERFLOW = 1, /* a cell held more than max_atom_in_grid_numbers */
+    SPONGE_AMD_NL_LIST_OVERFLOW = 2  /* an atom had more than max_neighbor_numbers */
+};
+
+typedef struct sponge_nl sponge_nl;
+typedef struct sponge_lj sponge_lj;
+typedef struct sponge_pme sponge_pme;
+
+/* ------------------------------------------------------------------------- */
+/* Library                                                                   */
+/* ------------------------------------------------------------------------- */
+
+/* Version string of the library. */
+const char* sponge_amd_version(void);
+
+/* Ewald splitting parameter: bisection on erfc(beta*cutoff)/cutoff = tolerance
+ * (Get_Beta, SPONGE/PM_force/PM_force.cpp:246-274). Host only. */
+float sponge_amd_get_beta(float cutoff, float tolerance);
+
+/* Automatic FFT dimension for a box length in grid units
+ * (Get_Fft_Patameter, SPONGE/common.cpp:639-659). Host only. */
+int sponge_amd_get_fft_parameter(float length);
+
+/* Per-kernel event timers (no counterpart call in the reference, whose
+ * TIME_RECORDER, SPONGE/utils/control/time_recorder.hpp:10-42, times host
+ * regions): when enabled, HIP events are recorded on the launching stream
+ * around the hot kernels ("lj_pairs", "nl_search", "pme_spread",
+ * "pme_gather", "pme_fft_z_r2c", "pme_fft_y", "pme_fft_x_conv",
+ * "pme_ifft_y", "pme_fft_z_c2r").  sponge_amd_kernel_timing synchronises
+ * with the recorded events and returns the accumulated milliseconds and
+ * launch count of one timer. */
+int sponge_amd_set_kernel_timing(int enable);
+int sponge_amd_kernel_timing(const char* name, double* total_ms, long long* launches);
+int sponge_amd_reset_kernel_timing(void);
+
+/* ------------------------------------------------------------------------- */
+/* Neighbor list: NEIGHBOR_LIST (SPONGE/neighbor_list/neighbor_list.h:7-127)  */
+/* ------------------------------------------------------------------------- */
+
+/* The [neighbor_list] mdin keys read by NEIGHBOR_LIST::Initial
+ * (neighbor_list.cpp:271-300, 871-972). */
+typedef struct sponge_nl_options {
+    int refresh_interval;         /* 0: rebuild when any atom moved > skin*skin_permit */
+    float skin_permit;            /* 0.5 */
+    int max_neighbor_numbers;     /* 1200 */
+    int max_atom_in_grid_numbers; /* 150 (reported as a flag; never drops atoms) */
+    int throw_error_when_overflow;/* 0 */
+    int check_overflow_interval;  /* 150 */
+} sponge_nl_options;
+
+void sponge_nl_default_options(sponge_nl_options* opt);
+
+/* NEIGHBOR_LIST::Initial(controller, atom_numbers, cutoff, skin, boundary)
+ * (neighbor_list.cpp:871-972).  Fails with VALUE_ERROR_COMMAND when the cell
+ * grid has 2 or fewer cells along an axis (the reference's "box is too
+ * small", :949-954, raised there as MallocFailed). */
+int sponge_nl_initial(sponge_nl** out, int atom_numbers, float cutoff, float skin,
+                      const float* cell, const float* rcell, const sponge_nl_options* opt,
+                      void* stream);
+
+/* NEIGHBOR_LIST::Update(atom_local, local_atom_numbers, ghost_numbers, crd,
+ * boundary, step, update, excluded_list_start, excluded_list,
+ * excluded_numbers) (neighbor_list.cpp:974-1025).  atom_local maps local to
+ * global ids (NULL = identity); the exclusion CSR is indexed by global id and
+ * holds, per atom, ascending global ids j > i.  One GPU holds no ghost atoms:
+ * ghost_numbers must be 0. */
+int sponge_nl_update(sponge_nl* nl, const int* atom_local, int local_atom_numbers,
+                     int ghost_numbers, const float* crd, const float* cell, const float* rcell,
+                     int step, int update, const int* excluded_list_start,
+                     const int* excluded_list, const int* excluded_numbers);
+
+/* NEIGHBOR_LIST::Check_Overflow (neighbor_list.cpp:1027-1103), reporting
+ * instead of re-initialising: synchronises the stream, returns the sticky
+ * flags (sponge_amd_nl_flag bits) and clears them.  Returns OVERFLOW when a
+ * flag was set and throw_error_when_overflow is on. */
+int sponge_nl_check_overflow(sponge_nl* nl, int* flags);
+
+/* The half list in the reference's representation: for local atom i,
+ * counts[i] neighbours in serial[i*max_numbers ...] (local ids, j > i by
+ * global id, exclusions removed), as ATOM_GROUP d_nl holds it
+ * (SPONGE/common.h:177-182).  Device arrays; entry order within an atom is
+ * unspecified, as in the reference. */
+int sponge_nl_export_half_list(sponge_nl* nl, int* counts, int* serial, int max_numbers);
+
+/* Fills and returns (in *d_nl) a device array of ATOM_GROUP {int atom_numbers;
+ * int ghost_numbers; int* atom_serial;} for modules that read
+ * NEIGHBOR_LIST::d_nl directly.  The storage is owned by the list. */
+int sponge_nl_atom_group(sponge_nl* nl, void** d_nl);
+
+/* Statistics of the current list: entries and the cell grid dims. */
+int sponge_nl_info(sponge_nl* nl, long long* list_entries, int* grid_dims);
+
+/* Number of list builds since Initial (synchronises). */
+int sponge_nl_rebuild_count(sponge_nl* nl, long long* count);
+
+/* NEIGHBOR_LIST::Clear (neighbor_list.cpp:1105-1119). */
+int sponge_nl_clear(sponge_nl* nl);
+
+/* ------------------------------------------------------------------------- */
+/* LJ + PME direct: LENNARD_JONES_INFORMATION (Lennard_Jones_force.h:87-173)  */
+/* ------------------------------------------------------------------------- */
+
+/* LENNARD_JONES_INFORMATION::Initial (Lennard_Jones_force.cpp:147-250).
+ * LJ_A / LJ_B are the in-memory tables (file values x12 and x6, lj.hpp:47,57)
+ * of size T(T+1)/2; atom_LJ_type has atom_numbers entries.  Host arrays. */
+int sponge_lj_initial(sponge_lj** out, int atom_numbers, int atom_type_numbers,
+                      const float* LJ_A, const float* LJ_B, const int* atom_LJ_type, float cutoff,
+                      void* stream);
+
+/* long_range_factor of Initial (Lennard_Jones_force.cpp:208-235). */
+int sponge_lj_long_range_factor(sponge_lj* lj, float* factor);
+
+/* LJ_PME_Direct_Force_With_Atom_Energy_And_Virial (Lennard_Jones_force.cpp:
+ * 326-384) over the list held by nl.  frc, atom_energy, atom_virial are
+ * accumulated; atom_direct_pme_energy (PM's d_direct_atom_energy) and the
+ * module's own LJ energy array are overwritten when need_atom_energy.
+ * solvent_numbers must be 0 (the NVE configurations run without the
+ * solvent_LJ specialisation). */
+int sponge_lj_pme_direct_force(sponge_lj* lj, sponge_nl* nl, int atom_numbers,
+                               int local_atom_numbers, int solvent_numbers, int ghost_numbers,
+                               const float* crd, const float* charge, float* frc,
+                               const float* cell, const float* rcell, float pme_beta,
+                               int need_atom_energy, float* atom_energy, int need_virial,
+                               float* atom_virial, float* atom_direct_pme_energy);
+
+/* The same force call over a caller-provided half list in the reference's
+ * representation (counts[N], serial[N*max_numbers] of local ids), so the pair
+ * kernel can be checked on exactly the reference's pairs. */
+int sponge_lj_pme_direct_force_list(sponge_lj* lj, const int* counts, const int* serial,
+                                    int max_numbers, int atom_numbers, const float* crd,
+                                    const float* charge, float* frc, const float* cell,
+                                    const float* rcell, float pme_beta, int need_atom_energy,
+                                    float* atom_energy, int need_virial, float* atom_virial,
+                                    float* atom_direct_pme_energy);
+
+/* Long_Range_Correction (Lennard_Jones_force.cpp:287-308): d_potential[0] +=
+ * factor/volume; diagonal of d_virial[0] += 2*factor/volume. */
+int sponge_lj_long_range_correction(sponge_lj* lj, int need_pressure, float* d_virial,
+                                    int need_potential, float* d_potential, float volume);
+
+/* Step_Print (Lennard_Jones_force.cpp:386-400): synchronises; LJ_short is the
+ * sum of the per-atom LJ energies (double accumulation), LJ_long the last
+ * long-range term added. */
+int sponge_lj_step_print(sponge_lj* lj, float* LJ_short, float* LJ_long);
+
+/* Device pointer of the per-atom LJ energy array (d_LJ_energy_atom). */
+float* sponge_lj_atom_energy(sponge_lj* lj);
+
+int sponge_lj_clear(sponge_lj* lj);
+
+/* ------------------------------------------------------------------------- */
+/* PME: Particle_Mesh (SPONGE/PM_force/PM_force.h:152-209)                    */
+/* ------------------------------------------------------------------------- */
+
+/* The [PM] / [PME] mdin keys read by Particle_Mesh::Initial
+ * (PM_force.cpp:315-502). fft* <= 0 selects the automatic size. */
+typedef struct sponge_pme_options {
+    int fftx, ffty, fftz;          /* -1 */
+    float grid_spacing;            /* 1.0 Angstrom */
+    float direct_tolerance;        /* 1e-5 */
+    int update_interval;           /* 1 */
+    int calculate_reciprocal_part; /* 1 */
+    int calculate_excluded_part;   /* 1 */
+} sponge_pme_options;
+
+void sponge_pme_default_options(sponge_pme_options* opt);
+
+/* Particle_Mesh::Initial(controller, atom_numbers, boundary, box_length,
+ * cutoff, ...) (PM_force.cpp:295-654), orthorhombic boxes. FFT sizes must be
+ * 2,3,5,7-smooth and at most 512. */
+int sponge_pme_initial(sponge_pme** out, int atom_numbers, const float* cell, const float* rcell,
+                       float cutoff, const sponge_pme_options* opt, void* stream);
+
+/* beta and the grid: out_beta, out_fft[3]. */
+int sponge_pme_info(sponge_pme* pme, float* beta, int* fft);
+
+/* Device pointer of d_direct_atom_energy (written by the LJ direct call). */
+float* sponge_pme_direct_atom_energy(sponge_pme* pme);
+
+/* PME_Reciprocal_Force_With_Energy_And_Virial(crd, boundary, charge, force,
+ * need_virial, need_energy, d_virial, d_potential, step)
+ * (PM_force.cpp:1070-1168): on steps with step % update_interval == 0, frc +=
+ * update_interval * F_reciprocal; when need_energy, d_potential[0] +=
+ * E_reciprocal + E_self + E_neutralizing; when need_virial, d_virial[0] +=
+ * the reciprocal virial. */
+int sponge_pme_reciprocal_force(sponge_pme* pme, const float* crd, const float* cell,
+                                const float* rcell, const float* charge, float* frc,
+                                int need_virial, int need_energy, float* d_virial,
+                                float* d_potential, int step);
+
+/* MPI_PME_Excluded_Force_With_Atom_Energy(local_atom_numbers, atom_local,
+ * atom_local_id, crd, boundary, charge, excluded_list_start, excluded_list,
+ * excluded_numbers, frc, need_energy, atom_energy, need_virial, atom_virial)
+ * (PM_force.cpp:1745-1842); the exclusion CSR is indexed by local id. */
+int sponge_pme_excluded_force(sponge_pme* pme, int local_atom_numbers, const int* atom_local,
+                              const int* atom_local_id, const float* crd, const float* cell,
+                              const float* rcell, const float* charge,
+                              const int* excluded_list_start, const int* excluded_list,
+                              const int* excluded_numbers, float* frc, int need_energy,
+                              float* atom_energy, int need_virial, float* atom_virial);
+
+/* Step_Print (PM_force.cpp:1844-1869): synchronises; PM = direct + reciprocal
+ * + self + correction of the last energy step. */
+int sponge_pme_step_print(sponge_pme* pme, float* PM, float* direct, float* reciprocal,
+                          float* self_energy, float* correction);
+
+/* The real-space grids of the last reciprocal step, [fftx][ffty][fftz]
+ * (z fastest): Q (spread charges) and FBCFQ (convolved potential).  Device
+ * arrays of fftx*ffty*fftz floats; either may be NULL. */
+int sponge_pme_grids(sponge_pme* pme, float* Q, float* FBCFQ);
+
+int sponge_pme_clear(sponge_pme* pme);
+
+/* ------------------------------------------------------------------------- */
+/* NVE step driver (the caller side of the path)                              */
+/* ------------------------------------------------------------------------- */
+
+/* One periodic orthorhombic system of rigid 3-site waters (+ ions) owned on
+ * the device, stepped as SPONGE's main loop steps it in NVE mode with SETTLE:
+ * Main_Calculate_Force (SPONGE/main.cpp:1355-1626: reset, neighbor_list
+ * CONDITIONAL update, PME excluded correction, LJ + PME direct, long-range
+ * correction on energy steps, PME reciprocal) followed by Main_Iteration
+ * (:1690-1777: remember, Leap_Frog, Do_SETTLE).  All arrays are HOST arrays
+ * copied at set-up; water_atoms holds (O, H1, H2) per water, whose rigid
+ * geometry is taken from the first water's coordinates.  dt_ps is in ps. */
+typedef struct sponge_md sponge_md;
+
+int sponge_md_initial(sponge_md** out, int atom_numbers, const float* crd, const float* vel,
+                      const float* mass, const float* charge, int atom_type_numbers,
+                      const float* LJ_A, const float* LJ_B, const int* atom_LJ_type,
+                      const int* excluded_list_start, const int* excluded_numbers,
+                      const int* excluded_list, int n_water, const int* water_atoms,
+                      const float* box_length, float cutoff, float skin, float dt_ps,
+                      const sponge_nl_options* nl_opt, const sponge_pme_options* pme_opt,
+                      void* stream);
+
+/* Advances `steps` steps asynchronously.  Energies are evaluated on steps
+ * with step % energy_interval == 0 (0 = never).  use_graph != 0 captures the
+ * force-only step into one HIP graph on first use and replays it (default
+ * schedule only: automatic list refresh, reciprocal part every step). */
+int sponge_md_run(sponge_md* md, int steps, int energy_interval, int use_graph);
+
+/* Synchronises and reports out[10] = {potential, kinetic, LJ_short, LJ_long,
+ * PM, PM_direct, PM_reciprocal, PM_self, PM_correction, sum of per-atom
+ * energies} of the last energy step (kinetic from the current velocities). */
+int sponge_md_energies(sponge_md* md, double* out);
+
+/* Copies the current coordinates, velocities and last forces to host arrays
+ * of 3*atom_numbers floats (any may be NULL); synchronises. */
+int sponge_md_get_state(sponge_md* md, float* crd, float* vel, float* frc);
+
+/* The module handles the driver owns (for timing and list export). */
+int sponge_md_handles(sponge_md* md, sponge_nl** nl, sponge_lj** lj, sponge_pme** pme);
+
+int sponge_md_clear(sponge_md* md);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPONGE_AMD_H */

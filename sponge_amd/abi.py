@@ -1,0 +1,409 @@
+"""ctypes binding of include/sponge_amd.h plus thin Python mirrors of the
+reference's module classes (NEIGHBOR_LIST, LENNARD_JONES_INFORMATION,
+Particle_Mesh) over device tensors.
+
+Device memory and streams come from PyTorch; every computation runs in the
+HIP library sponge_amd/libsponge_amd.so.  Importing fails loudly when the
+library has not been built: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsponge_amd.so")
+
+SUCCESS = 0
+ERROR_NAMES = {
+    1001: "NotImplemented", 1002: "BadFileFormat", 1003: "ConflictingCommand",
+    1004: "MissingCommand", 1005: "TypeErrorCommand", 1006: "ValueErrorCommand",
+    1007: "SimulationBreakDown", 1008: "MallocFailed", 1009: "Overflow", 1010: "OpenFileFailed",
+}
+NL_CONDITIONAL, NL_FORCED = 0, 1
+
+
+class SpongeError(RuntimeError):
+    def __init__(self, code, where):
+        name = ERROR_NAMES.get(code, "hipError" if code < 1000 else "unknown")
+        super().__init__(f"{where} failed with {code} ({name})")
+        self.code = code
+
+
+def _check(rc, where):
+    if rc != 0:
+        raise SpongeError(rc, where)
+
+
+class NLOptions(C.Structure):
+    _fields_ = [("refresh_interval", C.c_int), ("skin_permit", C.c_float),
+                ("max_neighbor_numbers", C.c_int), ("max_atom_in_grid_numbers", C.c_int),
+                ("throw_error_when_overflow", C.c_int), ("check_overflow_interval", C.c_int)]
+
+
+class PMEOptions(C.Structure):
+    _fields_ = [("fftx", C.c_int), ("ffty", C.c_int), ("fftz", C.c_int),
+                ("grid_spacing", C.c_float), ("direct_tolerance", C.c_float),
+                ("update_interval", C.c_int), ("calculate_reciprocal_part", C.c_int),
+                ("calculate_excluded_part", C.c_int)]
+
+
+_P = C.c_void_p
+_F = C.c_float
+_I = C.c_int
+_FP = C.POINTER(C.c_float)
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "sponge_amd_version": (C.c_char_p, []),
+    "sponge_amd_get_beta": (_F, [_F, _F]),
+    "sponge_amd_get_fft_parameter": (_I, [_F]),
+    "sponge_amd_set_kernel_timing": (_I, [_I]),
+    "sponge_amd_kernel_timing": (_I, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
+    "sponge_amd_reset_kernel_timing": (_I, []),
+    "sponge_nl_default_options": (None, [C.POINTER(NLOptions)]),
+    "sponge_nl_initial": (_I, [C.POINTER(_P), _I, _F, _F, _FP, _FP, C.POINTER(NLOptions), _P]),
+    "sponge_nl_update": (_I, [_P, _P, _I, _I, _P, _FP, _FP, _I, _I, _P, _P, _P]),
+    "sponge_nl_check_overflow": (_I, [_P, C.POINTER(_I)]),
+    "sponge_nl_export_half_list": (_I, [_P, _P, _P, _I]),
+    "sponge_nl_atom_group": (_I, [_P, C.POINTER(_P)]),
+    "sponge_nl_info": (_I, [_P, C.POINTER(C.c_longlong), C.POINTER(_I)]),
+    "sponge_nl_rebuild_count": (_I, [_P, C.POINTER(C.c_longlong)]),
+    "sponge_nl_clear": (_I, [_P]),
+    "sponge_lj_initial": (_I, [C.POINTER(_P), _I, _I, _FP, _FP, C.POINTER(_I), _F, _P]),
+    "sponge_lj_long_range_factor": (_I, [_P, _FP]),
+    "sponge_lj_pme_direct_force": (_I, [_P, _P, _I, _I, _I, _I, _P, _P, _P, _FP, _FP, _F, _I, _P,
+                                        _I, _P, _P]),
+    "sponge_lj_pme_direct_force_list": (_I, [_P, _P, _P, _I, _I, _P, _P, _P, _FP, _FP, _F, _I, _P,
+                                             _I, _P, _P]),
+    "sponge_lj_long_range_correction": (_I, [_P, _I, _P, _I, _P, _F]),
+    "sponge_lj_step_print": (_I, [_P, _FP, _FP]),
+    "sponge_lj_atom_energy": (_P, [_P]),
+    "sponge_lj_clear": (_I, [_P]),
+    "sponge_pme_default_options": (None, [C.POINTER(PMEOptions)]),
+    "sponge_pme_initial": (_I, [C.POINTER(_P), _I, _FP, _FP, _F, C.POINTER(PMEOptions), _P]),
+    "sponge_pme_info": (_I, [_P, _FP, C.POINTER(_I)]),
+    "sponge_pme_direct_atom_energy": (_P, [_P]),
+    "sponge_pme_reciprocal_force": (_I, [_P, _P, _FP, _FP, _P, _P, _I, _I, _P, _P, _I]),
+    "sponge_pme_excluded_force": (_I, [_P, _I, _P, _P, _P, _FP, _FP, _P, _P, _P, _P, _P, _I, _P,
+                                       _I, _P]),
+    "sponge_pme_step_print": (_I, [_P, _FP, _FP, _FP, _FP, _FP]),
+    "sponge_pme_grids": (_I, [_P, _P, _P]),
+    "sponge_pme_clear": (_I, [_P]),
+    "sponge_md_initial": (_I, [C.POINTER(_P), _I, _FP, _FP, _FP, _FP, _I, _FP, _FP, C.POINTER(_I),
+                               C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), _I, C.POINTER(_I),
+                               _FP, _F, _F, _F, C.POINTER(NLOptions), C.POINTER(PMEOptions), _P]),
+    "sponge_md_run": (_I, [_P, _I, _I, _I]),
+    "sponge_md_energies": (_I, [_P, C.POINTER(C.c_double)]),
+    "sponge_md_get_state": (_I, [_P, _FP, _FP, _FP]),
+    "sponge_md_handles": (_I, [_P, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),
+    "sponge_md_clear": (_I, [_P]),
+}
+
+_lib = None
+
+
+def load():
+    """Load the HIP library (raises when it is missing or lacks a symbol)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m sponge_amd.build` "
+                           "(there is no CPU fallback)")
+    # PyTorch-ROCm ships its own HIP runtime with the same soname; loading it
+    # first makes this library bind to that one runtime instead of pulling a
+    # second copy from /opt/rocm into the process.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)  # AttributeError when a declared symbol is not exported
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def set_kernel_timing(enable):
+    _check(load().sponge_amd_set_kernel_timing(int(enable)), "set_kernel_timing")
+
+
+def kernel_timing(name):
+    """(total milliseconds, launches) of one kernel timer (synchronises)."""
+    ms = C.c_double()
+    n = C.c_longlong()
+    _check(load().sponge_amd_kernel_timing(name.encode(), C.byref(ms), C.byref(n)), "kernel_timing")
+    return ms.value, n.value
+
+
+def reset_kernel_timing():
+    _check(load().sponge_amd_reset_kernel_timing(), "reset_kernel_timing")
+
+
+def _fp(a):
+    a = np.ascontiguousarray(a, np.float32)
+    return a, a.ctypes.data_as(_FP)
+
+
+def _ptr(t):
+    """Device pointer of a torch tensor (or None)."""
+    if t is None:
+        return None
+    return C.c_void_p(t.data_ptr())
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        return None
+    return C.c_void_p(stream.cuda_stream)
+
+
+class Box:
+    """Boundary of the reference (cell and rcell LTMatrix3) for an
+    orthorhombic box, rcell inverted in float32 as
+    Invert_Lower_Triangular_Cell (SPONGE/utils/boundary.h:59-70)."""
+
+    def __init__(self, lengths):
+        L = np.asarray(lengths, np.float32)
+        one = np.float32(1.0)
+        self.cell = np.array([L[0], 0, L[1], 0, 0, L[2]], np.float32)
+        self.rcell = np.array([one / L[0], 0, one / L[1], 0, 0, one / L[2]], np.float32)
+        self.volume = float(np.float32(L[0] * L[1]) * L[2])
+
+    def args(self):
+        return self.cell.ctypes.data_as(_FP), self.rcell.ctypes.data_as(_FP)
+
+
+class NeighborList:
+    """NEIGHBOR_LIST (SPONGE/neighbor_list/neighbor_list.h:7-127)."""
+
+    def __init__(self, atom_numbers, cutoff, skin, box: Box, stream=None, **opts):
+        self.lib = load()
+        o = NLOptions()
+        self.lib.sponge_nl_default_options(C.byref(o))
+        for k, v in opts.items():
+            setattr(o, k, v)
+        self.opt = o
+        h = C.c_void_p()
+        _check(self.lib.sponge_nl_initial(C.byref(h), int(atom_numbers), float(cutoff), float(skin),
+                                          *box.args(), C.byref(o), _stream_ptr(stream)),
+               "sponge_nl_initial")
+        self.h = h
+        self.N = int(atom_numbers)
+        self.box = box
+
+    def update(self, crd, step=0, update=NL_CONDITIONAL, excl=None, atom_local=None):
+        es, el, en = excl if excl is not None else (None, None, None)
+        _check(self.lib.sponge_nl_update(self.h, _ptr(atom_local), self.N, 0, _ptr(crd),
+                                         *self.box.args(), int(step), int(update), _ptr(es),
+                                         _ptr(el), _ptr(en)), "sponge_nl_update")
+
+    def check_overflow(self):
+        f = C.c_int()
+        rc = self.lib.sponge_nl_check_overflow(self.h, C.byref(f))
+        return rc, f.value
+
+    def export_half_list(self, counts, serial, max_numbers):
+        _check(self.lib.sponge_nl_export_half_list(self.h, _ptr(counts), _ptr(serial),
+                                                   int(max_numbers)), "sponge_nl_export_half_list")
+
+    def info(self):
+        n = C.c_longlong()
+        d = (C.c_int * 3)()
+        _check(self.lib.sponge_nl_info(self.h, C.byref(n), d), "sponge_nl_info")
+        return n.value, tuple(d)
+
+    def clear(self):
+        if self.h:
+            self.lib.sponge_nl_clear(self.h)
+            self.h = None
+
+    __del__ = clear
+
+
+class LennardJones:
+    """LENNARD_JONES_INFORMATION (Lennard_Jones_force.h:87-173)."""
+
+    def __init__(self, atom_numbers, type_numbers, A, B, types, cutoff, stream=None):
+        self.lib = load()
+        A, pA = _fp(A)
+        B, pB = _fp(B)
+        t = np.ascontiguousarray(types, np.int32)
+        h = C.c_void_p()
+        _check(self.lib.sponge_lj_initial(C.byref(h), int(atom_numbers), int(type_numbers), pA, pB,
+                                          t.ctypes.data_as(C.POINTER(C.c_int)), float(cutoff),
+                                          _stream_ptr(stream)), "sponge_lj_initial")
+        self.h = h
+        self.N = int(atom_numbers)
+
+    def long_range_factor(self):
+        f = C.c_float()
+        _check(self.lib.sponge_lj_long_range_factor(self.h, C.byref(f)), "long_range_factor")
+        return f.value
+
+    def direct_force(self, nl, crd, charge, frc, box, beta, atom_energy=None, atom_virial=None,
+                     direct_energy=None):
+        _check(self.lib.sponge_lj_pme_direct_force(
+            self.h, nl.h, self.N, self.N, 0, 0, _ptr(crd), _ptr(charge), _ptr(frc), *box.args(),
+            float(beta), int(atom_energy is not None), _ptr(atom_energy),
+            int(atom_virial is not None), _ptr(atom_virial), _ptr(direct_energy)),
+            "sponge_lj_pme_direct_force")
+
+    def direct_force_list(self, counts, serial, max_numbers, crd, charge, frc, box, beta,
+                          atom_energy=None, atom_virial=None, direct_energy=None):
+        _check(self.lib.sponge_lj_pme_direct_force_list(
+            self.h, _ptr(counts), _ptr(serial), int(max_numbers), self.N, _ptr(crd), _ptr(charge),
+            _ptr(frc), *box.args(), float(beta), int(atom_energy is not None), _ptr(atom_energy),
+            int(atom_virial is not None), _ptr(atom_virial), _ptr(direct_energy)),
+            "sponge_lj_pme_direct_force_list")
+
+    def long_range_correction(self, need_pressure, virial, need_potential, potential, volume):
+        _check(self.lib.sponge_lj_long_range_correction(self.h, int(need_pressure), _ptr(virial),
+                                                        int(need_potential), _ptr(potential),
+                                                        float(volume)), "long_range_correction")
+
+    def step_print(self):
+        a, b = C.c_float(), C.c_float()
+        _check(self.lib.sponge_lj_step_print(self.h, C.byref(a), C.byref(b)), "lj step_print")
+        return a.value, b.value
+
+    def atom_energy_ptr(self):
+        return self.lib.sponge_lj_atom_energy(self.h)
+
+    def clear(self):
+        if self.h:
+            self.lib.sponge_lj_clear(self.h)
+            self.h = None
+
+    __del__ = clear
+
+
+class ParticleMesh:
+    """Particle_Mesh (SPONGE/PM_force/PM_force.h:152-209)."""
+
+    def __init__(self, atom_numbers, box: Box, cutoff, stream=None, **opts):
+        self.lib = load()
+        o = PMEOptions()
+        self.lib.sponge_pme_default_options(C.byref(o))
+        for k, v in opts.items():
+            setattr(o, k, v)
+        h = C.c_void_p()
+        _check(self.lib.sponge_pme_initial(C.byref(h), int(atom_numbers), *box.args(),
+                                           float(cutoff), C.byref(o), _stream_ptr(stream)),
+               "sponge_pme_initial")
+        self.h = h
+        self.N = int(atom_numbers)
+        self.box = box
+        b = C.c_float()
+        f = (C.c_int * 3)()
+        _check(self.lib.sponge_pme_info(self.h, C.byref(b), f), "sponge_pme_info")
+        self.beta = b.value
+        self.fft = tuple(f)
+
+    def direct_atom_energy_ptr(self):
+        return self.lib.sponge_pme_direct_atom_energy(self.h)
+
+    def reciprocal_force(self, crd, charge, frc, step=0, virial=None, potential=None):
+        _check(self.lib.sponge_pme_reciprocal_force(
+            self.h, _ptr(crd), *self.box.args(), _ptr(charge), _ptr(frc), int(virial is not None),
+            int(potential is not None), _ptr(virial), _ptr(potential), int(step)),
+            "sponge_pme_reciprocal_force")
+
+    def excluded_force(self, crd, charge, excl, frc, atom_energy=None, atom_virial=None):
+        es, el, en = excl
+        _check(self.lib.sponge_pme_excluded_force(
+            self.h, self.N, None, None, _ptr(crd), *self.box.args(), _ptr(charge), _ptr(es),
+            _ptr(el), _ptr(en), _ptr(frc), int(atom_energy is not None), _ptr(atom_energy),
+            int(atom_virial is not None), _ptr(atom_virial)), "sponge_pme_excluded_force")
+
+    def step_print(self):
+        v = [C.c_float() for _ in range(5)]
+        _check(self.lib.sponge_pme_step_print(self.h, *[C.byref(x) for x in v]), "pme step_print")
+        return dict(PM=v[0].value, direct=v[1].value, reciprocal=v[2].value,
+                    self_energy=v[3].value, correction=v[4].value)
+
+    def grids(self, Q=None, FBCFQ=None):
+        _check(self.lib.sponge_pme_grids(self.h, _ptr(Q), _ptr(FBCFQ)), "sponge_pme_grids")
+
+    def clear(self):
+        if self.h:
+            self.lib.sponge_pme_clear(self.h)
+            self.h = None
+
+    __del__ = clear
+
+
+class MDSystem:
+    """NVE step driver over a System (sponge_amd.system.System)."""
+
+    ENERGY_KEYS = ("potential", "kinetic", "LJ_short", "LJ_long", "PM", "PM_direct",
+                   "PM_reciprocal", "PM_self", "PM_correction", "atom_energy_sum")
+
+    def __init__(self, s, stream=None, nl_opts=None, pme_opts=None):
+        self.lib = load()
+        self.s = s
+        N = s.n_atoms
+        no = NLOptions()
+        self.lib.sponge_nl_default_options(C.byref(no))
+        for k, v in (nl_opts or {}).items():
+            setattr(no, k, v)
+        po = PMEOptions()
+        self.lib.sponge_pme_default_options(C.byref(po))
+        if s.fft is not None:
+            po.fftx, po.ffty, po.fftz = s.fft
+        po.direct_tolerance = s.tolerance
+        for k, v in (pme_opts or {}).items():
+            setattr(po, k, v)
+        ip = C.POINTER(C.c_int)
+        arr = lambda a, t=np.float32: np.ascontiguousarray(a, t)
+        self._keep = [arr(s.crd), arr(s.vel), arr(s.mass), arr(s.charge), arr(s.lj_A), arr(s.lj_B),
+                      arr(s.lj_type, np.int32), arr(s.excl_start, np.int32),
+                      arr(s.excl_num, np.int32),
+                      arr(s.excl_list if len(s.excl_list) else np.zeros(1), np.int32),
+                      arr(np.arange(3 * s.n_waters), np.int32), arr(s.box)]
+        k = self._keep
+        h = C.c_void_p()
+        _check(self.lib.sponge_md_initial(
+            C.byref(h), N, k[0].ctypes.data_as(_FP), k[1].ctypes.data_as(_FP),
+            k[2].ctypes.data_as(_FP), k[3].ctypes.data_as(_FP), s.n_types,
+            k[4].ctypes.data_as(_FP), k[5].ctypes.data_as(_FP), k[6].ctypes.data_as(ip),
+            k[7].ctypes.data_as(ip), k[8].ctypes.data_as(ip), k[9].ctypes.data_as(ip), s.n_waters,
+            k[10].ctypes.data_as(ip), k[11].ctypes.data_as(_FP), float(s.cutoff), float(s.skin),
+            float(s.dt_ps), C.byref(no), C.byref(po), _stream_ptr(stream)), "sponge_md_initial")
+        self.h = h
+        self.N = N
+
+    def run(self, steps, energy_interval=0, use_graph=False):
+        _check(self.lib.sponge_md_run(self.h, int(steps), int(energy_interval), int(use_graph)),
+               "sponge_md_run")
+
+    def energies(self):
+        out = (C.c_double * 10)()
+        _check(self.lib.sponge_md_energies(self.h, out), "sponge_md_energies")
+        return dict(zip(self.ENERGY_KEYS, list(out)))
+
+    def state(self):
+        crd = np.zeros((self.N, 3), np.float32)
+        vel = np.zeros((self.N, 3), np.float32)
+        frc = np.zeros((self.N, 3), np.float32)
+        _check(self.lib.sponge_md_get_state(self.h, crd.ctypes.data_as(_FP), vel.ctypes.data_as(_FP),
+                                            frc.ctypes.data_as(_FP)), "sponge_md_get_state")
+        return crd, vel, frc
+
+    def handles(self):
+        nl, lj, pm = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        _check(self.lib.sponge_md_handles(self.h, C.byref(nl), C.byref(lj), C.byref(pm)),
+               "sponge_md_handles")
+        return nl, lj, pm
+
+    def clear(self):
+        if self.h:
+            self.lib.sponge_md_clear(self.h)
+            self.h = None
+
+    __del__ = clear

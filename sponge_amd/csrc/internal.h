@@ -1,0 +1,101 @@
+// Module state behind the opaque handles of include/sponge_amd.h.
+#pragma once
+
+#include <vector>
+
+#include "common.h"
+
+// ATOM_GROUP of the reference (SPONGE/common.h:177-182).
+struct sa_atom_group {
+    int atom_numbers;
+    int ghost_numbers;
+    int* atom_serial;
+};
+
+// Neighbor list.  Atoms are counting-sorted by cell (x-major, z fastest) on
+// every rebuild; the list holds, per sorted atom p, the sorted indices q of
+// its half-list neighbours (global id of q > global id of p).
+struct sponge_nl {
+    int N = 0;
+    float cutoff = 0, skin = 0, grid_length = 0, thr = 0;  // thr = gl*gl*4 (cutoff_skin_square)
+    int Nx = 0, Ny = 0, Nz = 0, G = 0;
+    int ortho = 1;
+    sa::LT6 cell0{}, rcell0{};  // box the cell stencil was built for
+    sponge_nl_options opt{};
+    hipStream_t stream = nullptr;
+
+    int* d_need = nullptr;        // rebuild flag of this step
+    int* d_flags = nullptr;       // sticky overflow flags
+    unsigned long long* d_rebuilds = nullptr;  // number of list builds
+    int* d_cell_count = nullptr;  // [G]
+    int* d_cell_start = nullptr;  // [G+1]
+    int* d_atom_cell = nullptr;   // [N]
+    int* d_atom_slot = nullptr;   // [N]
+    int* d_sidx = nullptr;        // sorted -> local
+    int* d_rank = nullptr;        // local -> sorted
+    float4* d_spos = nullptr;     // sorted x, y, z at build time, w = global id bits
+    float* d_old_crd = nullptr;   // VECTOR[N] at the last build (auto refresh)
+    int2* d_ranges = nullptr;     // [G][max_ranges] runs of neighbour cells [begin, end)
+    int* d_nranges = nullptr;     // [G]
+    int max_ranges = 0;
+    int* d_list_count = nullptr;  // [N] per sorted atom
+    int* d_list = nullptr;        // [N][max_nb] sorted neighbour indices
+    int* d_exp_serial = nullptr;  // reference-format export storage (lazy)
+    int* d_exp_count = nullptr;
+    sa_atom_group* d_atom_group = nullptr;
+    int built = 0;
+};
+
+struct sponge_lj {
+    int N = 0, T = 0;
+    float cutoff = 10.0f;
+    float rc2_thr = 0;            // r2 < rc2_thr  <=>  sqrtf(r2) < cutoff
+    float long_range_factor = 0;
+    float h_LJ_long_energy = 0;
+    hipStream_t stream = nullptr;
+    std::vector<float> h_A, h_B;
+    std::vector<int> h_type;
+    float2* d_AB = nullptr;       // interleaved (A, B) per pair type
+    int* d_type = nullptr;        // [N] local order
+    float4* d_pos = nullptr;      // [N] packed x, y, z, charge (list index space)
+    int* d_ptype = nullptr;       // [N] LJ type (list index space)
+    float4* d_facc = nullptr;     // [N] force accumulator (list index space)
+    float2* d_eacc = nullptr;     // [N] (E_LJ, E_coulomb) per row
+    float* d_vacc = nullptr;      // [N][6] virial per row
+    float* d_LJ_energy_atom = nullptr;  // [N] local order
+    double* d_sum = nullptr;      // scratch for Step_Print reductions
+};
+
+struct sa_radix_plan {
+    int n;
+    int nst;
+    int r[16];
+};
+
+struct sponge_pme {
+    int N = 0;
+    int fx = 0, fy = 0, fz = 0, fzc = 0;
+    long long nall = 0, nfft = 0;
+    float beta = 0, cutoff = 0, volume = 0, neutralizing_factor = 0;
+    sa::LT6 cell{}, rcell{};
+    sponge_pme_options opt{};
+    hipStream_t stream = nullptr;
+    sa_radix_plan plan[3]{};
+    float2* d_tw[3] = {nullptr, nullptr, nullptr};  // exp(-2 pi i t / n), t < n
+    float* d_Q = nullptr;         // [fx][fy][fz]
+    float2* d_FQ = nullptr;       // [fzc][fx][fy]
+    float* d_FBCFQ = nullptr;     // [fx][fy][fz]
+    float* d_BCt = nullptr;       // [fzc][fy][fx]
+    float* d_vBCt = nullptr;      // 6 planes of [fzc][fy][fx]
+    double* d_acc = nullptr;      // [0] sum w BC |F|^2, [1..6] virial, [7] sum q^2, [8] sum q
+    float* d_ene = nullptr;       // [0] reciprocal, [1] self (incl. neutralizing)
+    float* d_direct_atom_energy = nullptr;
+    float* d_correction_atom_energy = nullptr;
+    double* d_sum = nullptr;
+    int tl_z = 2, tl_y = 2, tl_x = 2;  // lines per block of the FFT passes
+};
+
+namespace sa {
+int nl_pack_for_pairs(sponge_nl* nl, const float* crd, const float* charge, const int* type,
+                      float4* pos, int* ptype, float4* facc, hipStream_t s);
+}

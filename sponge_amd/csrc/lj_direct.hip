@@ -1,0 +1,409 @@
+// LJ + PME direct-space pair forces for gfx950.
+//
+// Reference: SPONGE/Lennard_Jones_force/Lennard_Jones_force.{h,cpp}.
+// Per in-cutoff pair (strict dr_abs < cutoff, Lennard_Jones_force.cpp:74):
+//   F_LJ/r   = (B - A r^-6) r^-8                      (Get_LJ_Force, .h:52-56)
+//   F_C/r    = qq r^-3 (2/sqrt(pi) br e^{-b^2r^2} + erfc(br))   (.h:67-74)
+//   f        = F_LJ/r - F_C/r;  F_i += f dr, F_j -= f dr with dr = r_j - r_i
+//   E_LJ     = (A/12 r^-6 - B/6) r^-6;  E_C = qq erfc(br)/r     (.h:45-50, 62-66)
+//   virial_i -= Get_Virial_From_Force_Dis(f dr, dr)
+// Energies and virial of a pair go to its row atom i (the lower global id),
+// exactly as in the reference's half-list kernel.
+//
+// Layout: the list is in the neighbor list's cell-sorted index space;
+// positions and charges are gathered once per step into a float4 SoA array in
+// that space, forces accumulate into a sorted float4 array (row sums via
+// wavefront shuffles, column updates via hardware float atomics), and a final
+// pass scatters into the caller's AoS arrays.
+#include <vector>
+
+#include "internal.h"
+#include "timing.h"
+
+using namespace sa;
+
+namespace {
+
+constexpr float kTwoOverSqrtPi = 1.1283791670218446f;  // Lennard_Jones_force.h:8
+
+template <bool ORTHO, bool NEED_E, bool NEED_V>
+__global__ void __launch_bounds__(256) k_lj_pairs(
+    int nrows, const int* __restrict__ list_count, const int* __restrict__ list, int max_nb,
+    const float4* __restrict__ pos, const int* __restrict__ ptype,
+    const float2* __restrict__ AB, LT6 c, LT6 r, float rc2_thr, float beta,
+    float4* __restrict__ facc, float2* __restrict__ eacc, float* __restrict__ vacc) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int p = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    if (p >= nrows) return;
+    const float4 pi4 = pos[p];
+    const float3 pi = make_float3(pi4.x, pi4.y, pi4.z);
+    const int ti = ptype[p];
+    const int n = list_count[p];
+    const int* row = list + (size_t)p * max_nb;
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    float elj = 0.f, ec = 0.f;
+    float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f, v5 = 0.f;
+    for (int k = lane; k < n; k += kWave) {
+        const int q = row[k];
+        const float4 pj4 = pos[q];
+        const float3 d = disp_pbc<ORTHO>(make_float3(pj4.x, pj4.y, pj4.z), pi, c, r);
+        const float r2 = norm2_exact(d);
+        if (r2 < rc2_thr) {
+            const float2 ab = AB[lj_pair_index(ti, ptype[q])];
+            const float rabs = sqrtf(r2);
+            const float rinv = 1.0f / rabs;
+            const float r2inv = rinv * rinv;
+            const float r6inv = r2inv * r2inv * r2inv;
+            const float qq = pi4.w * pj4.w;
+            const float br = beta * rabs;
+            const float erfc_br = erfcf(br);
+            const float flj = (ab.y - ab.x * r6inv) * r6inv * r2inv;
+            const float fc = qq * rinv * r2inv * (br * kTwoOverSqrtPi * expf(-br * br) + erfc_br);
+            const float f = flj - fc;
+            const float lx = f * d.x, ly = f * d.y, lz = f * d.z;
+            fx += lx;
+            fy += ly;
+            fz += lz;
+            unsafeAtomicAdd(&facc[q].x, -lx);
+            unsafeAtomicAdd(&facc[q].y, -ly);
+            unsafeAtomicAdd(&facc[q].z, -lz);
+            if (NEED_V) {
+                v0 -= lx * d.x;
+                v1 -= lx * d.y + ly * d.x;
+                v2 -= ly * d.y;
+                v3 -= lx * d.z + lz * d.x;
+                v4 -= ly * d.z + lz * d.y;
+                v5 -= lz * d.z;
+            }
+            if (NEED_E) {
+                elj += (0.083333333f * ab.x * r6inv - 0.166666667f * ab.y) * r6inv;
+                ec += qq * erfc_br * rinv;
+            }
+        }
+    }
+    fx = wave_sum(fx);
+    fy = wave_sum(fy);
+    fz = wave_sum(fz);
+    // energies and virial: a few terms per lane in FP32, the 64 lane
+    // partials summed in FP64
+    double delj = 0, dec = 0, dv0 = 0, dv1 = 0, dv2 = 0, dv3 = 0, dv4 = 0, dv5 = 0;
+    if (NEED_E) {
+        delj = wave_sum((double)elj);
+        dec = wave_sum((double)ec);
+    }
+    if (NEED_V) {
+        dv0 = wave_sum((double)v0);
+        dv1 = wave_sum((double)v1);
+        dv2 = wave_sum((double)v2);
+        dv3 = wave_sum((double)v3);
+        dv4 = wave_sum((double)v4);
+        dv5 = wave_sum((double)v5);
+    }
+    if (lane == 0) {
+        unsafeAtomicAdd(&facc[p].x, fx);
+        unsafeAtomicAdd(&facc[p].y, fy);
+        unsafeAtomicAdd(&facc[p].z, fz);
+        if (NEED_E) eacc[p] = make_float2((float)delj, (float)dec);
+        if (NEED_V) {
+            float* v = vacc + (size_t)6 * p;
+            v[0] = (float)dv0; v[1] = (float)dv1; v[2] = (float)dv2;
+            v[3] = (float)dv3; v[4] = (float)dv4; v[5] = (float)dv5;
+        }
+    }
+}
+
+// Scatter of the per-row accumulators to the caller's arrays: frc +=,
+// atom_energy += E_LJ + E_C, LJ energy and direct-PME energy overwritten
+// (the reference clears them before its kernel, Lennard_Jones_force.cpp:
+// 343-349), atom_virial +=.
+template <bool NEED_E, bool NEED_V>
+__global__ void k_lj_finalize(int N, const int* __restrict__ sidx, const float4* __restrict__ facc,
+                              const float2* __restrict__ eacc, const float* __restrict__ vacc,
+                              float* __restrict__ frc, float* __restrict__ atom_energy,
+                              float* __restrict__ lj_energy, float* __restrict__ direct_energy,
+                              float* __restrict__ atom_virial) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N) return;
+    const int i = sidx ? sidx[p] : p;
+    const float4 f = facc[p];
+    frc[3 * i] += f.x;
+    frc[3 * i + 1] += f.y;
+    frc[3 * i + 2] += f.z;
+    if (NEED_E) {
+        const float2 e = eacc[p];
+        atom_energy[i] += e.x + e.y;
+        if (lj_energy) lj_energy[i] = e.x;
+        if (direct_energy) direct_energy[i] = e.y;
+    }
+    if (NEED_V) {
+        const float* v = vacc + (size_t)6 * p;
+        float* o = atom_virial + (size_t)6 * i;
+        o[0] += v[0]; o[1] += v[1]; o[2] += v[2]; o[3] += v[3]; o[4] += v[4]; o[5] += v[5];
+    }
+}
+
+// Identity-order pack for the reference-format list entry point.
+__global__ void k_lj_pack_local(int N, const float* __restrict__ crd,
+                                const float* __restrict__ charge, const int* __restrict__ type,
+                                float4* __restrict__ pos, int* __restrict__ ptype,
+                                float4* __restrict__ facc) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    pos[i] = make_float4(crd[3 * i], crd[3 * i + 1], crd[3 * i + 2], charge[i]);
+    ptype[i] = type[i];
+    facc[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+__global__ void k_add_scalar(float* p, float v) { p[0] += v; }
+
+__global__ void k_add_diag(float* p, float v) {
+    p[0] += v;
+    p[2] += v;
+    p[5] += v;
+}
+
+// Sum of a float list with double accumulation (Sum_Of_List's double
+// partials, SPONGE/common.cpp:329-392).
+__global__ void k_sum_f32(int n, const float* __restrict__ a, double* out) {
+    double s = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) s += a[i];
+    s = wave_sum(s);
+    if ((threadIdx.x & (kWave - 1)) == 0) unsafeAtomicAdd(out, s);
+}
+
+template <bool ORTHO>
+int launch_pairs(sponge_lj* lj, int nrows, const int* cnt, const int* list, int max_nb,
+                 LT6 c, LT6 r, float beta, int need_e, int need_v, hipStream_t s) {
+    dim3 grid(grid_for(nrows, 4)), block(256);
+    ScopedTimer timer("lj_pairs", s);
+#define SA_LJ_LAUNCH(E, V)                                                                     \
+    hipLaunchKernelGGL((k_lj_pairs<ORTHO, E, V>), grid, block, 0, s, nrows, cnt, list, max_nb, \
+                       lj->d_pos, lj->d_ptype, lj->d_AB, c, r, lj->rc2_thr, beta, lj->d_facc,  \
+                       lj->d_eacc, lj->d_vacc)
+    if (need_e && need_v) SA_LJ_LAUNCH(true, true);
+    else if (need_e) SA_LJ_LAUNCH(true, false);
+    else if (need_v) SA_LJ_LAUNCH(false, true);
+    else SA_LJ_LAUNCH(false, false);
+#undef SA_LJ_LAUNCH
+    SA_LAUNCH_CHECK();
+    return 0;
+}
+
+int finalize(sponge_lj* lj, int N, const int* sidx, int need_e, int need_v, float* frc,
+             float* atom_energy, float* atom_virial, float* direct, hipStream_t s) {
+    dim3 grid(grid_for(N, 256)), block(256);
+#define SA_FIN(E, V)                                                                            \
+    hipLaunchKernelGGL((k_lj_finalize<E, V>), grid, block, 0, s, N, sidx, lj->d_facc,           \
+                       lj->d_eacc, lj->d_vacc, frc, atom_energy, lj->d_LJ_energy_atom, direct, \
+                       atom_virial)
+    if (need_e && need_v) SA_FIN(true, true);
+    else if (need_e) SA_FIN(true, false);
+    else if (need_v) SA_FIN(false, true);
+    else SA_FIN(false, false);
+#undef SA_FIN
+    SA_LAUNCH_CHECK();
+    return 0;
+}
+
+void lj_free(sponge_lj* lj) {
+    dfree(lj->d_AB);
+    dfree(lj->d_type);
+    dfree(lj->d_pos);
+    dfree(lj->d_ptype);
+    dfree(lj->d_facc);
+    dfree(lj->d_eacc);
+    dfree(lj->d_vacc);
+    dfree(lj->d_LJ_energy_atom);
+    dfree(lj->d_sum);
+}
+
+}  // namespace
+
+extern "C" int sponge_lj_initial(sponge_lj** out, int atom_numbers, int atom_type_numbers,
+                                 const float* LJ_A, const float* LJ_B, const int* atom_LJ_type,
+                                 float cutoff, void* stream) {
+    if (!out || atom_numbers <= 0 || atom_type_numbers <= 0 || !LJ_A || !LJ_B || !atom_LJ_type ||
+        !(cutoff > 0))
+        return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    *out = nullptr;
+    const int N = atom_numbers, T = atom_type_numbers, P = T * (T + 1) / 2;
+    for (int i = 0; i < N; ++i)
+        if (atom_LJ_type[i] < 0 || atom_LJ_type[i] >= T) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    sponge_lj* lj = new sponge_lj();
+    lj->N = N;
+    lj->T = T;
+    lj->cutoff = cutoff;
+    lj->rc2_thr = sqrt_threshold(cutoff);
+    lj->stream = (hipStream_t)stream;
+    lj->h_A.assign(LJ_A, LJ_A + P);
+    lj->h_B.assign(LJ_B, LJ_B + P);
+    lj->h_type.assign(atom_LJ_type, atom_LJ_type + N);
+    // long-range factor (Lennard_Jones_force.cpp:208-235): C6 sum over the
+    // type histogram in double, then the float scale
+    std::vector<long long> count(T, 0);
+    for (int i = 0; i < N; ++i) count[atom_LJ_type[i]]++;
+    double c6 = 0.0;
+    for (int a = 0; a < T; ++a) {
+        if (!count[a]) continue;
+        double inner = 0.0;
+        for (int b = 0; b < T; ++b) {
+            if (!count[b]) continue;
+            inner += (double)count[b] * (double)LJ_B[lj_pair_index(a, b)];
+        }
+        c6 += (double)count[a] * inner;
+    }
+    float f = (float)c6;
+    f *= -2.0f / 3.0f * 3.1415926535897932f / cutoff / cutoff / cutoff / 6.0f;
+    lj->long_range_factor = f;
+    std::vector<float2> ab(P);
+    for (int k = 0; k < P; ++k) ab[k] = make_float2(LJ_A[k], LJ_B[k]);
+    int e = 0;
+    e |= dmalloc(&lj->d_AB, P);
+    e |= dmalloc(&lj->d_type, N);
+    e |= dmalloc(&lj->d_pos, N);
+    e |= dmalloc(&lj->d_ptype, N);
+    e |= dmalloc(&lj->d_facc, N);
+    e |= dmalloc(&lj->d_eacc, N);
+    e |= dmalloc(&lj->d_vacc, (size_t)6 * N);
+    e |= dmalloc(&lj->d_LJ_energy_atom, N);
+    e |= dmalloc(&lj->d_sum, 1);
+    if (e) {
+        lj_free(lj);
+        delete lj;
+        return SPONGE_AMD_ERROR_MALLOC_FAILED;
+    }
+    hipError_t he = hipMemcpyAsync(lj->d_AB, ab.data(), P * sizeof(float2), hipMemcpyHostToDevice,
+                                   lj->stream);
+    if (he == hipSuccess)
+        he = hipMemcpyAsync(lj->d_type, atom_LJ_type, N * sizeof(int), hipMemcpyHostToDevice,
+                            lj->stream);
+    int fe = 0;
+    if (he == hipSuccess) {
+        fe |= fill_async(lj->d_LJ_energy_atom, 0.f, N, lj->stream);
+        fe |= fill_async(lj->d_eacc, make_float2(0.f, 0.f), N, lj->stream);
+        fe |= fill_async(lj->d_vacc, 0.f, (long long)6 * N, lj->stream);
+        he = hipStreamSynchronize(lj->stream);
+    }
+    if (he != hipSuccess || fe) {
+        lj_free(lj);
+        delete lj;
+        return he != hipSuccess ? (int)he : fe;
+    }
+    *out = lj;
+    return 0;
+}
+
+extern "C" int sponge_lj_long_range_factor(sponge_lj* lj, float* factor) {
+    if (!lj || !factor) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    *factor = lj->long_range_factor;
+    return 0;
+}
+
+extern "C" int sponge_lj_pme_direct_force(sponge_lj* lj, sponge_nl* nl, int atom_numbers,
+                                          int local_atom_numbers, int solvent_numbers,
+                                          int ghost_numbers, const float* crd,
+                                          const float* charge, float* frc, const float* cell,
+                                          const float* rcell, float pme_beta,
+                                          int need_atom_energy, float* atom_energy,
+                                          int need_virial, float* atom_virial,
+                                          float* atom_direct_pme_energy) {
+    if (!lj || !nl || !crd || !charge || !frc || !cell || !rcell)
+        return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    if (ghost_numbers != 0 || solvent_numbers != 0) return SPONGE_AMD_ERROR_NOT_IMPLEMENTED;
+    if (atom_numbers == 0 || local_atom_numbers == 0) return 0;  // :351
+    if (local_atom_numbers != lj->N || nl->N != lj->N) return SPONGE_AMD_ERROR_CONFLICTING_COMMAND;
+    if (!nl->built) return SPONGE_AMD_ERROR_MISSING_COMMAND;
+    if ((need_atom_energy && !atom_energy) || (need_virial && !atom_virial))
+        return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    hipStream_t s = lj->stream;
+    const LT6 c = lt_from(cell), r = lt_from(rcell);
+    int rc = nl_pack_for_pairs(nl, crd, charge, lj->d_type, lj->d_pos, lj->d_ptype, lj->d_facc, s);
+    if (rc) return rc;
+    if (lt_is_orthorhombic(c))
+        rc = launch_pairs<true>(lj, lj->N, nl->d_list_count, nl->d_list,
+                                nl->opt.max_neighbor_numbers, c, r, pme_beta, need_atom_energy,
+                                need_virial, s);
+    else
+        rc = launch_pairs<false>(lj, lj->N, nl->d_list_count, nl->d_list,
+                                 nl->opt.max_neighbor_numbers, c, r, pme_beta, need_atom_energy,
+                                 need_virial, s);
+    if (rc) return rc;
+    return finalize(lj, lj->N, nl->d_sidx, need_atom_energy, need_virial, frc, atom_energy,
+                    atom_virial, atom_direct_pme_energy, s);
+}
+
+extern "C" int sponge_lj_pme_direct_force_list(sponge_lj* lj, const int* counts,
+                                               const int* serial, int max_numbers,
+                                               int atom_numbers, const float* crd,
+                                               const float* charge, float* frc,
+                                               const float* cell, const float* rcell,
+                                               float pme_beta, int need_atom_energy,
+                                               float* atom_energy, int need_virial,
+                                               float* atom_virial,
+                                               float* atom_direct_pme_energy) {
+    if (!lj || !counts || !serial || !crd || !charge || !frc || !cell || !rcell || max_numbers <= 0)
+        return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    if (atom_numbers != lj->N) return SPONGE_AMD_ERROR_CONFLICTING_COMMAND;
+    if ((need_atom_energy && !atom_energy) || (need_virial && !atom_virial))
+        return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    hipStream_t s = lj->stream;
+    const LT6 c = lt_from(cell), r = lt_from(rcell);
+    hipLaunchKernelGGL(k_lj_pack_local, dim3(grid_for(lj->N, 256)), dim3(256), 0, s, lj->N, crd,
+                       charge, lj->d_type, lj->d_pos, lj->d_ptype, lj->d_facc);
+    SA_LAUNCH_CHECK();
+    int rc;
+    if (lt_is_orthorhombic(c))
+        rc = launch_pairs<true>(lj, lj->N, counts, serial, max_numbers, c, r, pme_beta,
+                                need_atom_energy, need_virial, s);
+    else
+        rc = launch_pairs<false>(lj, lj->N, counts, serial, max_numbers, c, r, pme_beta,
+                                 need_atom_energy, need_virial, s);
+    if (rc) return rc;
+    return finalize(lj, lj->N, nullptr, need_atom_energy, need_virial, frc, atom_energy,
+                    atom_virial, atom_direct_pme_energy, s);
+}
+
+extern "C" int sponge_lj_long_range_correction(sponge_lj* lj, int need_pressure, float* d_virial,
+                                               int need_potential, float* d_potential,
+                                               float volume) {
+    if (!lj) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    if (need_pressure) {
+        if (!d_virial) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+        hipLaunchKernelGGL(k_add_diag, dim3(1), dim3(1), 0, lj->stream, d_virial,
+                           2 * lj->long_range_factor / volume);
+        SA_LAUNCH_CHECK();
+    }
+    if (need_potential) {
+        if (!d_potential) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+        hipLaunchKernelGGL(k_add_scalar, dim3(1), dim3(1), 0, lj->stream, d_potential,
+                           lj->long_range_factor / volume);
+        SA_LAUNCH_CHECK();
+        lj->h_LJ_long_energy = lj->long_range_factor / volume;
+    }
+    return 0;
+}
+
+extern "C" int sponge_lj_step_print(sponge_lj* lj, float* LJ_short, float* LJ_long) {
+    if (!lj) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    int rc = fill_async(lj->d_sum, 0.0, 1, lj->stream);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_sum_f32, dim3(64), dim3(256), 0, lj->stream, lj->N,
+                       lj->d_LJ_energy_atom, lj->d_sum);
+    SA_LAUNCH_CHECK();
+    double h = 0;
+    SA_CHECK(hipMemcpyAsync(&h, lj->d_sum, sizeof(double), hipMemcpyDeviceToHost, lj->stream));
+    SA_CHECK(hipStreamSynchronize(lj->stream));
+    if (LJ_short) *LJ_short = (float)h;
+    if (LJ_long) *LJ_long = lj->h_LJ_long_energy;
+    return 0;
+}
+
+extern "C" float* sponge_lj_atom_energy(sponge_lj* lj) { return lj ? lj->d_LJ_energy_atom : nullptr; }
+
+extern "C" int sponge_lj_clear(sponge_lj* lj) {
+    if (!lj) return 0;
+    (void)hipStreamSynchronize(lj->stream);
+    lj_free(lj);
+    delete lj;
+    return 0;
+}

@@ -1,0 +1,474 @@
+// NVE step driver for rigid water (+ ions) on one GPU: the caller side of the
+// hot path (Main_Calculate_Force + Main_Iteration of SPONGE/main.cpp:1355-1836
+// for mode NVE with SETTLE), so that a whole MD step -- neighbor-list check /
+// rebuild, excluded-pair correction, LJ + PME direct, long-range correction,
+// PME reciprocal, leap-frog and SETTLE -- runs on one stream with no host
+// synchronisation and can be captured into one HIP graph.
+//
+// Leap-frog (SPONGE/MD_core/nve.hpp:3-16): v += dt F / m; x += dt v.
+// SETTLE: the analytic rigid-triangle constraint of Miyamoto & Kollman
+// (J. Comput. Chem. 13, 952, 1992), written here from the paper: rotate the
+// canonical (COM-centred, in-plane) triangle by phi, psi and theta so that the
+// constraint displacements are consistent with the previous geometry; the
+// velocity correction is (x_constrained - x_unconstrained) / dt, as the
+// reference applies it (SPONGE/constrain/settle.cpp:109-214, 216-258).
+#include <cmath>
+#include <vector>
+
+#include "internal.h"
+#include "timing.h"
+
+using namespace sa;
+
+struct sponge_md {
+    int N = 0, n_water = 0;
+    float dt = 0;  // internal time units
+    float cutoff = 0, skin = 0;
+    LT6 cell{}, rcell{};
+    float volume = 0;
+    hipStream_t stream = nullptr;
+    sponge_nl* nl = nullptr;
+    sponge_lj* lj = nullptr;
+    sponge_pme* pme = nullptr;
+    float* d_crd = nullptr;
+    float* d_vel = nullptr;
+    float* d_frc = nullptr;
+    float* d_inv_mass = nullptr;
+    float* d_mass = nullptr;
+    float* d_charge = nullptr;
+    float* d_atom_energy = nullptr;
+    float* d_atom_virial = nullptr;
+    float* d_potential = nullptr;  // [0] potential of energy steps
+    float* d_virial = nullptr;
+    int* d_excl_start = nullptr;
+    int* d_excl_num = nullptr;
+    int* d_excl_list = nullptr;
+    int4* d_water = nullptr;       // O, H1, H2 per water
+    float4* d_old = nullptr;       // [2*n_water] previous B-A and C-A
+    float geom[5] = {0, 0, 0, 0, 0};  // canonical ya, xb, yb, xc, yc
+    float mO = 0, mH1 = 0, mH2 = 0;
+    double* d_ek = nullptr;
+    long long step = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int use_graph = 0;
+};
+
+namespace {
+
+__device__ __forceinline__ float3 ld3(const float* p, int i) {
+    return make_float3(p[3 * i], p[3 * i + 1], p[3 * i + 2]);
+}
+__device__ __forceinline__ void st3(float* p, int i, float3 v) {
+    p[3 * i] = v.x;
+    p[3 * i + 1] = v.y;
+    p[3 * i + 2] = v.z;
+}
+__device__ __forceinline__ float3 f3add(float3 a, float3 b) { return make_float3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ float3 f3sub(float3 a, float3 b) { return make_float3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ float3 f3scale(float s, float3 a) { return make_float3(s * a.x, s * a.y, s * a.z); }
+__device__ __forceinline__ float f3dot(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float3 f3cross(float3 a, float3 b) {
+    return make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float3 f3unit(float3 a) { return f3scale(rsqrtf(f3dot(a, a)), a); }
+
+// previous geometry of each water: B - A and C - A (minimum image)
+__global__ void k_settle_remember(int n, const int4* __restrict__ w, const float* __restrict__ crd,
+                                  LT6 c, LT6 r, float4* __restrict__ old) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int4 t = w[k];
+    const float3 a = ld3(crd, t.x);
+    const float3 b = disp_pbc<true>(ld3(crd, t.y), a, c, r);
+    const float3 cc = disp_pbc<true>(ld3(crd, t.z), a, c, r);
+    old[2 * k] = make_float4(b.x, b.y, b.z, 0.f);
+    old[2 * k + 1] = make_float4(cc.x, cc.y, cc.z, 0.f);
+}
+
+__global__ void k_leap_frog(int N, float* __restrict__ vel, float* __restrict__ crd,
+                            const float* __restrict__ frc, const float* __restrict__ inv_mass,
+                            float dt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const float im = inv_mass[i];
+    float3 v = ld3(vel, i);
+    const float3 f = ld3(frc, i);
+    v = f3add(v, f3scale(dt, f3scale(im, f)));
+    st3(vel, i, v);
+    st3(crd, i, f3add(ld3(crd, i), f3scale(dt, v)));
+}
+
+// SETTLE for one rigid triangle per thread.  Canonical geometry (COM at the
+// origin, molecule in the xy plane): A = (0, ya), B = (xb, yb), C = (xc, yc).
+__global__ void k_settle(int n, const int4* __restrict__ w, float* __restrict__ crd,
+                         float* __restrict__ vel, const float4* __restrict__ old, LT6 c, LT6 r,
+                         float ya, float xb, float yb, float xc, float yc, float mA, float mB,
+                         float mC, float inv_dt) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int4 t = w[k];
+    const float3 A1 = ld3(crd, t.x);
+    const float3 B1 = f3add(A1, disp_pbc<true>(ld3(crd, t.y), A1, c, r));
+    const float3 C1 = f3add(A1, disp_pbc<true>(ld3(crd, t.z), A1, c, r));
+    const float M = mA + mB + mC;
+    const float3 com = f3scale(1.0f / M, f3add(f3scale(mA, A1), f3add(f3scale(mB, B1), f3scale(mC, C1))));
+    const float3 a1 = f3sub(A1, com), b1 = f3sub(B1, com), c1 = f3sub(C1, com);
+    const float4 o0 = old[2 * k], o1 = old[2 * k + 1];
+    const float3 B0 = make_float3(o0.x, o0.y, o0.z), C0 = make_float3(o1.x, o1.y, o1.z);
+    // frame: ez normal to the old plane, ex normal to ez and the new A
+    const float3 ez = f3unit(f3cross(B0, C0));
+    const float3 ex = f3unit(f3cross(a1, ez));
+    const float3 ey = f3cross(ez, ex);
+    const float xb0 = f3dot(ex, B0), yb0 = f3dot(ey, B0);
+    const float xc0 = f3dot(ex, C0), yc0 = f3dot(ey, C0);
+    const float za = f3dot(ez, a1);
+    const float xb1 = f3dot(ex, b1), yb1 = f3dot(ey, b1), zb = f3dot(ez, b1);
+    const float xc1 = f3dot(ex, c1), yc1 = f3dot(ey, c1), zc = f3dot(ez, c1);
+    // tilt out of the old plane: phi about x, psi about y
+    const float sphi = za / ya;
+    const float cphi = sqrtf(fmaxf(0.f, 1.0f - sphi * sphi));
+    const float spsi = (zb - zc - (yb - yc) * sphi) / ((xc - xb) * cphi);
+    const float cpsi = sqrtf(fmaxf(0.f, 1.0f - spsi * spsi));
+    const float ya2 = ya * cphi;
+    const float xb2 = xb * cpsi, yb2 = yb * cphi + xb * spsi * sphi;
+    const float xc2 = xc * cpsi, yc2 = yc * cphi + xc * spsi * sphi;
+    // in-plane rotation theta from the angular-momentum condition
+    const float alpha = mB * (xb2 * xb0 + yb2 * yb0) + mC * (xc2 * xc0 + yc2 * yc0);
+    const float beta = mB * (yb2 * xb0 - xb2 * yb0) + mC * (yc2 * xc0 - xc2 * yc0);
+    const float gamma = mB * (yb1 * xb0 - xb1 * yb0) + mC * (yc1 * xc0 - xc1 * yc0);
+    const float ab2 = alpha * alpha + beta * beta;
+    const float sth = (alpha * gamma - beta * sqrtf(fmaxf(0.f, ab2 - gamma * gamma))) / ab2;
+    const float cth = sqrtf(fmaxf(0.f, 1.0f - sth * sth));
+    const float xa3 = -ya2 * sth, ya3 = ya2 * cth;
+    const float xb3 = xb2 * cth - yb2 * sth, yb3 = xb2 * sth + yb2 * cth;
+    const float xc3 = xc2 * cth - yc2 * sth, yc3 = xc2 * sth + yc2 * cth;
+    const float3 a3 = f3add(f3add(f3scale(xa3, ex), f3scale(ya3, ey)), f3scale(za, ez));
+    const float3 b3 = f3add(f3add(f3scale(xb3, ex), f3scale(yb3, ey)), f3scale(zb, ez));
+    const float3 c3 = f3add(f3add(f3scale(xc3, ex), f3scale(yc3, ey)), f3scale(zc, ez));
+    st3(crd, t.x, f3add(com, a3));
+    st3(crd, t.y, f3add(com, b3));
+    st3(crd, t.z, f3add(com, c3));
+    st3(vel, t.x, f3add(ld3(vel, t.x), f3scale(inv_dt, f3sub(a3, a1))));
+    st3(vel, t.y, f3add(ld3(vel, t.y), f3scale(inv_dt, f3sub(b3, b1))));
+    st3(vel, t.z, f3add(ld3(vel, t.z), f3scale(inv_dt, f3sub(c3, c1))));
+}
+
+__global__ void k_kinetic(int N, const float* __restrict__ vel, const float* __restrict__ mass,
+                          double* out) {
+    double s = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        const float3 v = ld3(vel, i);
+        s += 0.5 * (double)mass[i] * (double)(v.x * v.x + v.y * v.y + v.z * v.z);
+    }
+    s = wave_sum(s);
+    if ((threadIdx.x & (kWave - 1)) == 0) unsafeAtomicAdd(out, s);
+}
+
+__global__ void k_sum_energy(int N, const float* __restrict__ e, double* out) {
+    double s = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) s += e[i];
+    s = wave_sum(s);
+    if ((threadIdx.x & (kWave - 1)) == 0) unsafeAtomicAdd(out, s);
+}
+
+// canonical in-plane coordinates of a triangle from its three side lengths
+// and masses (COM at the origin, A on the +y axis)
+void triangle_geometry(double dAB, double dAC, double dBC, double mA, double mB, double mC,
+                       float* g) {
+    // A at the origin, B on -x, C from the law of cosines
+    const double bx = -dAB, by = 0.0;
+    const double cosA = (dBC * dBC - dAC * dAC - dAB * dAB) / (2 * dAC * dAB);
+    const double cx = dAC * cosA, cy = dAC * std::sqrt(1.0 - cosA * cosA);
+    const double M = mA + mB + mC;
+    const double ox = (mB * bx + mC * cx) / M, oy = (mB * by + mC * cy) / M;
+    double ax_ = -ox, ay_ = -oy, bx_ = bx - ox, by_ = by - oy, cx_ = cx - ox, cy_ = cy - oy;
+    // rotate so that A lies on the +y axis
+    const double ang = std::atan2(ax_, ay_);
+    const double cs = std::cos(ang), sn = std::sin(ang);
+    auto rot = [&](double x, double y, double& X, double& Y) {
+        X = x * cs - y * sn;
+        Y = x * sn + y * cs;
+    };
+    double X, Y;
+    rot(ax_, ay_, X, Y);
+    g[0] = (float)Y;
+    rot(bx_, by_, X, Y);
+    g[1] = (float)X;
+    g[2] = (float)Y;
+    rot(cx_, cy_, X, Y);
+    g[3] = (float)X;
+    g[4] = (float)Y;
+}
+
+void md_free(sponge_md* m) {
+    if (m->graph_exec) (void)hipGraphExecDestroy(m->graph_exec);
+    if (m->graph) (void)hipGraphDestroy(m->graph);
+    m->graph_exec = nullptr;
+    m->graph = nullptr;
+    sponge_nl_clear(m->nl);
+    sponge_lj_clear(m->lj);
+    sponge_pme_clear(m->pme);
+    m->nl = nullptr;
+    m->lj = nullptr;
+    m->pme = nullptr;
+    dfree(m->d_crd);
+    dfree(m->d_vel);
+    dfree(m->d_frc);
+    dfree(m->d_inv_mass);
+    dfree(m->d_mass);
+    dfree(m->d_charge);
+    dfree(m->d_atom_energy);
+    dfree(m->d_atom_virial);
+    dfree(m->d_potential);
+    dfree(m->d_virial);
+    dfree(m->d_excl_start);
+    dfree(m->d_excl_num);
+    dfree(m->d_excl_list);
+    dfree(m->d_water);
+    dfree(m->d_old);
+    dfree(m->d_ek);
+}
+
+// One MD step (force evaluation + integration), all on m->stream.
+int md_step(sponge_md* m, long long step, int need_energy) {
+    hipStream_t s = m->stream;
+    const int N = m->N;
+    int rc = fill_async(m->d_frc, 0.f, (long long)3 * N, s);
+    if (!rc && need_energy) {
+        rc |= fill_async(m->d_atom_energy, 0.f, N, s);
+        rc |= fill_async(m->d_potential, 0.f, 1, s);
+    }
+    if (rc) return rc;
+    const float* cell = &m->cell.a11;
+    const float* rcell = &m->rcell.a11;
+    rc = sponge_nl_update(m->nl, nullptr, N, 0, m->d_crd, cell, rcell, (int)step,
+                          SPONGE_AMD_NL_CONDITIONAL, m->d_excl_start, m->d_excl_list, m->d_excl_num);
+    if (rc) return rc;
+    rc = sponge_pme_excluded_force(m->pme, N, nullptr, nullptr, m->d_crd, cell, rcell, m->d_charge,
+                                   m->d_excl_start, m->d_excl_list, m->d_excl_num, m->d_frc,
+                                   need_energy, m->d_atom_energy, 0, nullptr);
+    if (rc) return rc;
+    rc = sponge_lj_pme_direct_force(m->lj, m->nl, N, N, 0, 0, m->d_crd, m->d_charge, m->d_frc, cell,
+                                    rcell, m->pme->beta, need_energy, m->d_atom_energy, 0, nullptr,
+                                    m->pme->d_direct_atom_energy);
+    if (rc) return rc;
+    if (need_energy) {
+        rc = sponge_lj_long_range_correction(m->lj, 0, nullptr, 1, m->d_potential, m->volume);
+        if (rc) return rc;
+    }
+    rc = sponge_pme_reciprocal_force(m->pme, m->d_crd, cell, rcell, m->d_charge, m->d_frc, 0,
+                                     need_energy, nullptr, need_energy ? m->d_potential : nullptr,
+                                     (int)step);
+    if (rc) return rc;
+    // integration (Main_Iteration, SPONGE/main.cpp:1702-1774)
+    if (m->n_water) {
+        hipLaunchKernelGGL(k_settle_remember, dim3(grid_for(m->n_water, 256)), dim3(256), 0, s,
+                           m->n_water, m->d_water, m->d_crd, m->cell, m->rcell, m->d_old);
+        SA_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_leap_frog, dim3(grid_for(N, 256)), dim3(256), 0, s, N, m->d_vel, m->d_crd,
+                       m->d_frc, m->d_inv_mass, m->dt);
+    SA_LAUNCH_CHECK();
+    if (m->n_water) {
+        hipLaunchKernelGGL(k_settle, dim3(grid_for(m->n_water, 256)), dim3(256), 0, s, m->n_water,
+                           m->d_water, m->d_crd, m->d_vel, m->d_old, m->cell, m->rcell, m->geom[0],
+                           m->geom[1], m->geom[2], m->geom[3], m->geom[4], m->mO, m->mH1, m->mH2,
+                           1.0f / m->dt);
+        SA_LAUNCH_CHECK();
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int sponge_md_initial(sponge_md** out, int atom_numbers, const float* crd,
+                                 const float* vel, const float* mass, const float* charge,
+                                 int atom_type_numbers, const float* LJ_A, const float* LJ_B,
+                                 const int* atom_LJ_type, const int* excluded_list_start,
+                                 const int* excluded_numbers, const int* excluded_list,
+                                 int n_water, const int* water_atoms, const float* box_length,
+                                 float cutoff, float skin, float dt_ps,
+                                 const sponge_nl_options* nl_opt,
+                                 const sponge_pme_options* pme_opt, void* stream) {
+    if (!out || atom_numbers <= 0 || !crd || !vel || !mass || !charge || !box_length ||
+        !excluded_list_start || !excluded_numbers || (n_water > 0 && !water_atoms) || !(dt_ps > 0))
+        return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    *out = nullptr;
+    sponge_md* m = new sponge_md();
+    const int N = atom_numbers;
+    m->N = N;
+    m->n_water = n_water;
+    m->dt = dt_ps * 20.455f;  // CONSTANT_TIME_CONVERTION, SPONGE/MD_core/MD_core.cpp:132
+    m->cutoff = cutoff;
+    m->skin = skin;
+    m->stream = (hipStream_t)stream;
+    const float one = 1.0f;
+    m->cell = {box_length[0], 0.f, box_length[1], 0.f, 0.f, box_length[2]};
+    m->rcell = {one / box_length[0], 0.f, one / box_length[1], 0.f, 0.f, one / box_length[2]};
+    m->volume = box_length[0] * box_length[1] * box_length[2];
+    long long E = 0;
+    for (int i = 0; i < N; ++i) E += excluded_numbers[i];
+    std::vector<float> inv_mass(N);
+    for (int i = 0; i < N; ++i) inv_mass[i] = mass[i] > 0 ? 1.0f / mass[i] : 0.0f;
+    std::vector<int4> water(n_water);
+    if (n_water > 0) {
+        for (int k = 0; k < n_water; ++k) {
+            water[k] = make_int4(water_atoms[3 * k], water_atoms[3 * k + 1], water_atoms[3 * k + 2], 0);
+            if (water[k].x < 0 || water[k].x >= N || water[k].y < 0 || water[k].y >= N ||
+                water[k].z < 0 || water[k].z >= N) {
+                delete m;
+                return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+            }
+        }
+        // constraint lengths from the first water as given (rigid model)
+        const int a = water[0].x, b = water[0].y, c = water[0].z;
+        auto dist = [&](int i, int j) {
+            double d2 = 0;
+            for (int q = 0; q < 3; ++q) {
+                double d = (double)crd[3 * i + q] - crd[3 * j + q];
+                d2 += d * d;
+            }
+            return std::sqrt(d2);
+        };
+        m->mO = mass[a];
+        m->mH1 = mass[b];
+        m->mH2 = mass[c];
+        triangle_geometry(dist(a, b), dist(a, c), dist(b, c), mass[a], mass[b], mass[c], m->geom);
+    }
+    int e = 0;
+    e |= dmalloc(&m->d_crd, (size_t)3 * N);
+    e |= dmalloc(&m->d_vel, (size_t)3 * N);
+    e |= dmalloc(&m->d_frc, (size_t)3 * N);
+    e |= dmalloc(&m->d_inv_mass, N);
+    e |= dmalloc(&m->d_mass, N);
+    e |= dmalloc(&m->d_charge, N);
+    e |= dmalloc(&m->d_atom_energy, N);
+    e |= dmalloc(&m->d_atom_virial, (size_t)6 * N);
+    e |= dmalloc(&m->d_potential, 1);
+    e |= dmalloc(&m->d_virial, 6);
+    e |= dmalloc(&m->d_excl_start, N);
+    e |= dmalloc(&m->d_excl_num, N);
+    e |= dmalloc(&m->d_excl_list, E > 0 ? E : 1);
+    e |= dmalloc(&m->d_water, n_water > 0 ? n_water : 1);
+    e |= dmalloc(&m->d_old, (size_t)2 * (n_water > 0 ? n_water : 1));
+    e |= dmalloc(&m->d_ek, 2);
+    if (e) {
+        md_free(m);
+        delete m;
+        return SPONGE_AMD_ERROR_MALLOC_FAILED;
+    }
+    hipStream_t s = m->stream;
+    hipError_t he = hipSuccess;
+    auto cp = [&](void* d, const void* h, size_t bytes) {
+        if (he == hipSuccess && bytes) he = hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s);
+    };
+    cp(m->d_crd, crd, sizeof(float) * 3 * N);
+    cp(m->d_vel, vel, sizeof(float) * 3 * N);
+    cp(m->d_inv_mass, inv_mass.data(), sizeof(float) * N);
+    cp(m->d_mass, mass, sizeof(float) * N);
+    cp(m->d_charge, charge, sizeof(float) * N);
+    cp(m->d_excl_start, excluded_list_start, sizeof(int) * N);
+    cp(m->d_excl_num, excluded_numbers, sizeof(int) * N);
+    if (E > 0) cp(m->d_excl_list, excluded_list, sizeof(int) * E);
+    if (n_water > 0) cp(m->d_water, water.data(), sizeof(int4) * n_water);
+    if (he == hipSuccess) he = hipStreamSynchronize(s);
+    int rc = he != hipSuccess ? (int)he : 0;
+    if (!rc) rc = sponge_nl_initial(&m->nl, N, cutoff, skin, &m->cell.a11, &m->rcell.a11, nl_opt, stream);
+    if (!rc) rc = sponge_lj_initial(&m->lj, N, atom_type_numbers, LJ_A, LJ_B, atom_LJ_type, cutoff, stream);
+    if (!rc) rc = sponge_pme_initial(&m->pme, N, &m->cell.a11, &m->rcell.a11, cutoff, pme_opt, stream);
+    if (rc) {
+        md_free(m);
+        delete m;
+        return rc;
+    }
+    *out = m;
+    return 0;
+}
+
+extern "C" int sponge_md_run(sponge_md* m, int steps, int energy_interval, int use_graph) {
+    if (!m || steps < 0) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    for (int k = 0; k < steps; ++k) {
+        const long long st = m->step;
+        const int need_e = energy_interval > 0 && st % energy_interval == 0;
+        // graph replay needs the fixed default schedule: automatic list
+        // refresh and a reciprocal part on every step
+        const bool graphable = use_graph && !need_e && m->nl->built &&
+                               m->nl->opt.refresh_interval <= 0 &&
+                               m->pme->opt.update_interval == 1;
+        if (graphable) {
+            if (!m->graph_exec) {
+                SA_CHECK(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
+                int rc = md_step(m, st, 0);
+                hipGraph_t g = nullptr;
+                hipError_t he = hipStreamEndCapture(m->stream, &g);
+                if (rc) return rc;
+                SA_CHECK(he);
+                m->graph = g;
+                SA_CHECK(hipGraphInstantiate(&m->graph_exec, g, nullptr, nullptr, 0));
+            }
+            SA_CHECK(hipGraphLaunch(m->graph_exec, m->stream));
+        } else {
+            int rc = md_step(m, st, need_e);
+            if (rc) return rc;
+        }
+        m->step++;
+    }
+    return 0;
+}
+
+extern "C" int sponge_md_energies(sponge_md* m, double* out) {
+    // out: [0] potential (last energy step), [1] kinetic, [2] LJ_short,
+    // [3] LJ_long, [4] PM, [5] PM_direct, [6] PM_reciprocal, [7] PM_self,
+    // [8] PM_correction, [9] sum of per-atom energies of the last energy step
+    if (!m || !out) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    hipStream_t s = m->stream;
+    int rc = fill_async(m->d_ek, 0.0, 2, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_kinetic, dim3(64), dim3(256), 0, s, m->N, m->d_vel, m->d_mass, m->d_ek);
+    hipLaunchKernelGGL(k_sum_energy, dim3(64), dim3(256), 0, s, m->N, m->d_atom_energy, m->d_ek + 1);
+    SA_LAUNCH_CHECK();
+    double ek[2];
+    float pot = 0;
+    SA_CHECK(hipMemcpyAsync(ek, m->d_ek, sizeof(ek), hipMemcpyDeviceToHost, s));
+    SA_CHECK(hipMemcpyAsync(&pot, m->d_potential, sizeof(float), hipMemcpyDeviceToHost, s));
+    SA_CHECK(hipStreamSynchronize(s));
+    float lj_short = 0, lj_long = 0, pm[5] = {0, 0, 0, 0, 0};
+    rc = sponge_lj_step_print(m->lj, &lj_short, &lj_long);
+    if (!rc) rc = sponge_pme_step_print(m->pme, &pm[0], &pm[1], &pm[2], &pm[3], &pm[4]);
+    if (rc) return rc;
+    out[0] = (double)pot + ek[1];
+    out[1] = ek[0];
+    out[2] = lj_short;
+    out[3] = lj_long;
+    for (int k = 0; k < 5; ++k) out[4 + k] = pm[k];
+    out[9] = ek[1];
+    return 0;
+}
+
+extern "C" int sponge_md_get_state(sponge_md* m, float* crd, float* vel, float* frc) {
+    if (!m) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    hipStream_t s = m->stream;
+    const size_t b = sizeof(float) * 3 * m->N;
+    if (crd) SA_CHECK(hipMemcpyAsync(crd, m->d_crd, b, hipMemcpyDeviceToHost, s));
+    if (vel) SA_CHECK(hipMemcpyAsync(vel, m->d_vel, b, hipMemcpyDeviceToHost, s));
+    if (frc) SA_CHECK(hipMemcpyAsync(frc, m->d_frc, b, hipMemcpyDeviceToHost, s));
+    SA_CHECK(hipStreamSynchronize(s));
+    return 0;
+}
+
+extern "C" int sponge_md_handles(sponge_md* m, sponge_nl** nl, sponge_lj** lj, sponge_pme** pme) {
+    if (!m) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    if (nl) *nl = m->nl;
+    if (lj) *lj = m->lj;
+    if (pme) *pme = m->pme;
+    return 0;
+}
+
+extern "C" int sponge_md_clear(sponge_md* m) {
+    if (!m) return 0;
+    (void)hipStreamSynchronize(m->stream);
+    md_free(m);
+    delete m;
+    return 0;
+}

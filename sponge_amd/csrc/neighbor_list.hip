@@ -1,0 +1,580 @@
+// Cell-grid half neighbor list for gfx950.
+//
+// Reference: SPONGE/neighbor_list/neighbor_list.cpp.  The pair set is the
+// reference's: pair (i, j) is listed iff global(j) > global(i), the
+// minimum-image r^2 (Get_Displacement<Periodic>, evaluated without FMA
+// contraction) is < cutoff_skin_square = gl*gl*4 with gl = 0.5f*(cutoff+skin)
+// (neighbor_list.cpp:859, 998), and j is not excluded from i
+// (Delete_Excluded_Atoms_Serial_In_Neighbor_List, :736-807).
+//
+// The MI355X design differs from the reference's bucket-per-cell layout:
+//   1. atoms are counting-sorted by cell (atomic histogram, one-block scan,
+//      scatter) into a dense SoA float4 array, so every stencil column is a
+//      contiguous, coalesced range of the sorted array;
+//   2. the stencil of each cell (the reference's ascending +-2 neighbour cells,
+//      Find_Neighor_Grids_Device :76-191) is compressed on the host into runs
+//      of consecutive cell ids;
+//   3. one wavefront owns one i atom and sweeps the runs 64 candidates at a
+//      time; acceptance is compacted with a 64-bit ballot and mbcnt, so list
+//      slots need no atomics and the exclusions are filtered inline instead
+//      of by a separate swap-with-last pass.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "internal.h"
+#include "timing.h"
+
+using namespace sa;
+
+namespace {
+
+constexpr int kMaxGridNeighbors = 192;  // MAX_GRID_NEIGHBORS, neighbor_list.cpp:5
+
+__global__ void k_nl_set(int* need, int v) { *need = v; }
+
+// Check_Refresh (neighbor_list.cpp:307-321)
+template <bool ORTHO>
+__global__ void k_nl_check(int N, const float* __restrict__ crd, const float* __restrict__ old,
+                           LT6 c, LT6 r, float permit2, int* need) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    float3 a = make_float3(crd[3 * i], crd[3 * i + 1], crd[3 * i + 2]);
+    float3 b = make_float3(old[3 * i], old[3 * i + 1], old[3 * i + 2]);
+    float3 d = disp_pbc<ORTHO>(a, b, c, r);
+    if (norm2_exact(d) > permit2) *need = 1;
+}
+
+// Cell of an atom as Put_Atom_In_Grids wraps it (neighbor_list.cpp:357-377).
+__device__ __forceinline__ int cell_of(float x, float y, float z, const LT6& c, float gl, int Nx,
+                                       int Ny, int Nz) {
+#pragma clang fp contract(off)
+    float k3 = floorf(z / c.a33);
+    z -= k3 * c.a33;
+    y -= k3 * c.a32;
+    float k2 = floorf(y / c.a22);
+    y -= k2 * c.a22;
+    x -= k3 * c.a31 + k2 * c.a21;
+    x -= floorf(x / c.a11) * c.a11;
+    int nx = (int)(x / gl), ny = (int)(y / gl), nz = (int)(z / gl);
+    nx = nx < 0 ? 0 : (nx < Nx ? nx : Nx - 1);
+    ny = ny < 0 ? 0 : (ny < Ny ? ny : Ny - 1);
+    nz = nz < 0 ? 0 : (nz < Nz ? nz : Nz - 1);
+    return nx * Ny * Nz + ny * Nz + nz;
+}
+
+__global__ void k_nl_bin(const int* __restrict__ need, int N, const float* __restrict__ crd,
+                         float* __restrict__ old, int need_copy, LT6 c, float gl, int Nx, int Ny,
+                         int Nz, int* __restrict__ count, int* __restrict__ atom_cell,
+                         int* __restrict__ atom_slot) {
+    if (*need == 0) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    float x = crd[3 * i], y = crd[3 * i + 1], z = crd[3 * i + 2];
+    if (need_copy) {
+        old[3 * i] = x;
+        old[3 * i + 1] = y;
+        old[3 * i + 2] = z;
+    }
+    int cell = cell_of(x, y, z, c, gl, Nx, Ny, Nz);
+    atom_cell[i] = cell;
+    atom_slot[i] = atomicAdd(&count[cell], 1);
+}
+
+// One-block exclusive scan of the cell counts -> cell_start[G+1]; flags cells
+// above max_in_grid (the reference's grid overflow, :382-386) without
+// dropping atoms.
+__global__ void __launch_bounds__(1024) k_nl_scan(const int* __restrict__ need, int G,
+                                                  int* __restrict__ count, int* __restrict__ start,
+                                                  int max_in_grid, int* __restrict__ flags,
+                                                  unsigned long long* __restrict__ rebuilds) {
+    if (*need == 0) return;
+    __shared__ int part[1024];
+    const int t = threadIdx.x;
+    const int chunk = (G + 1023) / 1024;
+    const int b = t * chunk, e = min(G, b + chunk);
+    int s = 0, over = 0;
+    for (int c = b; c < e; ++c) {
+        int v = count[c];
+        s += v;
+        over |= v > max_in_grid;
+    }
+    if (over) atomicOr(flags, SPONGE_AMD_NL_GRID_OVERFLOW);
+    part[t] = s;
+    __syncthreads();
+    // Hillis-Steele inclusive scan over 1024 partial sums
+    for (int off = 1; off < 1024; off <<= 1) {
+        int v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = part[t] - s;  // exclusive prefix of this chunk
+    for (int c = b; c < e; ++c) {
+        start[c] = run;
+        run += count[c];
+        count[c] = 0;  // left clear for the next rebuild
+    }
+    if (t == 1023) {
+        start[G] = part[1023];
+        rebuilds[0] += 1;
+    }
+}
+
+__global__ void k_nl_scatter(const int* __restrict__ need, int N, const float* __restrict__ crd,
+                             const int* __restrict__ atom_local,
+                             const int* __restrict__ atom_cell, const int* __restrict__ atom_slot,
+                             const int* __restrict__ start, int* __restrict__ sidx,
+                             int* __restrict__ rank, float4* __restrict__ spos) {
+    if (*need == 0) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    int p = start[atom_cell[i]] + atom_slot[i];
+    int gi = atom_local ? atom_local[i] : i;
+    sidx[p] = i;
+    rank[i] = p;
+    spos[p] = make_float4(crd[3 * i], crd[3 * i + 1], crd[3 * i + 2], __int_as_float(gi));
+}
+
+// One wavefront per sorted atom p.  Lanes take 64 consecutive candidates of
+// a run of neighbour cells; accepted candidates are compacted by ballot.
+template <bool ORTHO>
+__global__ void __launch_bounds__(256) k_nl_search(
+    const int* __restrict__ need, int N, const float4* __restrict__ spos,
+    const int* __restrict__ sidx, const int* __restrict__ atom_cell,
+    const int* __restrict__ start, const int2* __restrict__ ranges,
+    const int* __restrict__ nranges, int max_ranges, LT6 c, LT6 r, float thr,
+    const int* __restrict__ ex_start, const int* __restrict__ ex_list,
+    const int* __restrict__ ex_num, int max_nb, int* __restrict__ list_count,
+    int* __restrict__ list, int* __restrict__ flags) {
+    if (*need == 0) return;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int p = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    if (p >= N) return;
+    const float4 pi4 = spos[p];
+    const float3 pi = make_float3(pi4.x, pi4.y, pi4.z);
+    const int gi = __float_as_int(pi4.w);
+    const int cell = atom_cell[sidx[p]];
+    int ex_n = 0, ex_s = 0, ex_lo = 0, ex_hi = -1;
+    if (ex_num) {
+        ex_n = ex_num[gi];
+        if (ex_n > 0) {
+            ex_s = ex_start[gi];
+            ex_lo = ex_list[ex_s];
+            ex_hi = ex_list[ex_s + ex_n - 1];
+        }
+    }
+    int* out = list + (size_t)p * max_nb;
+    int count = 0;
+    const int nr = nranges[cell];
+    for (int k = 0; k < nr; ++k) {
+        const int2 cr = ranges[(size_t)cell * max_ranges + k];
+        const int a0 = start[cr.x], a1 = start[cr.y];
+        for (int base = a0; base < a1; base += kWave) {
+            const int q = base + lane;
+            bool ok = false;
+            if (q < a1) {
+                const float4 pj4 = spos[q];
+                const int gj = __float_as_int(pj4.w);
+                if (gj > gi) {
+                    // Get_Displacement(crd_i, crd_j) as in Find_Neighbors_Gridly
+                    float3 d = disp_pbc<ORTHO>(pi, make_float3(pj4.x, pj4.y, pj4.z), c, r);
+                    if (norm2_exact(d) < thr) {
+                        ok = true;
+                        if (ex_n > 0 && gj >= ex_lo && gj <= ex_hi) {
+                            for (int e = ex_s; e < ex_s + ex_n; ++e)
+                                if (ex_list[e] == gj) ok = false;
+                        }
+                    }
+                }
+            }
+            const unsigned long long m = __ballot(ok);
+            if (ok) {
+                const int pos = count + lanes_below(m);
+                if (pos < max_nb) out[pos] = q;
+            }
+            count += __popcll(m);
+        }
+    }
+    if (lane == 0) {
+        list_count[p] = count < max_nb ? count : max_nb;
+        if (count > max_nb) atomicOr(flags, SPONGE_AMD_NL_LIST_OVERFLOW);
+    }
+}
+
+// Reference-format export: local ids, rows by local atom.
+__global__ void k_nl_export(int N, const int* __restrict__ sidx,
+                            const int* __restrict__ list_count, const int* __restrict__ list,
+                            int max_nb, int* __restrict__ counts, int* __restrict__ serial,
+                            int max_out, sa_atom_group* group, int* __restrict__ flags) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int p = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    if (p >= N) return;
+    const int i = sidx[p];
+    const int n = list_count[p];
+    const int nout = n < max_out ? n : max_out;
+    const int* in = list + (size_t)p * max_nb;
+    int* o = serial + (size_t)i * max_out;
+    for (int k = lane; k < nout; k += kWave) o[k] = sidx[in[k]];
+    if (lane == 0) {
+        counts[i] = nout;
+        if (group) {
+            group[i].atom_numbers = nout;
+            group[i].ghost_numbers = 0;
+            group[i].atom_serial = o;
+        }
+        if (n > max_out) atomicOr(flags, SPONGE_AMD_NL_LIST_OVERFLOW);
+    }
+}
+
+__global__ void k_nl_sum_counts(int N, const int* __restrict__ cnt, unsigned long long* out) {
+    unsigned long long s = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) s += cnt[i];
+    s = wave_sum(s);
+    if ((threadIdx.x & (kWave - 1)) == 0) atomicAdd(out, s);
+}
+
+// Neighbour cells of every cell, as the enumeration branch of
+// Find_Neighor_Grids_Device (neighbor_list.cpp:96-181): for each of the 27
+// periodic images the cells with offset in [-2, 2] (the low bound widened to
+// -3 along x / y for sheared images), sorted ascending and de-duplicated;
+// then compressed into runs of consecutive ids.
+int build_stencil(sponge_nl* nl, std::vector<int2>& ranges, std::vector<int>& nranges,
+                  int& max_ranges) {
+    const int Nx = nl->Nx, Ny = nl->Ny, Nz = nl->Nz;
+    const LT6 cell = nl->cell0;
+    const float gl = nl->grid_length;
+    const int64_t dxy = (int64_t)(cell.a21 / gl), dxz = (int64_t)(cell.a31 / gl),
+                  dyz = (int64_t)(cell.a32 / gl);
+    std::vector<std::vector<int2>> all(nl->G);
+    max_ranges = 0;
+    std::vector<int> cand;
+    for (int g = 0; g < nl->G; ++g) {
+        const int64_t gx = g / ((int64_t)Ny * Nz), gy = (g % ((int64_t)Ny * Nz)) / Nz, gz = g % Nz;
+        cand.clear();
+        for (int k = 0; k < 27; ++k) {
+            const int dx = k / 9 - 1, dy = (k % 9) / 3 - 1, dz = k % 3 - 1;
+            const int cx = (double)dy * cell.a21 + (double)dz * cell.a31 == 0.0 ? -2 : -3;
+            const int cy = (double)dz * cell.a32 == 0.0 ? -2 : -3;
+            const int64_t sx = (int64_t)dx * Nx + dy * dxy + dz * dxz;
+            const int64_t sy = (int64_t)dy * Ny + dz * dyz;
+            const int64_t sz = (int64_t)dz * Nz;
+            const int64_t lox = std::max<int64_t>(gx + sx - 2, 0), hix = std::min<int64_t>(gx + sx - cx, Nx - 1);
+            const int64_t loy = std::max<int64_t>(gy + sy - 2, 0), hiy = std::min<int64_t>(gy + sy - cy, Ny - 1);
+            const int64_t loz = std::max<int64_t>(gz + sz - 2, 0), hiz = std::min<int64_t>(gz + sz + 2, Nz - 1);
+            for (int64_t jx = lox; jx <= hix; ++jx)
+                for (int64_t jy = loy; jy <= hiy; ++jy)
+                    for (int64_t jz = loz; jz <= hiz; ++jz) cand.push_back((int)((jx * Ny + jy) * Nz + jz));
+        }
+        std::sort(cand.begin(), cand.end());
+        cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+        if ((int)cand.size() > kMaxGridNeighbors) return SPONGE_AMD_ERROR_OVERFLOW;
+        auto& rr = all[g];
+        for (size_t a = 0; a < cand.size();) {
+            size_t b = a + 1;
+            while (b < cand.size() && cand[b] == cand[b - 1] + 1) ++b;
+            rr.push_back(make_int2(cand[a], cand[b - 1] + 1));
+            a = b;
+        }
+        max_ranges = std::max(max_ranges, (int)rr.size());
+    }
+    ranges.assign((size_t)nl->G * max_ranges, make_int2(0, 0));
+    nranges.assign(nl->G, 0);
+    for (int g = 0; g < nl->G; ++g) {
+        nranges[g] = (int)all[g].size();
+        for (size_t k = 0; k < all[g].size(); ++k) ranges[(size_t)g * max_ranges + k] = all[g][k];
+    }
+    return 0;
+}
+
+void nl_free(sponge_nl* nl) {
+    dfree(nl->d_need);
+    dfree(nl->d_flags);
+    dfree(nl->d_rebuilds);
+    dfree(nl->d_cell_count);
+    dfree(nl->d_cell_start);
+    dfree(nl->d_atom_cell);
+    dfree(nl->d_atom_slot);
+    dfree(nl->d_sidx);
+    dfree(nl->d_rank);
+    dfree(nl->d_spos);
+    dfree(nl->d_old_crd);
+    dfree(nl->d_ranges);
+    dfree(nl->d_nranges);
+    dfree(nl->d_list_count);
+    dfree(nl->d_list);
+    dfree(nl->d_exp_serial);
+    dfree(nl->d_exp_count);
+    dfree(nl->d_atom_group);
+}
+
+}  // namespace
+
+extern "C" void sponge_nl_default_options(sponge_nl_options* o) {
+    o->refresh_interval = 0;
+    o->skin_permit = 0.5f;
+    o->max_neighbor_numbers = 1200;
+    o->max_atom_in_grid_numbers = 150;
+    o->throw_error_when_overflow = 0;
+    o->check_overflow_interval = 150;
+}
+
+extern "C" int sponge_nl_initial(sponge_nl** out, int atom_numbers, float cutoff, float skin,
+                                 const float* cell, const float* rcell,
+                                 const sponge_nl_options* opt, void* stream) {
+    if (!out || !cell || !rcell || atom_numbers <= 0 || !(cutoff > 0) || !(skin >= 0))
+        return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    *out = nullptr;
+    sponge_nl* nl = new sponge_nl();
+    if (opt) nl->opt = *opt;
+    else sponge_nl_default_options(&nl->opt);
+    if (nl->opt.max_neighbor_numbers <= 0 || nl->opt.max_atom_in_grid_numbers <= 0) {
+        delete nl;
+        return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    }
+    nl->N = atom_numbers;
+    nl->cutoff = cutoff;
+    nl->skin = skin;
+    nl->stream = (hipStream_t)stream;
+    nl->cell0 = lt_from(cell);
+    nl->rcell0 = lt_from(rcell);
+    nl->ortho = lt_is_orthorhombic(nl->cell0) ? 1 : 0;
+    nl->grid_length = (cutoff + skin) * 0.5f;                       // :948
+    nl->thr = nl->grid_length * nl->grid_length * 4.0f;             // :859
+    nl->Nx = (int)floorf(nl->cell0.a11 / nl->grid_length);          // :198-200
+    nl->Ny = (int)floorf(nl->cell0.a22 / nl->grid_length);
+    nl->Nz = (int)floorf(nl->cell0.a33 / nl->grid_length);
+    if (nl->Nx <= 2 || nl->Ny <= 2 || nl->Nz <= 2) {  // :949-954
+        delete nl;
+        return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    }
+    nl->G = nl->Nx * nl->Ny * nl->Nz;
+    std::vector<int2> ranges;
+    std::vector<int> nranges;
+    int rc = build_stencil(nl, ranges, nranges, nl->max_ranges);
+    if (rc) {
+        delete nl;
+        return rc;
+    }
+    const int N = nl->N, G = nl->G, M = nl->opt.max_neighbor_numbers;
+    int e = 0;
+    e |= dmalloc(&nl->d_need, 1);
+    e |= dmalloc(&nl->d_flags, 1);
+    e |= dmalloc(&nl->d_rebuilds, 1);
+    e |= dmalloc(&nl->d_cell_count, G);
+    e |= dmalloc(&nl->d_cell_start, G + 1);
+    e |= dmalloc(&nl->d_atom_cell, N);
+    e |= dmalloc(&nl->d_atom_slot, N);
+    e |= dmalloc(&nl->d_sidx, N);
+    e |= dmalloc(&nl->d_rank, N);
+    e |= dmalloc(&nl->d_spos, N);
+    e |= dmalloc(&nl->d_old_crd, (size_t)3 * N);
+    e |= dmalloc(&nl->d_ranges, ranges.size());
+    e |= dmalloc(&nl->d_nranges, G);
+    e |= dmalloc(&nl->d_list_count, N);
+    e |= dmalloc(&nl->d_list, (size_t)N * M);
+    if (e) {
+        nl_free(nl);
+        delete nl;
+        return SPONGE_AMD_ERROR_MALLOC_FAILED;
+    }
+    hipStream_t s = nl->stream;
+    hipError_t he = hipMemcpyAsync(nl->d_ranges, ranges.data(), ranges.size() * sizeof(int2),
+                                   hipMemcpyHostToDevice, s);
+    if (he == hipSuccess)
+        he = hipMemcpyAsync(nl->d_nranges, nranges.data(), nranges.size() * sizeof(int),
+                            hipMemcpyHostToDevice, s);
+    int fe = 0;
+    if (he == hipSuccess) {
+        fe |= fill_async(nl->d_flags, 0, 1, s);
+        fe |= fill_async(nl->d_rebuilds, 0ull, 1, s);
+        fe |= fill_async(nl->d_cell_count, 0, G, s);
+        fe |= fill_async(nl->d_list_count, 0, N, s);
+        he = hipStreamSynchronize(s);
+    }
+    if (he != hipSuccess || fe) {
+        nl_free(nl);
+        delete nl;
+        return he != hipSuccess ? (int)he : fe;
+    }
+    *out = nl;
+    return 0;
+}
+
+extern "C" int sponge_nl_update(sponge_nl* nl, const int* atom_local, int local_atom_numbers,
+                                int ghost_numbers, const float* crd, const float* cell,
+                                const float* rcell, int step, int update,
+                                const int* excluded_list_start, const int* excluded_list,
+                                const int* excluded_numbers) {
+    if (!nl || !crd || !cell || !rcell) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    if (ghost_numbers != 0) return SPONGE_AMD_ERROR_NOT_IMPLEMENTED;
+    if (local_atom_numbers <= 0) return 0;  // :980
+    if (local_atom_numbers != nl->N) return SPONGE_AMD_ERROR_CONFLICTING_COMMAND;
+    if ((excluded_numbers != nullptr) != (excluded_list_start != nullptr))
+        return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    const LT6 c = lt_from(cell), r = lt_from(rcell);
+    // the cell stencil belongs to the box of Initial (the reference rebuilds
+    // the module when the box changes, SPONGE/main.cpp:2104-2110)
+    if (memcmp(&c, &nl->cell0, sizeof(LT6)) != 0) return SPONGE_AMD_ERROR_CONFLICTING_COMMAND;
+    const int N = nl->N;
+    hipStream_t s = nl->stream;
+    const bool auto_refresh = nl->opt.refresh_interval <= 0;
+    // need flag (neighbor_list.cpp:983-998)
+    if (update == SPONGE_AMD_NL_FORCED || !nl->built) {
+        hipLaunchKernelGGL(k_nl_set, dim3(1), dim3(1), 0, s, nl->d_need, 1);
+    } else if (auto_refresh) {
+        hipLaunchKernelGGL(k_nl_set, dim3(1), dim3(1), 0, s, nl->d_need, 0);
+        const float permit2 = nl->skin * nl->skin * nl->opt.skin_permit * nl->opt.skin_permit;
+        if (nl->ortho)
+            hipLaunchKernelGGL(k_nl_check<true>, dim3(grid_for(N, 256)), dim3(256), 0, s, N, crd,
+                               nl->d_old_crd, c, r, permit2, nl->d_need);
+        else
+            hipLaunchKernelGGL(k_nl_check<false>, dim3(grid_for(N, 256)), dim3(256), 0, s, N, crd,
+                               nl->d_old_crd, c, r, permit2, nl->d_need);
+    } else {
+        hipLaunchKernelGGL(k_nl_set, dim3(1), dim3(1), 0, s, nl->d_need,
+                           ((step + 1) % nl->opt.refresh_interval == 0) ? 1 : 0);
+    }
+    SA_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_nl_bin, dim3(grid_for(N, 256)), dim3(256), 0, s, nl->d_need, N, crd,
+                       nl->d_old_crd, auto_refresh ? 1 : 0, c, nl->grid_length, nl->Nx, nl->Ny,
+                       nl->Nz, nl->d_cell_count, nl->d_atom_cell, nl->d_atom_slot);
+    SA_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_nl_scan, dim3(1), dim3(1024), 0, s, nl->d_need, nl->G, nl->d_cell_count,
+                       nl->d_cell_start, nl->opt.max_atom_in_grid_numbers, nl->d_flags, nl->d_rebuilds);
+    SA_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_nl_scatter, dim3(grid_for(N, 256)), dim3(256), 0, s, nl->d_need, N, crd,
+                       atom_local, nl->d_atom_cell, nl->d_atom_slot, nl->d_cell_start, nl->d_sidx,
+                       nl->d_rank, nl->d_spos);
+    SA_LAUNCH_CHECK();
+    const int waves_per_block = 4;
+    dim3 grid(grid_for(N, waves_per_block)), block(waves_per_block * kWave);
+    ScopedTimer timer("nl_search", s);
+    if (nl->ortho)
+        hipLaunchKernelGGL(k_nl_search<true>, grid, block, 0, s, nl->d_need, N, nl->d_spos,
+                           nl->d_sidx, nl->d_atom_cell, nl->d_cell_start, nl->d_ranges,
+                           nl->d_nranges, nl->max_ranges, c, r, nl->thr, excluded_list_start,
+                           excluded_list, excluded_numbers, nl->opt.max_neighbor_numbers,
+                           nl->d_list_count, nl->d_list, nl->d_flags);
+    else
+        hipLaunchKernelGGL(k_nl_search<false>, grid, block, 0, s, nl->d_need, N, nl->d_spos,
+                           nl->d_sidx, nl->d_atom_cell, nl->d_cell_start, nl->d_ranges,
+                           nl->d_nranges, nl->max_ranges, c, r, nl->thr, excluded_list_start,
+                           excluded_list, excluded_numbers, nl->opt.max_neighbor_numbers,
+                           nl->d_list_count, nl->d_list, nl->d_flags);
+    SA_LAUNCH_CHECK();
+    nl->built = 1;
+    return 0;
+}
+
+extern "C" int sponge_nl_check_overflow(sponge_nl* nl, int* flags) {
+    if (!nl) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    int h = 0;
+    SA_CHECK(hipMemcpyAsync(&h, nl->d_flags, sizeof(int), hipMemcpyDeviceToHost, nl->stream));
+    SA_CHECK(hipStreamSynchronize(nl->stream));
+    if (h) {
+        int rc = fill_async(nl->d_flags, 0, 1, nl->stream);
+        if (rc) return rc;
+    }
+    if (flags) *flags = h;
+    if (h && nl->opt.throw_error_when_overflow) return SPONGE_AMD_ERROR_OVERFLOW;
+    return 0;
+}
+
+extern "C" int sponge_nl_export_half_list(sponge_nl* nl, int* counts, int* serial,
+                                          int max_numbers) {
+    if (!nl || !counts || !serial || max_numbers <= 0) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    if (!nl->built) return SPONGE_AMD_ERROR_MISSING_COMMAND;
+    dim3 grid(grid_for(nl->N, 4)), block(256);
+    hipLaunchKernelGGL(k_nl_export, grid, block, 0, nl->stream, nl->N, nl->d_sidx,
+                       nl->d_list_count, nl->d_list, nl->opt.max_neighbor_numbers, counts, serial,
+                       max_numbers, (sa_atom_group*)nullptr, nl->d_flags);
+    SA_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int sponge_nl_atom_group(sponge_nl* nl, void** d_nl) {
+    if (!nl || !d_nl) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    if (!nl->built) return SPONGE_AMD_ERROR_MISSING_COMMAND;
+    const int M = nl->opt.max_neighbor_numbers;
+    if (!nl->d_exp_serial) {
+        int e = dmalloc(&nl->d_exp_serial, (size_t)nl->N * M);
+        e |= dmalloc(&nl->d_exp_count, nl->N);
+        e |= dmalloc(&nl->d_atom_group, nl->N);
+        if (e) return SPONGE_AMD_ERROR_MALLOC_FAILED;
+    }
+    dim3 grid(grid_for(nl->N, 4)), block(256);
+    hipLaunchKernelGGL(k_nl_export, grid, block, 0, nl->stream, nl->N, nl->d_sidx,
+                       nl->d_list_count, nl->d_list, M, nl->d_exp_count, nl->d_exp_serial, M,
+                       nl->d_atom_group, nl->d_flags);
+    SA_LAUNCH_CHECK();
+    *d_nl = nl->d_atom_group;
+    return 0;
+}
+
+extern "C" int sponge_nl_info(sponge_nl* nl, long long* list_entries, int* grid_dims) {
+    if (!nl) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    if (grid_dims) {
+        grid_dims[0] = nl->Nx;
+        grid_dims[1] = nl->Ny;
+        grid_dims[2] = nl->Nz;
+    }
+    if (list_entries) {
+        unsigned long long* d = nullptr;
+        if (dmalloc(&d, 1)) return SPONGE_AMD_ERROR_MALLOC_FAILED;
+        unsigned long long h = 0;
+        int rc = fill_async(d, 0ull, 1, nl->stream);
+        if (!rc) {
+            hipLaunchKernelGGL(k_nl_sum_counts, dim3(256), dim3(256), 0, nl->stream, nl->N,
+                               nl->d_list_count, d);
+            hipError_t he = hipGetLastError();
+            if (he == hipSuccess) he = hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, nl->stream);
+            if (he == hipSuccess) he = hipStreamSynchronize(nl->stream);
+            rc = (int)he;
+        }
+        dfree(d);
+        if (rc) return rc;
+        *list_entries = (long long)h;
+    }
+    return 0;
+}
+
+extern "C" int sponge_nl_rebuild_count(sponge_nl* nl, long long* count) {
+    if (!nl || !count) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    unsigned long long h = 0;
+    SA_CHECK(hipMemcpyAsync(&h, nl->d_rebuilds, sizeof(h), hipMemcpyDeviceToHost, nl->stream));
+    SA_CHECK(hipStreamSynchronize(nl->stream));
+    *count = (long long)h;
+    return 0;
+}
+
+extern "C" int sponge_nl_clear(sponge_nl* nl) {
+    if (!nl) return 0;
+    (void)hipStreamSynchronize(nl->stream);
+    nl_free(nl);
+    delete nl;
+    return 0;
+}
+
+// Gather of the current positions, charges and LJ types into the sorted
+// order of the last build (the role of Copy_Crd_And_Charge_To_New_Crd,
+// Lennard_Jones_force.cpp:17-27), clearing the sorted force accumulator.
+__global__ void k_nl_pack(int N, const int* __restrict__ sidx, const float* __restrict__ crd,
+                          const float* __restrict__ charge, const int* __restrict__ type,
+                          float4* __restrict__ pos, int* __restrict__ ptype,
+                          float4* __restrict__ facc) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N) return;
+    int i = sidx[p];
+    pos[p] = make_float4(crd[3 * i], crd[3 * i + 1], crd[3 * i + 2], charge[i]);
+    ptype[p] = type[i];
+    facc[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+int sa::nl_pack_for_pairs(sponge_nl* nl, const float* crd, const float* charge, const int* type,
+                          float4* pos, int* ptype, float4* facc, hipStream_t s) {
+    hipLaunchKernelGGL(k_nl_pack, dim3(grid_for(nl->N, 256)), dim3(256), 0, s, nl->N, nl->d_sidx,
+                       crd, charge, type, pos, ptype, facc);
+    SA_LAUNCH_CHECK();
+    return 0;
+}

@@ -1,0 +1,1040 @@
+// PME reciprocal part and excluded-pair correction for gfx950.
+//
+// Reference: SPONGE/PM_force/PM_force.cpp.
+//   Initial          :295-654   beta, grid, B-spline moduli, BC / virial_BC
+//   PME_Atom_Near    :699-762   fractional coordinates, base point u, f
+//   PME_Q_Spread     :765-814   order-4 cardinal B-spline charge spreading
+//   R2C, PME_Sum_Virial, PME_BCFQ, C2R   :1102-1125
+//   PME_Final        :829-893   force gather, device_add_force :1014-1022
+//   energies         :1142-1166
+//   MPI_PME_Excluded_Force_With_Atom_Energy_Correction :1745-1817
+//
+// The 3D FFT is our own: five LDS-staged passes of a mixed-radix (4,2,3,5,7)
+// Stockham transform.  The complex half spectrum is kept as [kz][x][y] so
+// every pass reads and writes whole 128-byte segments:
+//   z-R2C (two real lines per complex transform) -> y-FFT ->
+//   x-FFT + BC multiply + energy / virial sums + inverse x-FFT ->
+//   inverse y-FFT -> z-C2R (two real lines per complex transform).
+// The stencil indices of PME_Atom_Near are recomputed on the fly in spread
+// and gather (they are a pure function of the coordinates), so no 64-int
+// per-atom table is stored.
+#include <cmath>
+#include <vector>
+
+#include "internal.h"
+#include "timing.h"
+
+using namespace sa;
+
+namespace {
+
+constexpr float kPI = 3.1415926f;                      // PM_force.cpp:17
+constexpr float kTwoOverSqrtPi = 1.1283791670218446f;  // PM_force.cpp:19
+
+__constant__ float cMa[4] = {(float)(1.0 / 6.0), -0.5f, 0.5f, (float)(-1.0 / 6.0)};
+__constant__ float cMb[4] = {0.f, 0.5f, -1.f, 0.5f};
+__constant__ float cMc[4] = {0.f, 0.5f, 0.f, -0.5f};
+__constant__ float cMd[4] = {0.f, (float)(1.0 / 6.0), (float)(4.0 / 6.0), (float)(1.0 / 6.0)};
+__constant__ float cdMa[4] = {0.5f, -1.5f, 1.5f, -0.5f};
+__constant__ float cdMb[4] = {0.f, 1.f, -2.f, 1.f};
+__constant__ float cdMc[4] = {0.f, 0.5f, 0.f, -0.5f};
+
+// ---------------------------------------------------------------------------
+// complex helpers
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cscale(float s, float2 a) { return make_float2(s * a.x, s * a.y); }
+// multiply by SIGN * i
+template <int SIGN>
+__device__ __forceinline__ float2 mul_si(float2 a) {
+    return SIGN < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+}
+
+// Small DFTs X_p = sum_q v_q exp(SIGN 2 pi i p q / R)
+template <int R, int SIGN>
+struct Dft;
+
+template <int SIGN>
+struct Dft<2, SIGN> {
+    __device__ __forceinline__ static void run(float2* v) {
+        float2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    }
+};
+
+template <int SIGN>
+struct Dft<4, SIGN> {
+    __device__ __forceinline__ static void run(float2* v) {
+        float2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+        float2 t2 = cadd(v[1], v[3]), t3 = mul_si<SIGN>(csub(v[1], v[3]));
+        v[0] = cadd(t0, t2);
+        v[2] = csub(t0, t2);
+        v[1] = cadd(t1, t3);
+        v[3] = csub(t1, t3);
+    }
+};
+
+template <int SIGN>
+struct Dft<3, SIGN> {
+    __device__ __forceinline__ static void run(float2* v) {
+        const float h = 0.86602540378443864676f;  // sqrt(3)/2
+        float2 s = cadd(v[1], v[2]);
+        float2 d = mul_si<SIGN>(csub(v[1], v[2]));
+        float2 m = csub(v[0], cscale(0.5f, s));
+        v[0] = cadd(v[0], s);
+        v[1] = cadd(m, cscale(h, d));
+        v[2] = csub(m, cscale(h, d));
+    }
+};
+
+// cos / sin of 2 pi k / R for the R = 5, 7 butterflies
+template <int R>
+struct Roots;
+template <>
+struct Roots<5> {
+    static constexpr float c[5] = {1.000000000f, 0.309016994f, -0.809016994f, -0.809016994f,
+                                   0.309016994f};
+    static constexpr float s[5] = {0.000000000f, 0.951056516f, 0.587785252f, -0.587785252f,
+                                   -0.951056516f};
+};
+template <>
+struct Roots<7> {
+    static constexpr float c[7] = {1.000000000f,  0.623489802f,  -0.222520934f, -0.900968868f,
+                                   -0.900968868f, -0.222520934f, 0.623489802f};
+    static constexpr float s[7] = {0.000000000f,  0.781831482f,  0.974927912f, 0.433883739f,
+                                   -0.433883739f, -0.974927912f, -0.781831482f};
+};
+
+// R = 5, 7: direct small DFT
+template <int R, int SIGN>
+struct Dft {
+    __device__ __forceinline__ static void run(float2* v) {
+        float cs[R], sn[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            cs[k] = Roots<R>::c[k];
+            sn[k] = SIGN * Roots<R>::s[k];
+        }
+        float2 o[R];
+#pragma unroll
+        for (int p = 0; p < R; ++p) {
+            float2 acc = v[0];
+#pragma unroll
+            for (int q = 1; q < R; ++q) {
+                const int k = (p * q) % R;
+                acc = cadd(acc, cmul(v[q], make_float2(cs[k], sn[k])));
+            }
+            o[p] = acc;
+        }
+#pragma unroll
+        for (int p = 0; p < R; ++p) v[p] = o[p];
+    }
+};
+
+// One radix-R Stockham stage over nl lines of length n (line stride ld) held
+// in LDS: out[(j/Ns)*Ns*R + j%Ns + r*Ns] = DFT_R(in[j + r*n/R] * w^(r*(j%Ns))).
+template <int R, int SIGN>
+__device__ __forceinline__ void stockham_stage(const float2* a, float2* b, int nl, int n, int ld,
+                                               int Ns, const float2* __restrict__ tw) {
+    const int nb = n / R;
+    const int total = nl * nb;
+    const int step = n / (Ns * R);
+    for (int t = threadIdx.x; t < total; t += blockDim.x) {
+        const int line = t / nb, j = t - line * nb;
+        const float2* in = a + line * ld;
+        float2* out = b + line * ld;
+        float2 v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = in[j + r * nb];
+        const int k = j % Ns;
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+            float2 w = tw[r * k * step];
+            if (SIGN > 0) w.y = -w.y;
+            v[r] = cmul(v[r], w);
+        }
+        Dft<R, SIGN>::run(v);
+        const int idx = (j / Ns) * Ns * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) out[idx + r * Ns] = v[r];
+    }
+}
+
+// Transform nl lines in place (result returned; ping-pong between a and b).
+template <int SIGN>
+__device__ float2* fft_lines(float2* a, float2* b, int nl, int n, int ld, const sa_radix_plan& pl,
+                             const float2* __restrict__ tw) {
+    int Ns = 1;
+    for (int s = 0; s < pl.nst; ++s) {
+        const int R = pl.r[s];
+        switch (R) {
+            case 4: stockham_stage<4, SIGN>(a, b, nl, n, ld, Ns, tw); break;
+            case 2: stockham_stage<2, SIGN>(a, b, nl, n, ld, Ns, tw); break;
+            case 3: stockham_stage<3, SIGN>(a, b, nl, n, ld, Ns, tw); break;
+            case 5: stockham_stage<5, SIGN>(a, b, nl, n, ld, Ns, tw); break;
+            default: stockham_stage<7, SIGN>(a, b, nl, n, ld, Ns, tw); break;
+        }
+        __syncthreads();
+        float2* t = a;
+        a = b;
+        b = t;
+        Ns *= R;
+    }
+    return a;
+}
+
+// z pass, real -> half spectrum: block = (y block, x)
+__global__ void __launch_bounds__(256) k_fft_z_r2c(const float* __restrict__ Q,
+                                                   float2* __restrict__ FQ, int fx, int fy, int fz,
+                                                   int fzc, int TL, sa_radix_plan pl,
+                                                   const float2* __restrict__ tw) {
+    extern __shared__ float2 sm[];
+    const int x = blockIdx.y, y0 = blockIdx.x * TL;
+    const int nl = min(TL, fy - y0), npair = (nl + 1) >> 1;
+    float2* a = sm;
+    float2* b = sm + npair * fz;
+    const float* src = Q + ((size_t)x * fy + y0) * fz;
+    for (int e = threadIdx.x; e < npair * fz; e += blockDim.x) {
+        const int m = e / fz, z = e - m * fz;
+        const float v0 = src[(size_t)(2 * m) * fz + z];
+        const float v1 = (2 * m + 1 < nl) ? src[(size_t)(2 * m + 1) * fz + z] : 0.f;
+        a[e] = make_float2(v0, v1);
+    }
+    __syncthreads();
+    const float2* res = fft_lines<-1>(a, b, npair, fz, fz, pl, tw);
+    for (int e = threadIdx.x; e < fzc * nl; e += blockDim.x) {
+        const int kz = e / nl, l = e - kz * nl;
+        const int m = l >> 1;
+        const float2 Z = res[m * fz + kz];
+        const float2 Zm = res[m * fz + (kz == 0 ? 0 : fz - kz)];
+        float2 v;
+        if ((l & 1) == 0) v = make_float2(0.5f * (Z.x + Zm.x), 0.5f * (Z.y - Zm.y));
+        else v = make_float2(0.5f * (Z.y + Zm.y), -0.5f * (Z.x - Zm.x));
+        FQ[((size_t)kz * fx + x) * fy + y0 + l] = v;
+    }
+}
+
+// y pass over contiguous lines [kz][x][0..fy)
+template <int SIGN>
+__global__ void __launch_bounds__(256) k_fft_y(float2* __restrict__ FQ, int nlines, int fy, int TL,
+                                               sa_radix_plan pl, const float2* __restrict__ tw) {
+    extern __shared__ float2 sm[];
+    const int l0 = blockIdx.x * TL;
+    const int nl = min(TL, nlines - l0);
+    float2* a = sm;
+    float2* b = sm + TL * fy;
+    float2* base = FQ + (size_t)l0 * fy;
+    for (int e = threadIdx.x; e < nl * fy; e += blockDim.x) a[e] = base[e];
+    __syncthreads();
+    const float2* res = fft_lines<SIGN>(a, b, nl, fy, fy, pl, tw);
+    for (int e = threadIdx.x; e < nl * fy; e += blockDim.x) base[e] = res[e];
+}
+
+// x pass with the reciprocal-space convolution: block = (y block, kz).
+// acc[0] += sum w(kz) BC |F|^2 (w = 1 on kz = 0 and kz = fz/2 for even fz,
+// 2 otherwise: the full-spectrum sum of the reference's real-space
+// 0.5 * sum Q * FBCFQ, PM_force.cpp:896-919); acc[1..6] -= factor |F|^2
+// virial_BC with factor 0.5 on kz = 0 and kz = fzc-1 (PME_Sum_Virial,
+// :1024-1068, including its odd-fz weighting).
+__global__ void __launch_bounds__(256) k_fft_x_conv(
+    float2* __restrict__ FQ, const float* __restrict__ BCt, const float* __restrict__ vBCt,
+    long long nfft, int fx, int fy, int fz, int fzc, int TL, sa_radix_plan pl,
+    const float2* __restrict__ tw, int need_virial, double* __restrict__ acc) {
+    extern __shared__ float2 sm[];
+    __shared__ double red[7][4];
+    const int kz = blockIdx.y, y0 = blockIdx.x * TL;
+    const int nl = min(TL, fy - y0);
+    float2* a = sm;
+    float2* b = sm + TL * fx;
+    for (int e = threadIdx.x; e < fx * nl; e += blockDim.x) {
+        const int xi = e / nl, l = e - xi * nl;
+        a[l * fx + xi] = FQ[((size_t)kz * fx + xi) * fy + y0 + l];
+    }
+    __syncthreads();
+    float2* res = fft_lines<-1>(a, b, nl, fx, fx, pl, tw);
+    float2* other = (res == a) ? b : a;
+    const double wE = (kz == 0 || 2 * kz == fz) ? 1.0 : 2.0;
+    const float vfac = (kz == 0 || kz == fzc - 1) ? 0.5f : 1.0f;
+    double e_acc = 0.0;
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    for (int e = threadIdx.x; e < nl * fx; e += blockDim.x) {
+        const int l = e / fx, kx = e - l * fx;
+        const size_t ti = ((size_t)kz * fy + y0 + l) * fx + kx;
+        const float2 F = res[e];
+        const float bc = BCt[ti];
+        const float mag = F.x * F.x + F.y * F.y;
+        e_acc += wE * (double)bc * (double)mag;
+        if (need_virial) {
+            const float s = vfac * mag;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) v[c] -= (double)(s * vBCt[(size_t)c * nfft + ti]);
+        }
+        res[e] = make_float2(F.x * bc, F.y * bc);
+    }
+    __syncthreads();
+    const float2* out = fft_lines<+1>(res, other, nl, fx, fx, pl, tw);
+    for (int e = threadIdx.x; e < fx * nl; e += blockDim.x) {
+        const int xi = e / nl, l = e - xi * nl;
+        FQ[((size_t)kz * fx + xi) * fy + y0 + l] = out[l * fx + xi];
+    }
+    // block reduction of the energy and virial sums
+    e_acc = wave_sum(e_acc);
+    if (need_virial)
+        for (int c = 0; c < 6; ++c) v[c] = wave_sum(v[c]);
+    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    if (lane == 0) {
+        red[0][w] = e_acc;
+        for (int c = 0; c < 6; ++c) red[1 + c][w] = v[c];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = blockDim.x / kWave;
+        double s0 = 0;
+        for (int k = 0; k < nw; ++k) s0 += red[0][k];
+        unsafeAtomicAdd(&acc[0], s0);
+        if (need_virial)
+            for (int c = 0; c < 6; ++c) {
+                double s = 0;
+                for (int k = 0; k < nw; ++k) s += red[1 + c][k];
+                unsafeAtomicAdd(&acc[1 + c], s);
+            }
+    }
+}
+
+// z pass, half spectrum -> real (unnormalised C2R): block = (y block, x)
+__global__ void __launch_bounds__(256) k_fft_z_c2r(const float2* __restrict__ FQ,
+                                                   float* __restrict__ out, int fx, int fy, int fz,
+                                                   int fzc, int TL, sa_radix_plan pl,
+                                                   const float2* __restrict__ tw) {
+    extern __shared__ float2 sm[];
+    const int x = blockIdx.y, y0 = blockIdx.x * TL;
+    const int nl = min(TL, fy - y0), npair = (nl + 1) >> 1;
+    float2* h = sm;                       // [nl][fzc]
+    float2* a = sm + TL * fzc;            // [npair][fz]
+    float2* b = a + ((TL + 1) >> 1) * fz;
+    for (int e = threadIdx.x; e < fzc * nl; e += blockDim.x) {
+        const int kz = e / nl, l = e - kz * nl;
+        float2 v = FQ[((size_t)kz * fx + x) * fy + y0 + l];
+        // the imaginary parts of the DC and Nyquist terms do not enter a
+        // real-to-complex inverse
+        if (kz == 0 || 2 * kz == fz) v.y = 0.f;
+        h[l * fzc + kz] = v;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < npair * fz; e += blockDim.x) {
+        const int m = e / fz, k = e - m * fz;
+        float2 A, B = make_float2(0.f, 0.f);
+        if (k < fzc) A = h[(2 * m) * fzc + k];
+        else {
+            A = h[(2 * m) * fzc + fz - k];
+            A.y = -A.y;
+        }
+        if (2 * m + 1 < nl) {
+            if (k < fzc) B = h[(2 * m + 1) * fzc + k];
+            else {
+                B = h[(2 * m + 1) * fzc + fz - k];
+                B.y = -B.y;
+            }
+        }
+        a[e] = make_float2(A.x - B.y, A.y + B.x);
+    }
+    __syncthreads();
+    const float2* res = fft_lines<+1>(a, b, npair, fz, fz, pl, tw);
+    float* dst = out + ((size_t)x * fy + y0) * fz;
+    for (int e = threadIdx.x; e < nl * fz; e += blockDim.x) {
+        const int l = e / fz, z = e - l * fz;
+        const float2 v = res[(l >> 1) * fz + z];
+        dst[e] = (l & 1) ? v.y : v.x;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// PME_Atom_Near (PM_force.cpp:706-733): base point u and fraction f
+struct Near {
+    int ux, uy, uz;
+    float fx, fy, fz;
+};
+
+__device__ __forceinline__ void near_axis(float frac, int n, int& u, float& f) {
+    frac *= n;
+    int t = (int)frac;
+    t = t < 0 ? 0 : (t < n ? t : n - 1);
+    u = t;
+    f = frac - t;
+    f = f - floorf(f);
+}
+
+__device__ __forceinline__ Near atom_near(const float* __restrict__ crd, int i, const LT6& rc,
+                                          int fx, int fy, int fz) {
+    const float x = crd[3 * i], y = crd[3 * i + 1], z = crd[3 * i + 2];
+    // VECTOR * LTMatrix3 then frac - floorf(frac)
+    float sx = x * rc.a11 + y * rc.a21 + z * rc.a31;
+    float sy = y * rc.a22 + z * rc.a32;
+    float sz = z * rc.a33;
+    sx = sx - floorf(sx);
+    sy = sy - floorf(sy);
+    sz = sz - floorf(sz);
+    if (!isfinite(sx) || !isfinite(sy) || !isfinite(sz)) sx = sy = sz = 0.f;
+    Near o;
+    near_axis(sx, fx, o.ux, o.fx);
+    near_axis(sy, fy, o.uy, o.fy);
+    near_axis(sz, fz, o.uz, o.fz);
+    return o;
+}
+
+__device__ __forceinline__ float bsp_w(int k, float f) {
+    const float f2 = f * f;
+    return cMa[k] * f * f2 + cMb[k] * f2 + cMc[k] * f + cMd[k];
+}
+__device__ __forceinline__ float bsp_dw(int k, float f) {
+    const float f2 = f * f;
+    return cdMa[k] * f2 + cdMb[k] * f + cdMc[k];
+}
+
+// grid index of stencil point (kx, ky, kz) below u (PM_force.cpp:737-759)
+__device__ __forceinline__ int stencil_index(const Near& nr, int kx, int ky, int kz, int fx,
+                                             int fy, int fz) {
+    int x = nr.ux - kx;
+    if (x < 0) x += fx;
+    int y = nr.uy - ky;
+    if (y < 0) y += fy;
+    int z = nr.uz - kz;
+    if (z < 0) z += fz;
+    return (x * fy + y) * fz + z;
+}
+
+// One wavefront per atom, one lane per stencil point (lane = 16 kx + 4 ky + kz
+// as in PME_Q_Spread's k loop, :787-812).
+__global__ void __launch_bounds__(256) k_pme_spread(int N, const float* __restrict__ crd,
+                                                    const float* __restrict__ charge, LT6 rc,
+                                                    int fx, int fy, int fz,
+                                                    float* __restrict__ Q) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int kx = lane >> 4, ky = (lane >> 2) & 3, kz = lane & 3;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int nwaves = gridDim.x * blockDim.x / kWave;
+    for (int i = wave; i < N; i += nwaves) {
+        const Near nr = atom_near(crd, i, rc, fx, fy, fz);
+        float tq = charge[i] * bsp_w(kx, nr.fx);
+        tq = tq * bsp_w(ky, nr.fy);
+        tq = tq * bsp_w(kz, nr.fz);
+        unsafeAtomicAdd(&Q[stencil_index(nr, kx, ky, kz, fx, fy, fz)], tq);
+    }
+}
+
+// PME_Final + device_add_force: frc += update_interval * F (PM_force.cpp:
+// 852-892, 1014-1022), one wavefront per atom.
+__global__ void __launch_bounds__(256) k_pme_gather(int N, const float* __restrict__ crd,
+                                                    const float* __restrict__ charge, LT6 rc,
+                                                    int fx, int fy, int fz,
+                                                    const float* __restrict__ phi,
+                                                    float update_interval,
+                                                    float* __restrict__ frc) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int kx = lane >> 4, ky = (lane >> 2) & 3, kz = lane & 3;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int nwaves = gridDim.x * blockDim.x / kWave;
+    for (int i = wave; i < N; i += nwaves) {
+        const Near nr = atom_near(crd, i, rc, fx, fy, fz);
+        const float dqf = -phi[stencil_index(nr, kx, ky, kz, fx, fy, fz)] * charge[i];
+        const float wx = bsp_w(kx, nr.fx), wy = bsp_w(ky, nr.fy), wz = bsp_w(kz, nr.fz);
+        const float dx = bsp_dw(kx, nr.fx), dy = bsp_dw(ky, nr.fy), dz = bsp_dw(kz, nr.fz);
+        const float gx = dx * wy * wz * fx, gy = dy * wx * wz * fy, gz = dz * wx * wy * fz;
+        // MultiplyTranspose(g, rcell) = {gx a11, gx a21 + gy a22, gx a31 + gy a32 + gz a33}
+        float ox = dqf * (gx * rc.a11);
+        float oy = dqf * (gx * rc.a21 + gy * rc.a22);
+        float oz = dqf * (gx * rc.a31 + gy * rc.a32 + gz * rc.a33);
+        ox = wave_sum(ox);
+        oy = wave_sum(oy);
+        oz = wave_sum(oz);
+        if (lane == 0) {
+            frc[3 * i] = frc[3 * i] + update_interval * ox;
+            frc[3 * i + 1] = frc[3 * i + 1] + update_interval * oy;
+            frc[3 * i + 2] = frc[3 * i + 2] + update_interval * oz;
+        }
+    }
+}
+
+__global__ void k_pme_charge_sums(int N, const float* __restrict__ q, double* __restrict__ acc) {
+    double s2 = 0, s1 = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        const float v = q[i];
+        s2 += (double)(v * v);
+        s1 += (double)v;
+    }
+    s2 = wave_sum(s2);
+    s1 = wave_sum(s1);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        unsafeAtomicAdd(&acc[7], s2);
+        unsafeAtomicAdd(&acc[8], s1);
+    }
+}
+
+// Self, neutralizing and reciprocal energies added to the potential
+// (PM_force.cpp:1142-1166).
+__global__ void k_pme_energy_final(const double* __restrict__ acc, float* __restrict__ ene,
+                                   float* __restrict__ potential, float self_factor,
+                                   float neutralizing_factor) {
+    const float rec = (float)acc[0] * 0.5f;
+    float self = (float)acc[7];
+    self = self * self_factor;
+    const float sumq = (float)acc[8];
+    self += neutralizing_factor * sumq * sumq;
+    ene[0] = rec;
+    ene[1] = self;
+    if (potential) potential[0] += self + rec;
+}
+
+__global__ void k_pme_virial_add(const double* __restrict__ acc, float* __restrict__ virial) {
+    const int c = threadIdx.x;
+    if (c < 6) virial[c] += (float)acc[1 + c];
+}
+
+// g(x) = 2/sqrt(pi) x e^{-x^2} - erf(x)
+//      = 2/sqrt(pi) sum_{n>=1} (-1)^n 2n / ((2n+1) n!) x^{2n+1},
+// series below x = 0.75 (truncation < 1e-9 relative), direct form above.
+__device__ __forceinline__ float excl_bracket(float x) {
+    if (x < 0.75f) {
+        const float x2 = x * x;
+        float s = 2.62450659e-7f;         // n = 10
+        s = s * x2 - 2.61069340e-6f;      // n = 9
+        s = s * x2 + 2.33426704e-5f;      // n = 8
+        s = s * x2 - 1.85185185e-4f;      // n = 7
+        s = s * x2 + 1.28205128e-3f;      // n = 6
+        s = s * x2 - 7.57575758e-3f;      // n = 5
+        s = s * x2 + 3.70370370e-2f;      // n = 4
+        s = s * x2 - 1.42857143e-1f;      // n = 3
+        s = s * x2 + 4.00000000e-1f;      // n = 2
+        s = s * x2 - 6.66666667e-1f;      // n = 1
+        return kTwoOverSqrtPi * s * x2 * x;
+    }
+    return x * kTwoOverSqrtPi * expf(-x * x) - erff(x);
+}
+
+// MPI_PME_Excluded_Force_With_Atom_Energy_Correction with factor 1
+// (PM_force.cpp:1744-1817), one thread per local atom.
+template <bool ORTHO>
+__global__ void k_pme_excluded(int N, const float* __restrict__ crd,
+                               const float* __restrict__ charge, LT6 c, LT6 r, float beta,
+                               const int* __restrict__ ex_start, const int* __restrict__ ex_list,
+                               const int* __restrict__ ex_num, float* __restrict__ frc,
+                               int need_e, float* __restrict__ atom_ene,
+                               float* __restrict__ corr_ene, int need_v,
+                               float* __restrict__ atom_virial) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int n = ex_num[i];
+    if (n <= 0) return;
+    const int s = ex_start[i];
+    const float qi = charge[i];
+    const float3 r1 = make_float3(crd[3 * i], crd[3 * i + 1], crd[3 * i + 2]);
+    float fx = 0, fy = 0, fz = 0, e = 0;
+    float v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
+    for (int k = s; k < s + n; ++k) {
+        const int j = ex_list[k];
+        const float3 r2 = make_float3(crd[3 * j], crd[3 * j + 1], crd[3 * j + 2]);
+        const float qj = charge[j];
+        const float3 d = disp_pbc<ORTHO>(r2, r1, c, r);
+        const float dr2 = norm2_exact(d);
+        const float dr_abs = sqrtf(dr2);
+        const float br = beta * dr_abs;
+        // (2/sqrt(pi) x e^{-x^2} + erfc(x) - 1) / r^3; the bracket cancels to
+        // a few percent of its terms for bonded distances, so it is summed as
+        // its odd Taylor series there
+        float fa = excl_bracket(br) / dr2 / dr_abs;
+        fa = -qi * qj * fa;
+        const float lx = fa * d.x, ly = fa * d.y, lz = fa * d.z;
+        unsafeAtomicAdd(&frc[3 * j], -lx);
+        unsafeAtomicAdd(&frc[3 * j + 1], -ly);
+        unsafeAtomicAdd(&frc[3 * j + 2], -lz);
+        fx += lx;
+        fy += ly;
+        fz += lz;
+        if (need_e) e -= qi * qj * erff(br) / dr_abs;
+        if (need_v) {
+            v0 -= lx * d.x;
+            v1 -= lx * d.y + ly * d.x;
+            v2 -= ly * d.y;
+            v3 -= lx * d.z + lz * d.x;
+            v4 -= ly * d.z + lz * d.y;
+            v5 -= lz * d.z;
+        }
+    }
+    unsafeAtomicAdd(&frc[3 * i], fx);
+    unsafeAtomicAdd(&frc[3 * i + 1], fy);
+    unsafeAtomicAdd(&frc[3 * i + 2], fz);
+    if (need_e) {
+        unsafeAtomicAdd(&atom_ene[i], e);
+        corr_ene[i] = e;
+    }
+    if (need_v) {
+        float* o = atom_virial + (size_t)6 * i;
+        unsafeAtomicAdd(&o[0], v0);
+        unsafeAtomicAdd(&o[1], v1);
+        unsafeAtomicAdd(&o[2], v2);
+        unsafeAtomicAdd(&o[3], v3);
+        unsafeAtomicAdd(&o[4], v4);
+        unsafeAtomicAdd(&o[5], v5);
+    }
+}
+
+__global__ void k_sum2_f32(int n, const float* __restrict__ a, const float* __restrict__ b,
+                           double* out) {
+    double sa_ = 0, sb = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        sa_ += a[i];
+        sb += b[i];
+    }
+    sa_ = wave_sum(sa_);
+    sb = wave_sum(sb);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        unsafeAtomicAdd(&out[0], sa_);
+        unsafeAtomicAdd(&out[1], sb);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host set-up
+
+// getb (PM_force.cpp:42-64): |B(k)|^2 = 1 / |sum_kk M4(kk+1) e^{2 pi i k kk/N}|^2
+// evaluated in float with the reference's expc / divc (SPONGE/utils/fft.hpp:26-51).
+float bspline_M(float u, int n) {
+    if (n == 2) {
+        if (u > 2 || u < 0) return 0;
+        return 1 - fabsf(u - 1);
+    }
+    return u / (n - 1) * bspline_M(u, n - 1) + (n - u) / (n - 1) * bspline_M(u - 1, n - 1);
+}
+
+float getb(int k, int NFFT, int order) {
+    float t2r = 0, t2i = 0;
+    const float ang0 = 2 * (order - 1) * kPI * k / NFFT;
+    const float rr = cosf(ang0), ri = sinf(ang0);
+    for (int kk = 0; kk < order - 1; ++kk) {
+        const float ang = 2.0f * kPI * k / NFFT * kk;
+        const float tf = bspline_M((float)(kk + 1), order);
+        t2r += tf * cosf(ang);
+        t2i += tf * sinf(ang);
+    }
+    const float den = t2r * t2r + t2i * t2i;
+    const float re = (rr * t2r + ri * t2i) / den, im = (ri * t2r - rr * t2i) / den;
+    return re * re + im * im;
+}
+
+bool make_plan(int n, sa_radix_plan* p) {
+    p->n = n;
+    p->nst = 0;
+    int m = n;
+    for (int f : {4, 2, 3, 5, 7})
+        while (m % f == 0) {
+            if (p->nst >= 16) return false;
+            p->r[p->nst++] = f;
+            m /= f;
+        }
+    return m == 1;
+}
+
+int lines_per_block(int n) {
+    int t = 2048 / n;
+    if (t > 16) t = 16;
+    if (t < 2) t = 2;
+    return t & ~1;
+}
+
+void pme_free(sponge_pme* p) {
+    for (auto& t : p->d_tw) dfree(t);
+    dfree(p->d_Q);
+    dfree(p->d_FQ);
+    dfree(p->d_FBCFQ);
+    dfree(p->d_BCt);
+    dfree(p->d_vBCt);
+    dfree(p->d_acc);
+    dfree(p->d_ene);
+    dfree(p->d_direct_atom_energy);
+    dfree(p->d_correction_atom_energy);
+    dfree(p->d_sum);
+}
+
+}  // namespace
+
+extern "C" void sponge_pme_default_options(sponge_pme_options* o) {
+    o->fftx = o->ffty = o->fftz = -1;
+    o->grid_spacing = 1.0f;
+    o->direct_tolerance = 0.00001f;
+    o->update_interval = 1;
+    o->calculate_reciprocal_part = 1;
+    o->calculate_excluded_part = 1;
+}
+
+extern "C" float sponge_amd_get_beta(float cutoff, float tolerance) {
+    // Get_Beta (PM_force.cpp:246-274): float variables, double erfc
+    float beta = 0.f, low, high, tempf;
+    int ilow, ihigh;
+    high = 1.0;
+    ihigh = 1;
+    while (1) {
+        tempf = erfc(high * cutoff) / cutoff;
+        if (tempf <= tolerance) break;
+        high *= 2;
+        ihigh++;
+    }
+    ihigh += 50;
+    low = 0.0;
+    for (ilow = 1; ilow < ihigh; ilow++) {
+        beta = (low + high) / 2;
+        tempf = erfc(beta * cutoff) / cutoff;
+        if (tempf >= tolerance) low = beta;
+        else high = beta;
+    }
+    return beta;
+}
+
+static int check_2357(int n) {
+    if (n <= 0) return 0;
+    for (int f : {2, 3, 5, 7})
+        while (n % f == 0) n /= f;
+    return n == 1;
+}
+
+extern "C" int sponge_amd_get_fft_parameter(float length) {
+    // Get_Fft_Patameter (SPONGE/common.cpp:639-659)
+    int t = (int)ceil(length + 3) >> 2 << 2;
+    if (t >= 60 && t <= 68) t = 64;
+    else if (t >= 120 && t <= 136) t = 128;
+    else if (t >= 240 && t <= 272) t = 256;
+    else if (t >= 480 && t <= 544) t = 512;
+    else if (t >= 960 && t <= 1088) t = 1024;
+    while (!check_2357(t)) t += 4;
+    return t;
+}
+
+extern "C" int sponge_pme_initial(sponge_pme** out, int atom_numbers, const float* cell,
+                                  const float* rcell, float cutoff, const sponge_pme_options* opt,
+                                  void* stream) {
+    if (!out || !cell || !rcell || atom_numbers <= 0 || !(cutoff > 0))
+        return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    *out = nullptr;
+    sponge_pme* p = new sponge_pme();
+    if (opt) p->opt = *opt;
+    else sponge_pme_default_options(&p->opt);
+    p->N = atom_numbers;
+    p->cutoff = cutoff;
+    p->cell = lt_from(cell);
+    p->rcell = lt_from(rcell);
+    p->stream = (hipStream_t)stream;
+    if (!lt_is_orthorhombic(p->cell) || p->opt.update_interval <= 0 || !(p->opt.grid_spacing > 0)) {
+        delete p;
+        return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    }
+    // grid (PM_force.cpp:383-390): box_length / grid_spacing when not given
+    p->fx = p->opt.fftx > 0 ? p->opt.fftx : sponge_amd_get_fft_parameter(p->cell.a11 / p->opt.grid_spacing);
+    p->fy = p->opt.ffty > 0 ? p->opt.ffty : sponge_amd_get_fft_parameter(p->cell.a22 / p->opt.grid_spacing);
+    p->fz = p->opt.fftz > 0 ? p->opt.fftz : sponge_amd_get_fft_parameter(p->cell.a33 / p->opt.grid_spacing);
+    const int dims[3] = {p->fx, p->fy, p->fz};
+    for (int d = 0; d < 3; ++d) {
+        if (dims[d] < 4 || dims[d] > 512 || !make_plan(dims[d], &p->plan[d])) {
+            delete p;
+            return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+        }
+    }
+    const int fx = p->fx, fy = p->fy, fz = p->fz;
+    p->fzc = fz / 2 + 1;
+    p->nall = (long long)fx * fy * fz;
+    p->nfft = (long long)fx * fy * p->fzc;
+    p->volume = p->cell.a11 * p->cell.a22 * p->cell.a33;
+    p->beta = sponge_amd_get_beta(cutoff, p->opt.direct_tolerance);
+    p->neutralizing_factor = -0.5 * 3.1415926535897932f / (p->beta * p->beta * p->volume);
+    p->tl_z = lines_per_block(fz);
+    p->tl_y = lines_per_block(fy);
+    p->tl_x = lines_per_block(fx);
+
+    // BC and virial_BC (PM_force.cpp:545-611) in the [kz][ky][kx] order of
+    // the convolution pass
+    std::vector<float> B1(fx), B2(fy), B3(fz);
+    for (int k = 0; k < fx; ++k) B1[k] = getb(k, fx, 4);
+    for (int k = 0; k < fy; ++k) B2[k] = getb(k, fy, 4);
+    for (int k = 0; k < fz; ++k) B3[k] = getb(k, fz, 4);
+    std::vector<float> bct(p->nfft), vbct((size_t)6 * p->nfft);
+    const float mprefactor = kPI * kPI / p->beta / p->beta;
+    const LT6& rc = p->rcell;
+    for (int kx = 0; kx < fx; ++kx) {
+        const float kxrp = kx > fx / 2 ? kx - fx : kx;
+        for (int ky = 0; ky < fy; ++ky) {
+            const float kyrp = ky > fy / 2 ? ky - fy : ky;
+            for (int kz = 0; kz <= fz / 2; ++kz) {
+                const float kzrp = kz;
+                // m = MultiplyTranspose({kx, ky, kz}, rcell)
+                const float mx = kxrp * rc.a11;
+                const float my = kxrp * rc.a21 + kyrp * rc.a22;
+                const float mz = kxrp * rc.a31 + kyrp * rc.a32 + kzrp * rc.a33;
+                const float msq = mx * mx + my * my + mz * mz;
+                float bc, v[6];
+                if (kx + ky + kz == 0) {
+                    bc = 0;
+                    for (auto& t : v) t = 0;
+                } else {
+                    bc = (float)1.0 / kPI / msq * expf(-mprefactor * msq) / p->volume;
+                    const float t = 2 / msq * (1 + mprefactor * msq);
+                    v[0] = 1 - t * mx * mx;
+                    v[1] = 0 - t * my * mx;
+                    v[2] = 1 - t * my * my;
+                    v[3] = 0 - t * mz * mx;
+                    v[4] = 0 - t * mz * my;
+                    v[5] = 1 - t * mz * mz;
+                }
+                bc *= B1[kx] * B2[ky] * B3[kz];
+                const size_t ti = ((size_t)kz * fy + ky) * fx + kx;
+                bct[ti] = bc;
+                for (int c = 0; c < 6; ++c) vbct[(size_t)c * p->nfft + ti] = 0.5f * bc * v[c];
+            }
+        }
+    }
+    int e = 0;
+    for (int d = 0; d < 3; ++d) e |= dmalloc(&p->d_tw[d], dims[d]);
+    e |= dmalloc(&p->d_Q, p->nall);
+    e |= dmalloc(&p->d_FQ, p->nfft);
+    e |= dmalloc(&p->d_FBCFQ, p->nall);
+    e |= dmalloc(&p->d_BCt, p->nfft);
+    e |= dmalloc(&p->d_vBCt, (size_t)6 * p->nfft);
+    e |= dmalloc(&p->d_acc, 9);
+    e |= dmalloc(&p->d_ene, 2);
+    e |= dmalloc(&p->d_direct_atom_energy, p->N);
+    e |= dmalloc(&p->d_correction_atom_energy, p->N);
+    e |= dmalloc(&p->d_sum, 2);
+    if (e) {
+        pme_free(p);
+        delete p;
+        return SPONGE_AMD_ERROR_MALLOC_FAILED;
+    }
+    hipStream_t s = p->stream;
+    hipError_t he = hipSuccess;
+    for (int d = 0; d < 3 && he == hipSuccess; ++d) {
+        std::vector<float2> tw(dims[d]);
+        for (int t = 0; t < dims[d]; ++t) {
+            const double a = -2.0 * M_PI * (double)t / dims[d];
+            tw[t] = make_float2((float)cos(a), (float)sin(a));
+        }
+        he = hipMemcpyAsync(p->d_tw[d], tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice, s);
+        if (he == hipSuccess) he = hipStreamSynchronize(s);
+    }
+    if (he == hipSuccess)
+        he = hipMemcpyAsync(p->d_BCt, bct.data(), bct.size() * sizeof(float), hipMemcpyHostToDevice, s);
+    if (he == hipSuccess)
+        he = hipMemcpyAsync(p->d_vBCt, vbct.data(), vbct.size() * sizeof(float), hipMemcpyHostToDevice, s);
+    int fe = 0;
+    if (he == hipSuccess) {
+        fe |= fill_async(p->d_acc, 0.0, 9, s);
+        fe |= fill_async(p->d_ene, 0.f, 2, s);
+        fe |= fill_async(p->d_Q, 0.f, p->nall, s);
+        fe |= fill_async(p->d_FBCFQ, 0.f, p->nall, s);
+        fe |= fill_async(p->d_direct_atom_energy, 0.f, p->N, s);
+        fe |= fill_async(p->d_correction_atom_energy, 0.f, p->N, s);
+        he = hipStreamSynchronize(s);
+    }
+    if (he != hipSuccess || fe) {
+        pme_free(p);
+        delete p;
+        return he != hipSuccess ? (int)he : fe;
+    }
+    *out = p;
+    return 0;
+}
+
+extern "C" int sponge_pme_info(sponge_pme* p, float* beta, int* fft) {
+    if (!p) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    if (beta) *beta = p->beta;
+    if (fft) {
+        fft[0] = p->fx;
+        fft[1] = p->fy;
+        fft[2] = p->fz;
+    }
+    return 0;
+}
+
+extern "C" float* sponge_pme_direct_atom_energy(sponge_pme* p) {
+    return p ? p->d_direct_atom_energy : nullptr;
+}
+
+static int pme_fft_convolve(sponge_pme* p, int need_virial) {
+    hipStream_t s = p->stream;
+    const int fx = p->fx, fy = p->fy, fz = p->fz, fzc = p->fzc;
+    int rc = fill_async(p->d_acc, 0.0, 7, s);
+    if (rc) return rc;
+    {
+        const int TL = p->tl_z;
+        const size_t sh = (size_t)2 * ((TL + 1) / 2) * fz * sizeof(float2);
+        ScopedTimer timer("pme_fft_z_r2c", s);
+        hipLaunchKernelGGL(k_fft_z_r2c, dim3(grid_for(fy, TL), fx), dim3(256), sh, s, p->d_Q,
+                           p->d_FQ, fx, fy, fz, fzc, TL, p->plan[2], p->d_tw[2]);
+        SA_LAUNCH_CHECK();
+    }
+    {
+        const int TL = p->tl_y, nlines = fzc * fx;
+        const size_t sh = (size_t)2 * TL * fy * sizeof(float2);
+        ScopedTimer timer("pme_fft_y", s);
+        hipLaunchKernelGGL(k_fft_y<-1>, dim3(grid_for(nlines, TL)), dim3(256), sh, s, p->d_FQ,
+                           nlines, fy, TL, p->plan[1], p->d_tw[1]);
+        SA_LAUNCH_CHECK();
+    }
+    {
+        const int TL = p->tl_x;
+        const size_t sh = (size_t)2 * TL * fx * sizeof(float2);
+        ScopedTimer timer("pme_fft_x_conv", s);
+        hipLaunchKernelGGL(k_fft_x_conv, dim3(grid_for(fy, TL), fzc), dim3(256), sh, s, p->d_FQ,
+                           p->d_BCt, p->d_vBCt, p->nfft, fx, fy, fz, fzc, TL, p->plan[0],
+                           p->d_tw[0], need_virial, p->d_acc);
+        SA_LAUNCH_CHECK();
+    }
+    {
+        const int TL = p->tl_y, nlines = fzc * fx;
+        const size_t sh = (size_t)2 * TL * fy * sizeof(float2);
+        ScopedTimer timer("pme_ifft_y", s);
+        hipLaunchKernelGGL(k_fft_y<+1>, dim3(grid_for(nlines, TL)), dim3(256), sh, s, p->d_FQ,
+                           nlines, fy, TL, p->plan[1], p->d_tw[1]);
+        SA_LAUNCH_CHECK();
+    }
+    {
+        const int TL = p->tl_z;
+        const size_t sh = ((size_t)TL * fzc + (size_t)2 * ((TL + 1) / 2) * fz) * sizeof(float2);
+        ScopedTimer timer("pme_fft_z_c2r", s);
+        hipLaunchKernelGGL(k_fft_z_c2r, dim3(grid_for(fy, TL), fx), dim3(256), sh, s, p->d_FQ,
+                           p->d_FBCFQ, fx, fy, fz, fzc, TL, p->plan[2], p->d_tw[2]);
+        SA_LAUNCH_CHECK();
+    }
+    return 0;
+}
+
+extern "C" int sponge_pme_reciprocal_force(sponge_pme* p, const float* crd, const float* cell,
+                                           const float* rcell, const float* charge, float* frc,
+                                           int need_virial, int need_energy, float* d_virial,
+                                           float* d_potential, int step) {
+    if (!p || !crd || !cell || !rcell || !charge || !frc) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    if (!p->opt.calculate_reciprocal_part) return 0;
+    if ((need_virial && !d_virial) || (need_energy && !d_potential))
+        return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    const LT6 c = lt_from(cell), r = lt_from(rcell);
+    if (memcmp(&c, &p->cell, sizeof(LT6)) != 0) return SPONGE_AMD_ERROR_CONFLICTING_COMMAND;
+    hipStream_t s = p->stream;
+    const int N = p->N;
+    int rc;
+    if (step % p->opt.update_interval == 0) {
+        rc = fill_async(p->d_Q, 0.f, p->nall, s);
+        if (rc) return rc;
+        int blocks = grid_for(N, 4);
+        if (blocks > 16384) blocks = 16384;
+        {
+            ScopedTimer timer("pme_spread", s);
+            hipLaunchKernelGGL(k_pme_spread, dim3(blocks), dim3(256), 0, s, N, crd, charge, r,
+                               p->fx, p->fy, p->fz, p->d_Q);
+            SA_LAUNCH_CHECK();
+        }
+        rc = pme_fft_convolve(p, need_virial);
+        if (rc) return rc;
+        if (need_virial) {
+            hipLaunchKernelGGL(k_pme_virial_add, dim3(1), dim3(64), 0, s, p->d_acc, d_virial);
+            SA_LAUNCH_CHECK();
+        }
+        {
+            ScopedTimer timer("pme_gather", s);
+            hipLaunchKernelGGL(k_pme_gather, dim3(blocks), dim3(256), 0, s, N, crd, charge, r,
+                               p->fx, p->fy, p->fz, p->d_FBCFQ, (float)p->opt.update_interval,
+                               frc);
+            SA_LAUNCH_CHECK();
+        }
+    }
+    if (need_energy) {
+        rc = fill_async(p->d_acc + 7, 0.0, 2, s);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_pme_charge_sums, dim3(64), dim3(256), 0, s, N, charge, p->d_acc);
+        SA_LAUNCH_CHECK();
+        const float self_factor = -p->beta / sqrtf(kPI);
+        hipLaunchKernelGGL(k_pme_energy_final, dim3(1), dim3(1), 0, s, p->d_acc, p->d_ene,
+                           d_potential, self_factor, p->neutralizing_factor);
+        SA_LAUNCH_CHECK();
+    }
+    return 0;
+}
+
+extern "C" int sponge_pme_excluded_force(sponge_pme* p, int local_atom_numbers,
+                                         const int* atom_local, const int* atom_local_id,
+                                         const float* crd, const float* cell, const float* rcell,
+                                         const float* charge, const int* excluded_list_start,
+                                         const int* excluded_list, const int* excluded_numbers,
+                                         float* frc, int need_energy, float* atom_energy,
+                                         int need_virial, float* atom_virial) {
+    (void)atom_local;
+    (void)atom_local_id;
+    if (!p || !crd || !cell || !rcell || !charge || !frc) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    if (!p->opt.calculate_excluded_part) return 0;
+    if (!excluded_list_start || !excluded_list || !excluded_numbers)
+        return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    if ((need_energy && !atom_energy) || (need_virial && !atom_virial))
+        return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    if (local_atom_numbers <= 0) return 0;
+    if (local_atom_numbers > p->N) return SPONGE_AMD_ERROR_CONFLICTING_COMMAND;
+    hipStream_t s = p->stream;
+    if (need_energy) {
+        int rc = fill_async(p->d_correction_atom_energy, 0.f, local_atom_numbers, s);
+        if (rc) return rc;
+    }
+    const LT6 c = lt_from(cell), r = lt_from(rcell);
+    dim3 grid(grid_for(local_atom_numbers, 256)), block(256);
+    if (lt_is_orthorhombic(c))
+        hipLaunchKernelGGL(k_pme_excluded<true>, grid, block, 0, s, local_atom_numbers, crd, charge,
+                           c, r, p->beta, excluded_list_start, excluded_list, excluded_numbers, frc,
+                           need_energy, atom_energy, p->d_correction_atom_energy, need_virial,
+                           atom_virial);
+    else
+        hipLaunchKernelGGL(k_pme_excluded<false>, grid, block, 0, s, local_atom_numbers, crd,
+                           charge, c, r, p->beta, excluded_list_start, excluded_list,
+                           excluded_numbers, frc, need_energy, atom_energy,
+                           p->d_correction_atom_energy, need_virial, atom_virial);
+    SA_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int sponge_pme_step_print(sponge_pme* p, float* PM, float* direct, float* reciprocal,
+                                     float* self_energy, float* correction) {
+    if (!p) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    hipStream_t s = p->stream;
+    int rc = fill_async(p->d_sum, 0.0, 2, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_sum2_f32, dim3(64), dim3(256), 0, s, p->N, p->d_direct_atom_energy,
+                       p->d_correction_atom_energy, p->d_sum);
+    SA_LAUNCH_CHECK();
+    double hs[2];
+    float he[2];
+    SA_CHECK(hipMemcpyAsync(hs, p->d_sum, sizeof(hs), hipMemcpyDeviceToHost, s));
+    SA_CHECK(hipMemcpyAsync(he, p->d_ene, sizeof(he), hipMemcpyDeviceToHost, s));
+    SA_CHECK(hipStreamSynchronize(s));
+    const float dir = (float)hs[0], corr = (float)hs[1];
+    if (direct) *direct = dir;
+    if (correction) *correction = corr;
+    if (reciprocal) *reciprocal = he[0];
+    if (self_energy) *self_energy = he[1];
+    if (PM) *PM = dir + he[0] + he[1] + corr;
+    return 0;
+}
+
+extern "C" int sponge_pme_grids(sponge_pme* p, float* Q, float* FBCFQ) {
+    if (!p) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    if (Q) SA_CHECK(hipMemcpyAsync(Q, p->d_Q, p->nall * sizeof(float), hipMemcpyDeviceToDevice, p->stream));
+    if (FBCFQ)
+        SA_CHECK(hipMemcpyAsync(FBCFQ, p->d_FBCFQ, p->nall * sizeof(float), hipMemcpyDeviceToDevice,
+                                p->stream));
+    return 0;
+}
+
+extern "C" int sponge_pme_clear(sponge_pme* p) {
+    if (!p) return 0;
+    (void)hipStreamSynchronize(p->stream);
+    pme_free(p);
+    delete p;
+    return 0;
+}
+
+extern "C" const char* sponge_amd_version(void) { return "sponge_amd 0.1.0 (gfx950)"; }
