@@ -1146,7 +1146,8 @@ ORC_API void orc64_ewald_recip(int N, const double* crd, const double* charge, c
 // B-spline moduli: E_rec and forces.  Converges to orc64_ewald_recip as the
 // grid refines; the FP32 restatement must agree with it to float rounding.
 ORC_API void orc64_pme(int N, const double* crd, const double* charge, const double* box,
-                       double beta, int fx, int fy, int fz, double* frc, double* energy) {
+                       double beta, int fx, int fy, int fz, double* frc, double* energy,
+                       double* virial6) {
     const int dims[3] = {fx, fy, fz};
     const size_t nall = (size_t)fx * fy * fz;
     std::vector<cd> Q(nall, 0.0);
@@ -1195,6 +1196,7 @@ ORC_API void orc64_pme(int N, const double* crd, const double* charge, const dou
         return 1.0 / std::norm(s);
     };
     double E = 0;
+    double vir[6] = {0, 0, 0, 0, 0, 0};
     for (int x = 0; x < fx; ++x)
         for (int y = 0; y < fy; ++y)
             for (int z = 0; z < fz; ++z) {
@@ -1209,8 +1211,25 @@ ORC_API void orc64_pme(int N, const double* crd, const double* charge, const dou
                 double m2 = mx * mx + my * my + mz * mz;
                 double bc = exp(-M_PI * M_PI * m2 / (beta * beta)) / (M_PI * m2 * V) * bmod(x, fx) * bmod(y, fy) * bmod(z, fz);
                 E += 0.5 * bc * std::norm(F[idx]);
+                // the reference's reciprocal virial: over the half spectrum
+                // kz <= fz/2 with weight 1/2 on kz = 0 and kz = fz/2,
+                // -w |F|^2 BC/2 (I - 2 (1 + pi^2 m^2/beta^2) m m^T / m^2)
+                // (PM_force.cpp:586-606, 1024-1068), here in double
+                if (z <= fz / 2) {
+                    double w = (z == 0 || z == fz / 2) ? 0.5 : 1.0;
+                    double t = 2.0 / m2 * (1.0 + M_PI * M_PI * m2 / (beta * beta));
+                    double s = -w * std::norm(F[idx]) * 0.5 * bc;
+                    vir[0] += s * (1 - t * mx * mx);
+                    vir[1] += s * (0 - t * my * mx);
+                    vir[2] += s * (1 - t * my * my);
+                    vir[3] += s * (0 - t * mz * mx);
+                    vir[4] += s * (0 - t * mz * my);
+                    vir[5] += s * (1 - t * mz * mz);
+                }
                 F[idx] *= bc;
             }
+    if (virial6)
+        for (int k = 0; k < 6; ++k) virial6[k] = vir[k];
     fft3d_c(F.data(), fx, fy, fz, +1);
     for (int a = 0; a < N; ++a) {
         double g[3] = {0, 0, 0};

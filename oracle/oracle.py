@@ -21,7 +21,7 @@ _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 
 
 def build():
-    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    subprocess.run(["make", "-s", "-B", "-C", HERE], check=True)
 
 
 def _load(name):
@@ -69,7 +69,7 @@ def _load(name):
     lib.orc64_ewald_recip.argtypes = [C.c_int, _f64p, _f64p, _f64p, C.c_double, _i32p, _f64p,
                                       _f64p]
     lib.orc64_pme.argtypes = [C.c_int, _f64p, _f64p, _f64p, C.c_double, C.c_int, C.c_int, C.c_int,
-                              _f64p, C.POINTER(C.c_double)]
+                              _f64p, C.POINTER(C.c_double), _f64p]
     lib.orc_sum_f32.restype = C.c_double
     lib.orc_sum_f32.argtypes = [_f32p, C.c_int]
     lib.orc_num_threads.restype = C.c_int
@@ -312,6 +312,7 @@ def pme64(crd, charge, box, beta, fft):
     N = len(crd)
     frc = np.zeros(3 * N)
     e = C.c_double()
+    vir = np.zeros(6)
     lib().orc64_pme(N, crd.reshape(-1), f64(charge), f64(box), float(beta), *[int(x) for x in fft],
-                    frc, C.byref(e))
-    return dict(frc=frc.reshape(N, 3), reciprocal=e.value)
+                    frc, C.byref(e), vir)
+    return dict(frc=frc.reshape(N, 3), reciprocal=e.value, virial=vir)

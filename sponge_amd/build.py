@@ -38,6 +38,12 @@ def build(verbose=True, extra=None):
         cmd = [HIPCC] + [f for f in FLAGS if f != "-shared"] + ["-c", src, "-o", obj] + (extra or [])
         procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
         objs.append(obj)
+    # host-only C++ (module facades over the C ABI)
+    for src in sorted(glob.glob(os.path.join(HERE, "host", "*.cpp"))):
+        obj = os.path.join(bdir, "host_" + os.path.basename(src).replace(".cpp", ".o"))
+        cmd = ["g++", "-std=c++17", "-O2", "-fPIC", "-Wall", "-c", src, "-o", obj]
+        procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
+        objs.append(obj)
     ok = True
     for cmd, p in procs:
         out, _ = p.communicate()

@@ -52,6 +52,7 @@ struct sponge_md {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     int use_graph = 0;
+    int own_stream = 0;  // the driver created its stream (graph capture needs one)
 };
 
 namespace {
@@ -173,7 +174,8 @@ __global__ void k_sum_energy(int N, const float* __restrict__ e, double* out) {
 }
 
 // canonical in-plane coordinates of a triangle from its three side lengths
-// and masses (COM at the origin, A on the +y axis)
+// and masses: COM at the origin, A on the +y axis, and (B - A) x (C - A)
+// along +z, the handedness the SETTLE frame (ez ~ B0 x C0) assumes
 void triangle_geometry(double dAB, double dAC, double dBC, double mA, double mB, double mC,
                        float* g) {
     // A at the origin, B on -x, C from the law of cosines
@@ -193,11 +195,14 @@ void triangle_geometry(double dAB, double dAC, double dBC, double mA, double mB,
     double X, Y;
     rot(ax_, ay_, X, Y);
     g[0] = (float)Y;
+    // B sits on -x and C above the x axis before the rotation, so
+    // (B - A) x (C - A) points along -z; the reflection x -> -x (which keeps A
+    // on the +y axis) restores +z
     rot(bx_, by_, X, Y);
-    g[1] = (float)X;
+    g[1] = (float)(-X);
     g[2] = (float)Y;
     rot(cx_, cy_, X, Y);
-    g[3] = (float)X;
+    g[3] = (float)(-X);
     g[4] = (float)Y;
 }
 
@@ -228,6 +233,8 @@ void md_free(sponge_md* m) {
     dfree(m->d_water);
     dfree(m->d_old);
     dfree(m->d_ek);
+    if (m->own_stream && m->stream) (void)hipStreamDestroy(m->stream);
+    m->stream = nullptr;
 }
 
 // One MD step (force evaluation + integration), all on m->stream.
@@ -303,6 +310,16 @@ extern "C" int sponge_md_initial(sponge_md** out, int atom_numbers, const float*
     m->cutoff = cutoff;
     m->skin = skin;
     m->stream = (hipStream_t)stream;
+    if (!m->stream) {
+        // a stream of its own, at the default priority: the NULL stream
+        // cannot be captured into a graph
+        if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete m;
+            return SPONGE_AMD_ERROR_MALLOC_FAILED;
+        }
+        m->own_stream = 1;
+    }
+    stream = m->stream;
     const float one = 1.0f;
     m->cell = {box_length[0], 0.f, box_length[1], 0.f, 0.f, box_length[2]};
     m->rcell = {one / box_length[0], 0.f, one / box_length[1], 0.f, 0.f, one / box_length[2]};
@@ -402,8 +419,11 @@ extern "C" int sponge_md_run(sponge_md* m, int steps, int energy_interval, int u
                 int rc = md_step(m, st, 0);
                 hipGraph_t g = nullptr;
                 hipError_t he = hipStreamEndCapture(m->stream, &g);
-                if (rc) return rc;
-                SA_CHECK(he);
+                if (rc || he != hipSuccess) {
+                    if (g) (void)hipGraphDestroy(g);
+                    (void)hipGetLastError();
+                    return rc ? rc : (int)he;
+                }
                 m->graph = g;
                 SA_CHECK(hipGraphInstantiate(&m->graph_exec, g, nullptr, nullptr, 0));
             }
