@@ -301,6 +301,11 @@ int sponge_md_energies(sponge_md* md, double* out);
  * of 3*atom_numbers floats (any may be NULL); synchronises. */
 int sponge_md_get_state(sponge_md* md, float* crd, float* vel, float* frc);
 
+/* Replaces the coordinates and / or velocities from host arrays (either may be
+ * NULL); synchronises.  A coordinate change is picked up by the next
+ * CONDITIONAL list update through the displacement check. */
+int sponge_md_set_state(sponge_md* md, const float* crd, const float* vel);
+
 /* The module handles the driver owns (for timing and list export). */
 int sponge_md_handles(sponge_md* md, sponge_nl** nl, sponge_lj** lj, sponge_pme** pme);
 

@@ -98,6 +98,7 @@ SIGNATURES = {
     "sponge_md_run": (_I, [_P, _I, _I, _I]),
     "sponge_md_energies": (_I, [_P, C.POINTER(C.c_double)]),
     "sponge_md_get_state": (_I, [_P, _FP, _FP, _FP]),
+    "sponge_md_set_state": (_I, [_P, _FP, _FP]),
     "sponge_md_handles": (_I, [_P, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),
     "sponge_md_clear": (_I, [_P]),
 }
@@ -394,6 +395,28 @@ class MDSystem:
         _check(self.lib.sponge_md_get_state(self.h, crd.ctypes.data_as(_FP), vel.ctypes.data_as(_FP),
                                             frc.ctypes.data_as(_FP)), "sponge_md_get_state")
         return crd, vel, frc
+
+    def set_state(self, crd=None, vel=None):
+        c = None if crd is None else np.ascontiguousarray(crd, np.float32)
+        v = None if vel is None else np.ascontiguousarray(vel, np.float32)
+        _check(self.lib.sponge_md_set_state(self.h, None if c is None else c.ctypes.data_as(_FP),
+                                            None if v is None else v.ctypes.data_as(_FP)),
+               "sponge_md_set_state")
+
+    def equilibrate(self, steps, temperature=300.0, every=10):
+        """Velocity rescaling to `temperature` every `every` steps (set-up of
+        a synthetic start; not part of any timed region)."""
+        from .system import KB
+        m = np.asarray(self.s.mass, np.float64)
+        # rigid 3-site waters have 6 degrees of freedom each, ions 3
+        dof = 6 * self.s.n_waters + 3 * (self.N - 3 * self.s.n_waters) - 3
+        for _ in range(max(1, steps // every)):
+            self.run(every)
+            _, vel, _ = self.state()
+            ek = 0.5 * (m[:, None] * vel.astype(np.float64) ** 2).sum()
+            t = 2 * ek / (dof * KB)
+            if t > 0:
+                self.set_state(vel=vel * np.float32(np.sqrt(temperature / t)))
 
     def handles(self):
         nl, lj, pm = C.c_void_p(), C.c_void_p(), C.c_void_p()

@@ -14,7 +14,8 @@ struct sa_atom_group {
 
 // Neighbor list.  Atoms are counting-sorted by cell (x-major, z fastest) on
 // every rebuild; the list holds, per sorted atom p, the sorted indices q of
-// its half-list neighbours (global id of q > global id of p).
+// all its neighbours (both directions); the reference's half list is the
+// subset with global id of q > global id of p.
 struct sponge_nl {
     int N = 0;
     float cutoff = 0, skin = 0, grid_length = 0, thr = 0;  // thr = gl*gl*4 (cutoff_skin_square)
@@ -38,8 +39,9 @@ struct sponge_nl {
     int2* d_ranges = nullptr;     // [G][max_ranges] runs of neighbour cells [begin, end)
     int* d_nranges = nullptr;     // [G]
     int max_ranges = 0;
+    int cap = 0;                  // row capacity (both directions)
     int* d_list_count = nullptr;  // [N] per sorted atom
-    int* d_list = nullptr;        // [N][max_nb] sorted neighbour indices
+    int* d_list = nullptr;        // [N][cap] sorted neighbour indices, both directions
     int* d_exp_serial = nullptr;  // reference-format export storage (lazy)
     int* d_exp_count = nullptr;
     sa_atom_group* d_atom_group = nullptr;
@@ -59,6 +61,7 @@ struct sponge_lj {
     int* d_type = nullptr;        // [N] local order
     float4* d_pos = nullptr;      // [N] packed x, y, z, charge (list index space)
     int* d_ptype = nullptr;       // [N] LJ type (list index space)
+    int* d_pgid = nullptr;        // [N] global id (list index space)
     float4* d_facc = nullptr;     // [N] force accumulator (list index space)
     float2* d_eacc = nullptr;     // [N] (E_LJ, E_coulomb) per row
     float* d_vacc = nullptr;      // [N][6] virial per row
@@ -97,5 +100,5 @@ struct sponge_pme {
 
 namespace sa {
 int nl_pack_for_pairs(sponge_nl* nl, const float* crd, const float* charge, const int* type,
-                      float4* pos, int* ptype, float4* facc, hipStream_t s);
+                      float4* pos, int* ptype, int* pgid, hipStream_t s);
 }

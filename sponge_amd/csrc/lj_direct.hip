@@ -10,11 +10,13 @@
 // Energies and virial of a pair go to its row atom i (the lower global id),
 // exactly as in the reference's half-list kernel.
 //
-// Layout: the list is in the neighbor list's cell-sorted index space;
-// positions and charges are gathered once per step into a float4 SoA array in
-// that space, forces accumulate into a sorted float4 array (row sums via
-// wavefront shuffles, column updates via hardware float atomics), and a final
-// pass scatters into the caller's AoS arrays.
+// Layout: the list is in the neighbor list's cell-sorted index space and
+// holds both directions of every pair; positions and charges are gathered
+// once per step into a float4 array in that space.  One wavefront owns one
+// row and reduces its force by shuffles into a plain store (no atomics); a
+// final pass scatters the rows into the caller's AoS arrays.  The kernel over
+// a caller-provided reference-format half list (k_lj_pairs) keeps the
+// reference's column updates, as hardware float atomics.
 #include <vector>
 
 #include "internal.h"
@@ -112,6 +114,94 @@ __global__ void __launch_bounds__(256) k_lj_pairs(
     }
 }
 
+// Full-list rows (both directions): one wavefront owns row p and writes its
+// force with no atomics.  The energy and virial of a pair are counted only in
+// the row of its lower global id, which is the reference's half-list row, so
+// per-atom energies and virials keep the reference's assignment.
+template <bool ORTHO, bool NEED_E, bool NEED_V>
+__global__ void __launch_bounds__(256) k_lj_rows(
+    int nrows, const int* __restrict__ list_count, const int* __restrict__ list, int cap,
+    const float4* __restrict__ pos, const int* __restrict__ ptype, const int* __restrict__ pgid,
+    const float2* __restrict__ AB, LT6 c, LT6 r, float rc2_thr, float beta,
+    float4* __restrict__ facc, float2* __restrict__ eacc, float* __restrict__ vacc) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int p = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    if (p >= nrows) return;
+    const float4 pi4 = pos[p];
+    const float3 pi = make_float3(pi4.x, pi4.y, pi4.z);
+    const int ti = ptype[p];
+    const int gi = (NEED_E || NEED_V) ? pgid[p] : 0;
+    const int n = list_count[p];
+    const int* row = list + (size_t)p * cap;
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    float elj = 0.f, ec = 0.f;
+    float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f, v5 = 0.f;
+    for (int k = lane; k < n; k += kWave) {
+        const int q = row[k];
+        const float4 pj4 = pos[q];
+        const float3 d = disp_pbc<ORTHO>(make_float3(pj4.x, pj4.y, pj4.z), pi, c, r);
+        const float r2 = norm2_exact(d);
+        if (r2 < rc2_thr) {
+            const float2 ab = AB[lj_pair_index(ti, ptype[q])];
+            const float rabs = sqrtf(r2);
+            const float rinv = 1.0f / rabs;
+            const float r2inv = rinv * rinv;
+            const float r6inv = r2inv * r2inv * r2inv;
+            const float qq = pi4.w * pj4.w;
+            const float br = beta * rabs;
+            const float erfc_br = erfcf(br);
+            const float flj = (ab.y - ab.x * r6inv) * r6inv * r2inv;
+            const float fc = qq * rinv * r2inv * (br * kTwoOverSqrtPi * expf(-br * br) + erfc_br);
+            const float f = flj - fc;
+            const float lx = f * d.x, ly = f * d.y, lz = f * d.z;
+            fx += lx;
+            fy += ly;
+            fz += lz;
+            if (NEED_E || NEED_V) {
+                if (pgid[q] > gi) {
+                    if (NEED_V) {
+                        v0 -= lx * d.x;
+                        v1 -= lx * d.y + ly * d.x;
+                        v2 -= ly * d.y;
+                        v3 -= lx * d.z + lz * d.x;
+                        v4 -= ly * d.z + lz * d.y;
+                        v5 -= lz * d.z;
+                    }
+                    if (NEED_E) {
+                        elj += (0.083333333f * ab.x * r6inv - 0.166666667f * ab.y) * r6inv;
+                        ec += qq * erfc_br * rinv;
+                    }
+                }
+            }
+        }
+    }
+    fx = wave_sum(fx);
+    fy = wave_sum(fy);
+    fz = wave_sum(fz);
+    double delj = 0, dec = 0, dv0 = 0, dv1 = 0, dv2 = 0, dv3 = 0, dv4 = 0, dv5 = 0;
+    if (NEED_E) {
+        delj = wave_sum((double)elj);
+        dec = wave_sum((double)ec);
+    }
+    if (NEED_V) {
+        dv0 = wave_sum((double)v0);
+        dv1 = wave_sum((double)v1);
+        dv2 = wave_sum((double)v2);
+        dv3 = wave_sum((double)v3);
+        dv4 = wave_sum((double)v4);
+        dv5 = wave_sum((double)v5);
+    }
+    if (lane == 0) {
+        facc[p] = make_float4(fx, fy, fz, 0.f);
+        if (NEED_E) eacc[p] = make_float2((float)delj, (float)dec);
+        if (NEED_V) {
+            float* v = vacc + (size_t)6 * p;
+            v[0] = (float)dv0; v[1] = (float)dv1; v[2] = (float)dv2;
+            v[3] = (float)dv3; v[4] = (float)dv4; v[5] = (float)dv5;
+        }
+    }
+}
+
 // Scatter of the per-row accumulators to the caller's arrays: frc +=,
 // atom_energy += E_LJ + E_C, LJ energy and direct-PME energy overwritten
 // (the reference clears them before its kernel, Lennard_Jones_force.cpp:
@@ -172,6 +262,24 @@ __global__ void k_sum_f32(int n, const float* __restrict__ a, double* out) {
 }
 
 template <bool ORTHO>
+int launch_rows(sponge_lj* lj, int nrows, const int* cnt, const int* list, int cap, LT6 c, LT6 r,
+                float beta, int need_e, int need_v, hipStream_t s) {
+    dim3 grid(grid_for(nrows, 4)), block(256);
+    ScopedTimer timer("lj_pairs", s);
+#define SA_LJ_ROWS(E, V)                                                                        \
+    hipLaunchKernelGGL((k_lj_rows<ORTHO, E, V>), grid, block, 0, s, nrows, cnt, list, cap,      \
+                       lj->d_pos, lj->d_ptype, lj->d_pgid, lj->d_AB, c, r, lj->rc2_thr, beta,   \
+                       lj->d_facc, lj->d_eacc, lj->d_vacc)
+    if (need_e && need_v) SA_LJ_ROWS(true, true);
+    else if (need_e) SA_LJ_ROWS(true, false);
+    else if (need_v) SA_LJ_ROWS(false, true);
+    else SA_LJ_ROWS(false, false);
+#undef SA_LJ_ROWS
+    SA_LAUNCH_CHECK();
+    return 0;
+}
+
+template <bool ORTHO>
 int launch_pairs(sponge_lj* lj, int nrows, const int* cnt, const int* list, int max_nb,
                  LT6 c, LT6 r, float beta, int need_e, int need_v, hipStream_t s) {
     dim3 grid(grid_for(nrows, 4)), block(256);
@@ -210,6 +318,7 @@ void lj_free(sponge_lj* lj) {
     dfree(lj->d_type);
     dfree(lj->d_pos);
     dfree(lj->d_ptype);
+    dfree(lj->d_pgid);
     dfree(lj->d_facc);
     dfree(lj->d_eacc);
     dfree(lj->d_vacc);
@@ -262,6 +371,7 @@ extern "C" int sponge_lj_initial(sponge_lj** out, int atom_numbers, int atom_typ
     e |= dmalloc(&lj->d_type, N);
     e |= dmalloc(&lj->d_pos, N);
     e |= dmalloc(&lj->d_ptype, N);
+    e |= dmalloc(&lj->d_pgid, N);
     e |= dmalloc(&lj->d_facc, N);
     e |= dmalloc(&lj->d_eacc, N);
     e |= dmalloc(&lj->d_vacc, (size_t)6 * N);
@@ -317,16 +427,14 @@ extern "C" int sponge_lj_pme_direct_force(sponge_lj* lj, sponge_nl* nl, int atom
         return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
     hipStream_t s = lj->stream;
     const LT6 c = lt_from(cell), r = lt_from(rcell);
-    int rc = nl_pack_for_pairs(nl, crd, charge, lj->d_type, lj->d_pos, lj->d_ptype, lj->d_facc, s);
+    int rc = nl_pack_for_pairs(nl, crd, charge, lj->d_type, lj->d_pos, lj->d_ptype, lj->d_pgid, s);
     if (rc) return rc;
     if (lt_is_orthorhombic(c))
-        rc = launch_pairs<true>(lj, lj->N, nl->d_list_count, nl->d_list,
-                                nl->opt.max_neighbor_numbers, c, r, pme_beta, need_atom_energy,
-                                need_virial, s);
+        rc = launch_rows<true>(lj, lj->N, nl->d_list_count, nl->d_list, nl->cap, c, r, pme_beta,
+                               need_atom_energy, need_virial, s);
     else
-        rc = launch_pairs<false>(lj, lj->N, nl->d_list_count, nl->d_list,
-                                 nl->opt.max_neighbor_numbers, c, r, pme_beta, need_atom_energy,
-                                 need_virial, s);
+        rc = launch_rows<false>(lj, lj->N, nl->d_list_count, nl->d_list, nl->cap, c, r, pme_beta,
+                                need_atom_energy, need_virial, s);
     if (rc) return rc;
     return finalize(lj, lj->N, nl->d_sidx, need_atom_energy, need_virial, frc, atom_energy,
                     atom_virial, atom_direct_pme_energy, s);

@@ -477,6 +477,16 @@ extern "C" int sponge_md_get_state(sponge_md* m, float* crd, float* vel, float* 
     return 0;
 }
 
+extern "C" int sponge_md_set_state(sponge_md* m, const float* crd, const float* vel) {
+    if (!m) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    hipStream_t s = m->stream;
+    const size_t b = sizeof(float) * 3 * m->N;
+    if (crd) SA_CHECK(hipMemcpyAsync(m->d_crd, crd, b, hipMemcpyHostToDevice, s));
+    if (vel) SA_CHECK(hipMemcpyAsync(m->d_vel, vel, b, hipMemcpyHostToDevice, s));
+    SA_CHECK(hipStreamSynchronize(s));
+    return 0;
+}
+
 extern "C" int sponge_md_handles(sponge_md* m, sponge_nl** nl, sponge_lj** lj, sponge_pme** pme) {
     if (!m) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
     if (nl) *nl = m->nl;

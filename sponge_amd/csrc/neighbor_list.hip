@@ -136,94 +136,167 @@ __global__ void k_nl_scatter(const int* __restrict__ need, int N, const float* _
     spos[p] = make_float4(crd[3 * i], crd[3 * i + 1], crd[3 * i + 2], __int_as_float(gi));
 }
 
-// One wavefront per sorted atom p.  Lanes take 64 consecutive candidates of
-// a run of neighbour cells; accepted candidates are compacted by ballot.
+// Exclusion of the unordered pair {gi, gj} as the reference decides it for
+// the half-list row of the lower id: the higher id must lie inside
+// [list[first], list[last]] of the lower id's exclusion list and equal one of
+// its entries (Delete_Excluded_Atoms_Serial_In_Neighbor_List,
+// neighbor_list.cpp:736-807).
+__device__ __forceinline__ bool excluded_in(int lo_id, int hi_id, const int* __restrict__ ex_start,
+                                            const int* __restrict__ ex_list,
+                                            const int* __restrict__ ex_num) {
+    const int n = ex_num[lo_id];
+    if (n <= 0) return false;
+    const int s = ex_start[lo_id];
+    if (hi_id < ex_list[s] || hi_id > ex_list[s + n - 1]) return false;
+    for (int e = s; e < s + n; ++e)
+        if (ex_list[e] == hi_id) return true;
+    return false;
+}
+
+// One workgroup per cell.  The cell's stencil -- runs of consecutive cells,
+// hence contiguous ranges of the sorted array -- is staged once into LDS
+// (up to kStage candidates; any excess is read from L2), then each wavefront
+// takes one i atom of the cell at a time and sweeps the staged candidates 64
+// per pass.  Acceptance is compacted by ballot + mbcnt into the i row.
+// Rows hold BOTH directions (full list), so the pair kernel can own its rows
+// without atomics; the reference half list is the subset global(j) >
+// global(i) (k_nl_export).
+constexpr int kStage = 3072;
+constexpr int kMaxRangesLds = 256;
+
 template <bool ORTHO>
-__global__ void __launch_bounds__(256) k_nl_search(
-    const int* __restrict__ need, int N, const float4* __restrict__ spos,
-    const int* __restrict__ sidx, const int* __restrict__ atom_cell,
+__global__ void __launch_bounds__(256) k_nl_search_cells(
+    const int* __restrict__ need, const float4* __restrict__ spos,
     const int* __restrict__ start, const int2* __restrict__ ranges,
     const int* __restrict__ nranges, int max_ranges, LT6 c, LT6 r, float thr,
     const int* __restrict__ ex_start, const int* __restrict__ ex_list,
-    const int* __restrict__ ex_num, int max_nb, int* __restrict__ list_count,
+    const int* __restrict__ ex_num, int cap, int* __restrict__ list_count,
     int* __restrict__ list, int* __restrict__ flags) {
     if (*need == 0) return;
-    const int lane = threadIdx.x & (kWave - 1);
-    const int p = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
-    if (p >= N) return;
-    const float4 pi4 = spos[p];
-    const float3 pi = make_float3(pi4.x, pi4.y, pi4.z);
-    const int gi = __float_as_int(pi4.w);
-    const int cell = atom_cell[sidx[p]];
-    int ex_n = 0, ex_s = 0, ex_lo = 0, ex_hi = -1;
-    if (ex_num) {
-        ex_n = ex_num[gi];
-        if (ex_n > 0) {
-            ex_s = ex_start[gi];
-            ex_lo = ex_list[ex_s];
-            ex_hi = ex_list[ex_s + ex_n - 1];
+    extern __shared__ float4 cand[];
+    __shared__ int r_first[kMaxRangesLds];  // sorted index of the run's first atom
+    __shared__ int r_off[kMaxRangesLds + 1];  // offset of the run in the staged stream
+    const int cell = blockIdx.x;
+    const int a0 = start[cell], a1 = start[cell + 1];
+    if (a0 == a1) return;
+    const int nr = nranges[cell];
+    if (threadIdx.x == 0) {
+        int off = 0;
+        for (int k = 0; k < nr; ++k) {
+            const int2 cr = ranges[(size_t)cell * max_ranges + k];
+            r_first[k] = start[cr.x];
+            r_off[k] = off;
+            off += start[cr.y] - start[cr.x];
+        }
+        r_off[nr] = off;
+    }
+    __syncthreads();
+    for (int k = 0; k < nr; ++k) {
+        const int len = r_off[k + 1] - r_off[k];
+        for (int u = threadIdx.x; u < len; u += blockDim.x) {
+            const int t = r_off[k] + u;
+            if (t < kStage) cand[t] = spos[r_first[k] + u];
         }
     }
-    int* out = list + (size_t)p * max_nb;
-    int count = 0;
-    const int nr = nranges[cell];
-    for (int k = 0; k < nr; ++k) {
-        const int2 cr = ranges[(size_t)cell * max_ranges + k];
-        const int a0 = start[cr.x], a1 = start[cr.y];
-        for (int base = a0; base < a1; base += kWave) {
-            const int q = base + lane;
-            bool ok = false;
-            if (q < a1) {
-                const float4 pj4 = spos[q];
-                const int gj = __float_as_int(pj4.w);
-                if (gj > gi) {
-                    // Get_Displacement(crd_i, crd_j) as in Find_Neighbors_Gridly
-                    float3 d = disp_pbc<ORTHO>(pi, make_float3(pj4.x, pj4.y, pj4.z), c, r);
-                    if (norm2_exact(d) < thr) {
-                        ok = true;
-                        if (ex_n > 0 && gj >= ex_lo && gj <= ex_hi) {
-                            for (int e = ex_s; e < ex_s + ex_n; ++e)
-                                if (ex_list[e] == gj) ok = false;
+    __syncthreads();
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave, nwave = blockDim.x / kWave;
+    for (int p = a0 + wave; p < a1; p += nwave) {
+        const float4 pi4 = spos[p];
+        const float3 pi = make_float3(pi4.x, pi4.y, pi4.z);
+        const int gi = __float_as_int(pi4.w);
+        int ex_n = 0, ex_s = 0, ex_lo = 0, ex_hi = -1;
+        if (ex_num) {
+            ex_n = ex_num[gi];
+            if (ex_n > 0) {
+                ex_s = ex_start[gi];
+                ex_lo = ex_list[ex_s];
+                ex_hi = ex_list[ex_s + ex_n - 1];
+            }
+        }
+        int* out = list + (size_t)p * cap;
+        int count = 0;
+        for (int k = 0; k < nr; ++k) {
+            const int off = r_off[k], len = r_off[k + 1] - off, first = r_first[k];
+            for (int base = 0; base < len; base += kWave) {
+                const int u = base + lane;
+                bool ok = false;
+                if (u < len) {
+                    const int t = off + u;
+                    const float4 pj4 = t < kStage ? cand[t] : spos[first + u];
+                    const int gj = __float_as_int(pj4.w);
+                    if (gj != gi) {
+                        // Get_Displacement(crd_i, crd_j) as in Find_Neighbors_Gridly
+                        const float3 d = disp_pbc<ORTHO>(pi, make_float3(pj4.x, pj4.y, pj4.z), c, r);
+                        if (norm2_exact(d) < thr) {
+                            ok = true;
+                            if (ex_num) {
+                                if (gj > gi) {
+                                    if (ex_n > 0 && gj >= ex_lo && gj <= ex_hi) {
+                                        for (int e = ex_s; e < ex_s + ex_n; ++e)
+                                            if (ex_list[e] == gj) ok = false;
+                                    }
+                                } else if (excluded_in(gj, gi, ex_start, ex_list, ex_num)) {
+                                    ok = false;
+                                }
+                            }
                         }
                     }
                 }
+                const unsigned long long m = __ballot(ok);
+                if (ok) {
+                    const int pos = count + lanes_below(m);
+                    if (pos < cap) out[pos] = first + u;
+                }
+                count += __popcll(m);
             }
-            const unsigned long long m = __ballot(ok);
-            if (ok) {
-                const int pos = count + lanes_below(m);
-                if (pos < max_nb) out[pos] = q;
-            }
-            count += __popcll(m);
         }
-    }
-    if (lane == 0) {
-        list_count[p] = count < max_nb ? count : max_nb;
-        if (count > max_nb) atomicOr(flags, SPONGE_AMD_NL_LIST_OVERFLOW);
+        if (lane == 0) {
+            list_count[p] = count < cap ? count : cap;
+            if (count > cap) atomicOr(flags, SPONGE_AMD_NL_LIST_OVERFLOW);
+        }
     }
 }
 
-// Reference-format export: local ids, rows by local atom.
-__global__ void k_nl_export(int N, const int* __restrict__ sidx,
+// Reference-format export: the half rows (global(j) > global(i)) in local
+// ids, rows by local atom; one wavefront per sorted row, ballot compaction.
+__global__ void k_nl_export(int N, const int* __restrict__ sidx, const float4* __restrict__ spos,
                             const int* __restrict__ list_count, const int* __restrict__ list,
-                            int max_nb, int* __restrict__ counts, int* __restrict__ serial,
+                            int cap, int* __restrict__ counts, int* __restrict__ serial,
                             int max_out, sa_atom_group* group, int* __restrict__ flags) {
     const int lane = threadIdx.x & (kWave - 1);
     const int p = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
     if (p >= N) return;
     const int i = sidx[p];
+    const int gi = __float_as_int(spos[p].w);
     const int n = list_count[p];
-    const int nout = n < max_out ? n : max_out;
-    const int* in = list + (size_t)p * max_nb;
+    const int* in = list + (size_t)p * cap;
     int* o = serial + (size_t)i * max_out;
-    for (int k = lane; k < nout; k += kWave) o[k] = sidx[in[k]];
+    int count = 0;
+    for (int base = 0; base < n; base += kWave) {
+        const int k = base + lane;
+        int q = 0;
+        bool keep = false;
+        if (k < n) {
+            q = in[k];
+            keep = __float_as_int(spos[q].w) > gi;
+        }
+        const unsigned long long m = __ballot(keep);
+        if (keep) {
+            const int pos = count + lanes_below(m);
+            if (pos < max_out) o[pos] = sidx[q];
+        }
+        count += __popcll(m);
+    }
     if (lane == 0) {
+        const int nout = count < max_out ? count : max_out;
         counts[i] = nout;
         if (group) {
             group[i].atom_numbers = nout;
             group[i].ghost_numbers = 0;
             group[i].atom_serial = o;
         }
-        if (n > max_out) atomicOr(flags, SPONGE_AMD_NL_LIST_OVERFLOW);
+        if (count > max_out) atomicOr(flags, SPONGE_AMD_NL_LIST_OVERFLOW);
     }
 }
 
@@ -356,7 +429,13 @@ extern "C" int sponge_nl_initial(sponge_nl** out, int atom_numbers, float cutoff
         delete nl;
         return rc;
     }
-    const int N = nl->N, G = nl->G, M = nl->opt.max_neighbor_numbers;
+    const int N = nl->N, G = nl->G;
+    // rows hold both directions: twice the reference's per-atom half capacity
+    nl->cap = 2 * nl->opt.max_neighbor_numbers;
+    if (nl->max_ranges > 256) {
+        delete nl;
+        return SPONGE_AMD_ERROR_OVERFLOW;
+    }
     int e = 0;
     e |= dmalloc(&nl->d_need, 1);
     e |= dmalloc(&nl->d_flags, 1);
@@ -372,7 +451,7 @@ extern "C" int sponge_nl_initial(sponge_nl** out, int atom_numbers, float cutoff
     e |= dmalloc(&nl->d_ranges, ranges.size());
     e |= dmalloc(&nl->d_nranges, G);
     e |= dmalloc(&nl->d_list_count, N);
-    e |= dmalloc(&nl->d_list, (size_t)N * M);
+    e |= dmalloc(&nl->d_list, (size_t)N * nl->cap);
     if (e) {
         nl_free(nl);
         delete nl;
@@ -447,22 +526,23 @@ extern "C" int sponge_nl_update(sponge_nl* nl, const int* atom_local, int local_
                        atom_local, nl->d_atom_cell, nl->d_atom_slot, nl->d_cell_start, nl->d_sidx,
                        nl->d_rank, nl->d_spos);
     SA_LAUNCH_CHECK();
-    const int waves_per_block = 4;
-    dim3 grid(grid_for(N, waves_per_block)), block(waves_per_block * kWave);
-    ScopedTimer timer("nl_search", s);
-    if (nl->ortho)
-        hipLaunchKernelGGL(k_nl_search<true>, grid, block, 0, s, nl->d_need, N, nl->d_spos,
-                           nl->d_sidx, nl->d_atom_cell, nl->d_cell_start, nl->d_ranges,
-                           nl->d_nranges, nl->max_ranges, c, r, nl->thr, excluded_list_start,
-                           excluded_list, excluded_numbers, nl->opt.max_neighbor_numbers,
-                           nl->d_list_count, nl->d_list, nl->d_flags);
-    else
-        hipLaunchKernelGGL(k_nl_search<false>, grid, block, 0, s, nl->d_need, N, nl->d_spos,
-                           nl->d_sidx, nl->d_atom_cell, nl->d_cell_start, nl->d_ranges,
-                           nl->d_nranges, nl->max_ranges, c, r, nl->thr, excluded_list_start,
-                           excluded_list, excluded_numbers, nl->opt.max_neighbor_numbers,
-                           nl->d_list_count, nl->d_list, nl->d_flags);
-    SA_LAUNCH_CHECK();
+    {
+        ScopedTimer timer("nl_search", s);
+        const size_t sh = sizeof(float4) * kStage;
+        if (nl->ortho)
+            hipLaunchKernelGGL(k_nl_search_cells<true>, dim3(nl->G), dim3(256), sh, s, nl->d_need,
+                               nl->d_spos, nl->d_cell_start, nl->d_ranges, nl->d_nranges,
+                               nl->max_ranges, c, r, nl->thr, excluded_list_start, excluded_list,
+                               excluded_numbers, nl->cap, nl->d_list_count, nl->d_list,
+                               nl->d_flags);
+        else
+            hipLaunchKernelGGL(k_nl_search_cells<false>, dim3(nl->G), dim3(256), sh, s, nl->d_need,
+                               nl->d_spos, nl->d_cell_start, nl->d_ranges, nl->d_nranges,
+                               nl->max_ranges, c, r, nl->thr, excluded_list_start, excluded_list,
+                               excluded_numbers, nl->cap, nl->d_list_count, nl->d_list,
+                               nl->d_flags);
+        SA_LAUNCH_CHECK();
+    }
     nl->built = 1;
     return 0;
 }
@@ -486,9 +566,9 @@ extern "C" int sponge_nl_export_half_list(sponge_nl* nl, int* counts, int* seria
     if (!nl || !counts || !serial || max_numbers <= 0) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
     if (!nl->built) return SPONGE_AMD_ERROR_MISSING_COMMAND;
     dim3 grid(grid_for(nl->N, 4)), block(256);
-    hipLaunchKernelGGL(k_nl_export, grid, block, 0, nl->stream, nl->N, nl->d_sidx,
-                       nl->d_list_count, nl->d_list, nl->opt.max_neighbor_numbers, counts, serial,
-                       max_numbers, (sa_atom_group*)nullptr, nl->d_flags);
+    hipLaunchKernelGGL(k_nl_export, grid, block, 0, nl->stream, nl->N, nl->d_sidx, nl->d_spos,
+                       nl->d_list_count, nl->d_list, nl->cap, counts, serial, max_numbers,
+                       (sa_atom_group*)nullptr, nl->d_flags);
     SA_LAUNCH_CHECK();
     return 0;
 }
@@ -504,8 +584,8 @@ extern "C" int sponge_nl_atom_group(sponge_nl* nl, void** d_nl) {
         if (e) return SPONGE_AMD_ERROR_MALLOC_FAILED;
     }
     dim3 grid(grid_for(nl->N, 4)), block(256);
-    hipLaunchKernelGGL(k_nl_export, grid, block, 0, nl->stream, nl->N, nl->d_sidx,
-                       nl->d_list_count, nl->d_list, M, nl->d_exp_count, nl->d_exp_serial, M,
+    hipLaunchKernelGGL(k_nl_export, grid, block, 0, nl->stream, nl->N, nl->d_sidx, nl->d_spos,
+                       nl->d_list_count, nl->d_list, nl->cap, nl->d_exp_count, nl->d_exp_serial, M,
                        nl->d_atom_group, nl->d_flags);
     SA_LAUNCH_CHECK();
     *d_nl = nl->d_atom_group;
@@ -534,7 +614,7 @@ extern "C" int sponge_nl_info(sponge_nl* nl, long long* list_entries, int* grid_
         }
         dfree(d);
         if (rc) return rc;
-        *list_entries = (long long)h;
+        *list_entries = (long long)(h / 2);  // each unordered pair is stored twice
     }
     return 0;
 }
@@ -559,22 +639,22 @@ extern "C" int sponge_nl_clear(sponge_nl* nl) {
 // Gather of the current positions, charges and LJ types into the sorted
 // order of the last build (the role of Copy_Crd_And_Charge_To_New_Crd,
 // Lennard_Jones_force.cpp:17-27), clearing the sorted force accumulator.
-__global__ void k_nl_pack(int N, const int* __restrict__ sidx, const float* __restrict__ crd,
-                          const float* __restrict__ charge, const int* __restrict__ type,
-                          float4* __restrict__ pos, int* __restrict__ ptype,
-                          float4* __restrict__ facc) {
+__global__ void k_nl_pack(int N, const int* __restrict__ sidx, const float4* __restrict__ spos,
+                          const float* __restrict__ crd, const float* __restrict__ charge,
+                          const int* __restrict__ type, float4* __restrict__ pos,
+                          int* __restrict__ ptype, int* __restrict__ pgid) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= N) return;
     int i = sidx[p];
     pos[p] = make_float4(crd[3 * i], crd[3 * i + 1], crd[3 * i + 2], charge[i]);
     ptype[p] = type[i];
-    facc[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+    pgid[p] = __float_as_int(spos[p].w);
 }
 
 int sa::nl_pack_for_pairs(sponge_nl* nl, const float* crd, const float* charge, const int* type,
-                          float4* pos, int* ptype, float4* facc, hipStream_t s) {
+                          float4* pos, int* ptype, int* pgid, hipStream_t s) {
     hipLaunchKernelGGL(k_nl_pack, dim3(grid_for(nl->N, 256)), dim3(256), 0, s, nl->N, nl->d_sidx,
-                       crd, charge, type, pos, ptype, facc);
+                       nl->d_spos, crd, charge, type, pos, ptype, pgid);
     SA_LAUNCH_CHECK();
     return 0;
 }
