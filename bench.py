@@ -10,7 +10,10 @@ gather), leap-frog and SETTLE.  Inputs are synthetic (seeded lattice water,
 random orientations, Maxwell-Boltzmann velocities) and resident on the GPU
 before the timed region.
 
-Prints one JSON line (see DESIGN.md for the fields).
+The timed region replays the step as a HIP graph; HIP events recorded on
+the MD stream around the pair kernel in every replay give its average
+launch duration for the roofline.  A short eager pass afterwards times the
+other kernels for the breakdown.  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -45,7 +48,7 @@ def cpu_baseline(s, gpu_rebuild_interval, budget_s=20.0):
     OpenMP + atomic structure, -O3 -ffast-math) on the same coordinates:
     force evaluations of the full system, list builds amortised over the
     rebuild interval the GPU run observed."""
-    from oracle import oracle as O
+    from oracle import oracle as O  # the CPU baseline leg only
     lib = O.lib(fast=True)
     threads = lib.orc_num_threads()
     beta = O.get_beta(s.cutoff, s.tolerance)
@@ -82,6 +85,10 @@ def main():
     ap.add_argument("--config", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--eager", action="store_true", help="time eager launches instead of graph replays")
+    ap.add_argument("--breakdown-steps", type=int, default=100)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 256 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the hot path is "
@@ -90,6 +97,9 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.copy_calibration:
+        copy_calibration(torch)
+        return
     from sponge_amd import abi
     from sponge_amd import system as S
 
@@ -103,18 +113,35 @@ def main():
     # 300 K (a synthetic-input set-up step, outside the timed region)
     md.equilibrate(args.warmup, temperature=300.0, every=10)
     torch.cuda.synchronize()
-    rb0 = C_rebuilds(lib, nl_h)
 
+    # timed region: graph replays, events around the pair kernel only
+    use_graph = not args.eager
+    abi.set_kernel_timing_filter(["lj_pairs"])
     abi.reset_kernel_timing()
     abi.set_kernel_timing(True)
+    md.run(1, use_graph=use_graph)  # capture outside the timed region
+    torch.cuda.synchronize()
+    abi.reset_kernel_timing()
+    rb0 = C_rebuilds(lib, nl_h)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    md.run(args.steps)
+    md.run(args.steps, use_graph=use_graph)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    abi.set_kernel_timing(False)
     rb1 = C_rebuilds(lib, nl_h)
-    kernel = {k: abi.kernel_timing(k) for k in TIMERS}
+    lj_ms, lj_n = abi.kernel_timing("lj_pairs")
+    abi.set_kernel_timing(False)
+
+    # breakdown of the other kernels: a short eager pass with every timer
+    kernel = {}
+    if args.breakdown_steps > 0:
+        abi.set_kernel_timing_filter(None)
+        abi.reset_kernel_timing()
+        abi.set_kernel_timing(True)
+        md.run(args.breakdown_steps, use_graph=False)
+        torch.cuda.synchronize()
+        abi.set_kernel_timing(False)
+        kernel = {k: abi.kernel_timing(k) for k in TIMERS}
 
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     ns_day = 86.4 * s.dt_ps / (ms_per_step * 1e-3)
@@ -133,7 +160,7 @@ def main():
     torch.cuda.synchronize()
     n_in, n_list = O.pair_counts(s, counts.cpu().numpy(), serial.view(N, M).cpu().numpy(), crd=crd)
 
-    lj_ms, lj_n = kernel["lj_pairs"]
+    assert lj_n == args.steps, (lj_n, args.steps)
     lj_avg_s = lj_ms * 1e-3 / max(lj_n, 1)
     flops = pair_kernel_flops(n_in, n_list)
     achieved = flops / lj_avg_s / 1e12
@@ -141,7 +168,9 @@ def main():
                     frac=achieved / FP32_PEAK_TFLOPS, traffic=None,
                     kernel="lj_pairs", flop_per_launch=flops, pairs_in_cutoff=n_in,
                     list_entries=n_list, avg_launch_us=lj_avg_s * 1e6,
-                    frac_of_unpacked_peak=achieved / FP32_PEAK_TFLOPS_UNPACKED)
+                    frac_of_unpacked_peak=achieved / FP32_PEAK_TFLOPS_UNPACKED,
+                    timing="HIP events on the MD stream around every pair-kernel launch of the "
+                           "timed region")
 
     out = dict(metric=METRIC, value=ns_day, unit="ns/day", n_gpus=1, steps=args.steps,
                warmup=args.warmup, ms_per_step=ms_per_step, higher_is_better=True,
@@ -154,12 +183,30 @@ def main():
                roofline=roofline,
                force_evals_per_s=1e3 / ms_per_step,
                list_rebuild_interval_steps=rebuild_interval,
-               kernel_avg_us={k: (v[0] * 1e3 / v[1] if v[1] else None) for k, v in kernel.items()})
+               launch="hip graph replay" if use_graph else "eager",
+               kernel_avg_us_eager_pass={k: (v[0] * 1e3 / v[1] if v[1] else None)
+                                         for k, v in kernel.items()})
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(s, rebuild_interval, args.cpu_budget)
+        import dataclasses
+        s_now = dataclasses.replace(s, crd=crd)
+        out["cpu_baseline"] = cpu_baseline(s_now, rebuild_interval, args.cpu_budget)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))
+
+
+def copy_calibration(torch):
+    """A plain device copy of known size, for scaling the HBM byte counters
+    (FETCH_SIZE / WRITE_SIZE) to bytes: 256 MiB read + 256 MiB written per
+    launch, 5 launches."""
+    n = 64 * 1024 * 1024
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": 4 * n, "copy_calibration_bytes_written": 4 * n,
+                      "launches": 5}))
 
 
 def C_rebuilds(lib, nl_h):

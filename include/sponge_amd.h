@@ -80,8 +80,13 @@ int sponge_amd_get_fft_parameter(float length);
  * "pme_gather", "pme_fft_z_r2c", "pme_fft_y", "pme_fft_x_conv",
  * "pme_ifft_y", "pme_fft_z_c2r").  sponge_amd_kernel_timing synchronises
  * with the recorded events and returns the accumulated milliseconds and
- * launch count of one timer. */
+ * launch count of one timer.  Timers also run inside the NVE driver's HIP
+ * graph: each replay records fresh events.
+ * sponge_amd_set_kernel_timing_filter limits the timers to a comma-separated
+ * list of names (NULL or "" = all), so that only the kernel being measured
+ * carries events. */
 int sponge_amd_set_kernel_timing(int enable);
+int sponge_amd_set_kernel_timing_filter(const char* names);
 int sponge_amd_kernel_timing(const char* name, double* total_ms, long long* launches);
 int sponge_amd_reset_kernel_timing(void);
 
@@ -154,7 +159,9 @@ int sponge_nl_clear(sponge_nl* nl);
 
 /* LENNARD_JONES_INFORMATION::Initial (Lennard_Jones_force.cpp:147-250).
  * LJ_A / LJ_B are the in-memory tables (file values x12 and x6, lj.hpp:47,57)
- * of size T(T+1)/2; atom_LJ_type has atom_numbers entries.  Host arrays. */
+ * of size T(T+1)/2; atom_LJ_type has atom_numbers entries.  Host arrays.
+ * At most 64 LJ types (the pair kernel holds the T x T table in LDS);
+ * more return SPONGE_AMD_ERROR_NOT_IMPLEMENTED. */
 int sponge_lj_initial(sponge_lj** out, int atom_numbers, int atom_type_numbers,
                       const float* LJ_A, const float* LJ_B, const int* atom_LJ_type, float cutoff,
                       void* stream);

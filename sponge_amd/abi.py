@@ -61,6 +61,7 @@ SIGNATURES = {
     "sponge_amd_get_beta": (_F, [_F, _F]),
     "sponge_amd_get_fft_parameter": (_I, [_F]),
     "sponge_amd_set_kernel_timing": (_I, [_I]),
+    "sponge_amd_set_kernel_timing_filter": (_I, [C.c_char_p]),
     "sponge_amd_kernel_timing": (_I, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
     "sponge_amd_reset_kernel_timing": (_I, []),
     "sponge_nl_default_options": (None, [C.POINTER(NLOptions)]),
@@ -132,6 +133,12 @@ def load():
 
 def set_kernel_timing(enable):
     _check(load().sponge_amd_set_kernel_timing(int(enable)), "set_kernel_timing")
+
+
+def set_kernel_timing_filter(names=None):
+    """Time only the named kernels (an iterable of timer names; None = all)."""
+    arg = None if not names else ",".join(names).encode()
+    _check(load().sponge_amd_set_kernel_timing_filter(arg), "set_kernel_timing_filter")
 
 
 def kernel_timing(name):

@@ -14,8 +14,9 @@ struct sa_atom_group {
 
 // Neighbor list.  Atoms are counting-sorted by cell (x-major, z fastest) on
 // every rebuild; the list holds, per sorted atom p, the sorted indices q of
-// all its neighbours (both directions); the reference's half list is the
-// subset with global id of q > global id of p.
+// all its neighbours (both directions) as 16-bit positions in the candidate
+// stream of p's cell (the concatenated runs of its stencil cells); the
+// reference's half list is the subset with global id of q > global id of p.
 struct sponge_nl {
     int N = 0;
     float cutoff = 0, skin = 0, grid_length = 0, thr = 0;  // thr = gl*gl*4 (cutoff_skin_square)
@@ -40,8 +41,14 @@ struct sponge_nl {
     int* d_nranges = nullptr;     // [G]
     int max_ranges = 0;
     int cap = 0;                  // row capacity (both directions)
+    int stage = 0;                // candidate-stream entries staged in LDS by the search
     int* d_list_count = nullptr;  // [N] per sorted atom
-    int* d_list = nullptr;        // [N][cap] sorted neighbour indices, both directions
+    unsigned short* d_list = nullptr;  // [N][cap] positions in the row cell's candidate stream
+    int2* d_sexw = nullptr;       // [N] combined exclusion window of each sorted atom
+    int2* d_gexw = nullptr;       // [N] combined exclusion window per global id
+    const int* exw_src[3] = {nullptr, nullptr, nullptr};  // lists the windows were built from
+    int* d_rfirst = nullptr;      // [G][max_ranges] first sorted index of each stream run
+    int* d_roff = nullptr;        // [G][max_ranges+1] stream offset of each run
     int* d_exp_serial = nullptr;  // reference-format export storage (lazy)
     int* d_exp_count = nullptr;
     sa_atom_group* d_atom_group = nullptr;
@@ -58,7 +65,9 @@ struct sponge_lj {
     std::vector<float> h_A, h_B;
     std::vector<int> h_type;
     float2* d_AB = nullptr;       // interleaved (A, B) per pair type
+    float2* d_AB_sq = nullptr;    // [T][T] square copy of the (A, B) table
     int* d_type = nullptr;        // [N] local order
+    int stage = 0;                // candidate-stream entries staged in LDS by the pair kernel
     float4* d_pos = nullptr;      // [N] packed x, y, z, charge (list index space)
     int* d_ptype = nullptr;       // [N] LJ type (list index space)
     int* d_pgid = nullptr;        // [N] global id (list index space)
@@ -99,6 +108,27 @@ struct sponge_pme {
 };
 
 namespace sa {
+// runs per cell stream (the stencil of a cell has at most 192 cells)
+constexpr int kMaxRanges = 256;
+
+// sorted index of position t of a cell's candidate stream, from the cell's
+// run tables (first sorted index and stream offset of each run)
+// (the last run k with r_off[k] <= t, by bisection: that run is non-empty)
+__device__ __forceinline__ int stream_to_sorted(int t, int nr, const int* r_first, const int* r_off) {
+    int lo = 0, hi = nr - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (r_off[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return r_first[lo] + (t - r_off[lo]);
+}
+
+// Candidate-stream entries a kernel stages in LDS, given the bytes per entry:
+// as many as the device's per-workgroup LDS holds beside 4 KiB of tables and
+// `reserve` further bytes, at most `want`, a multiple of 64.
+int stage_entries(size_t bytes_per_entry, int want, size_t reserve = 0);
+
 int nl_pack_for_pairs(sponge_nl* nl, const float* crd, const float* charge, const int* type,
                       float4* pos, int* ptype, int* pgid, hipStream_t s);
 }

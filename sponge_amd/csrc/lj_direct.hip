@@ -10,13 +10,15 @@
 // Energies and virial of a pair go to its row atom i (the lower global id),
 // exactly as in the reference's half-list kernel.
 //
-// Layout: the list is in the neighbor list's cell-sorted index space and
-// holds both directions of every pair; positions and charges are gathered
-// once per step into a float4 array in that space.  One wavefront owns one
-// row and reduces its force by shuffles into a plain store (no atomics); a
-// final pass scatters the rows into the caller's AoS arrays.  The kernel over
-// a caller-provided reference-format half list (k_lj_pairs) keeps the
-// reference's column updates, as hardware float atomics.
+// Layout: rows are in the neighbor list's cell-sorted index space and hold
+// both directions of every pair, each entry a 16-bit position in the row
+// cell's candidate stream (the atoms of its stencil cells, in sorted order).
+// Positions and charges are gathered once per step into a float4 array in
+// sorted space.  One workgroup per cell stages that stream in LDS; one
+// wavefront owns one row and reduces its force by shuffles into a plain store
+// (no atomics); a final pass scatters the rows into the caller's AoS arrays.
+// The kernel over a caller-provided reference-format half list (k_lj_pairs)
+// keeps the reference's column updates, as hardware float atomics.
 #include <vector>
 
 #include "internal.h"
@@ -114,64 +116,131 @@ __global__ void __launch_bounds__(256) k_lj_pairs(
     }
 }
 
-// Full-list rows (both directions): one wavefront owns row p and writes its
-// force with no atomics.  The energy and virial of a pair are counted only in
-// the row of its lower global id, which is the reference's half-list row, so
-// per-atom energies and virials keep the reference's assignment.
-template <bool ORTHO, bool NEED_E, bool NEED_V>
-__global__ void __launch_bounds__(256) k_lj_rows(
-    int nrows, const int* __restrict__ list_count, const int* __restrict__ list, int cap,
-    const float4* __restrict__ pos, const int* __restrict__ ptype, const int* __restrict__ pgid,
-    const float2* __restrict__ AB, LT6 c, LT6 r, float rc2_thr, float beta,
-    float4* __restrict__ facc, float2* __restrict__ eacc, float* __restrict__ vacc) {
-    const int lane = threadIdx.x & (kWave - 1);
-    const int p = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
-    if (p >= nrows) return;
+// Cell-block rows: one workgroup per neighbor-list cell stages the cell's
+// candidate stream -- current positions with charges, LJ types and global ids
+// of every atom of its stencil -- into LDS once, with the square (A, B) table
+// of the type pairs; each wavefront then owns one row (an atom of the cell)
+// and reads its partners from LDS through the 16-bit stream positions of the
+// list.  Forces need no atomics (rows hold both directions); energies and
+// virials are counted in the row of the lower global id only, the
+// reference's half-list row.
+//
+// Arithmetic per in-cutoff pair: 1/r from the hardware rsqrt, erfc(beta r)
+// as exp(-(beta r)^2) erfcx(beta r) with the polynomial erfcx (FAST, used
+// when beta * cutoff <= kErfcxMax) so that one exponential serves the
+// energy and the force bracket; otherwise the library erfcf.
+constexpr int kPairStage = 3584;
+constexpr int kMaxLdsTypes = 64;  // the square (A, B) table lives in LDS: up to 64 LJ types
+
+// Partner data of a row entry: from the block's LDS stage (every position of
+// the stream staged) or, for the rare stream longer than the stage, from the
+// sorted arrays in HBM.  The two sources are separate code paths chosen once
+// per block, so that LDS reads stay ds_read and HBM reads global loads.
+struct PairSrc {
+    const float4* pos;
+    const int* typ;
+    const int* gid;
+};
+
+template <bool STAGED>
+__device__ __forceinline__ int src_index(int t, int nr, const int* r_first, const int* r_off) {
+    if (STAGED) return t;
+    return stream_to_sorted(t, nr, r_first, r_off);
+}
+
+// Force (and energy / virial terms) of one pair, branch-free: pairs at or
+// beyond the cutoff contribute exact zeros (selects, so the non-finite
+// values of a far or padding pair never reach the sums).
+template <bool NEED_E, bool NEED_V, bool FAST>
+struct PairTerm {
+    float lx, ly, lz, elj, ec;
+    bool in;
+    __device__ __forceinline__ PairTerm(const float3& d, float r2, float qi, float qj, float2 ab,
+                                        float rc2_thr, float beta) {
+        in = r2 < rc2_thr;
+        const float rinv = __builtin_amdgcn_rsqf(r2);
+        const float rabs = r2 * rinv;
+        const float r2inv = rinv * rinv;
+        const float r6inv = r2inv * r2inv * r2inv;
+        const float qq = qi * qj;
+        const float br = beta * rabs;
+        float erfc_br, bracket;
+        if (FAST) {
+            const float ex = __expf(-br * br);
+            const float ecx = erfcx_poly(br);
+            erfc_br = ex * ecx;
+            bracket = ex * fmaf(br, kTwoOverSqrtPi, ecx);
+        } else {
+            erfc_br = erfcf(br);
+            bracket = fmaf(br * kTwoOverSqrtPi, expf(-br * br), erfc_br);
+        }
+        const float flj = (ab.y - ab.x * r6inv) * r6inv * r2inv;
+        const float fc = qq * rinv * r2inv * bracket;
+        const float f = in ? flj - fc : 0.f;
+        lx = f * d.x;
+        ly = f * d.y;
+        lz = f * d.z;
+        elj = 0.f;
+        ec = 0.f;
+        if (NEED_E) {
+            elj = in ? (0.083333333f * ab.x * r6inv - 0.166666667f * ab.y) * r6inv : 0.f;
+            ec = in ? qq * erfc_br * rinv : 0.f;
+        }
+    }
+};
+
+// One row (wavefront-wide): two partner entries per lane and pass, k and
+// k + 64, loaded together so that their latencies and arithmetic overlap.
+template <bool ORTHO, bool NEED_E, bool NEED_V, bool FAST, bool STAGED>
+__device__ __forceinline__ void lj_row(int p, int lane, int n, const unsigned short* __restrict__ row,
+                                       PairSrc src, int nr, const int* r_first, const int* r_off,
+                                       const float4* __restrict__ pos, const float2* trow, int gi,
+                                       const LT6& c, const LT6& r, float rc2_thr, float beta,
+                                       float4* __restrict__ facc, float2* __restrict__ eacc,
+                                       float* __restrict__ vacc) {
     const float4 pi4 = pos[p];
     const float3 pi = make_float3(pi4.x, pi4.y, pi4.z);
-    const int ti = ptype[p];
-    const int gi = (NEED_E || NEED_V) ? pgid[p] : 0;
-    const int n = list_count[p];
-    const int* row = list + (size_t)p * cap;
     float fx = 0.f, fy = 0.f, fz = 0.f;
     float elj = 0.f, ec = 0.f;
     float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f, v5 = 0.f;
-    for (int k = lane; k < n; k += kWave) {
-        const int q = row[k];
-        const float4 pj4 = pos[q];
-        const float3 d = disp_pbc<ORTHO>(make_float3(pj4.x, pj4.y, pj4.z), pi, c, r);
-        const float r2 = norm2_exact(d);
-        if (r2 < rc2_thr) {
-            const float2 ab = AB[lj_pair_index(ti, ptype[q])];
-            const float rabs = sqrtf(r2);
-            const float rinv = 1.0f / rabs;
-            const float r2inv = rinv * rinv;
-            const float r6inv = r2inv * r2inv * r2inv;
-            const float qq = pi4.w * pj4.w;
-            const float br = beta * rabs;
-            const float erfc_br = erfcf(br);
-            const float flj = (ab.y - ab.x * r6inv) * r6inv * r2inv;
-            const float fc = qq * rinv * r2inv * (br * kTwoOverSqrtPi * expf(-br * br) + erfc_br);
-            const float f = flj - fc;
-            const float lx = f * d.x, ly = f * d.y, lz = f * d.z;
-            fx += lx;
-            fy += ly;
-            fz += lz;
-            if (NEED_E || NEED_V) {
-                if (pgid[q] > gi) {
-                    if (NEED_V) {
-                        v0 -= lx * d.x;
-                        v1 -= lx * d.y + ly * d.x;
-                        v2 -= ly * d.y;
-                        v3 -= lx * d.z + lz * d.x;
-                        v4 -= ly * d.z + lz * d.y;
-                        v5 -= lz * d.z;
-                    }
-                    if (NEED_E) {
-                        elj += (0.083333333f * ab.x * r6inv - 0.166666667f * ab.y) * r6inv;
-                        ec += qq * erfc_br * rinv;
-                    }
-                }
+    for (int k = lane; k < n; k += 2 * kWave) {
+        const int kb = k + kWave;
+        const bool has_b = kb < n;
+        const int ta = row[k];
+        const int tb = row[has_b ? kb : k];
+        const int qa = src_index<STAGED>(ta, nr, r_first, r_off);
+        const int qb = src_index<STAGED>(tb, nr, r_first, r_off);
+        const float4 pa = src.pos[qa];
+        const float4 pb = src.pos[qb];
+        const float2 aba = trow[src.typ[qa]];
+        const float2 abb = trow[src.typ[qb]];
+        const float3 da = disp_pbc<ORTHO>(make_float3(pa.x, pa.y, pa.z), pi, c, r);
+        const float3 db = disp_pbc<ORTHO>(make_float3(pb.x, pb.y, pb.z), pi, c, r);
+        // an absent second entry is given r2 = cutoff^2 threshold: out of range
+        const float r2a = norm2_exact(da);
+        const float r2b = has_b ? norm2_exact(db) : rc2_thr;
+        const PairTerm<NEED_E, NEED_V, FAST> A(da, r2a, pi4.w, pa.w, aba, rc2_thr, beta);
+        const PairTerm<NEED_E, NEED_V, FAST> B(db, r2b, pi4.w, pb.w, abb, rc2_thr, beta);
+        fx += A.lx + B.lx;
+        fy += A.ly + B.ly;
+        fz += A.lz + B.lz;
+        if (NEED_E || NEED_V) {
+            // energies and virial in the row of the lower global id
+            const bool ca = A.in && src.gid[qa] > gi;
+            const bool cb = B.in && src.gid[qb] > gi;
+            if (NEED_V) {
+                const float ax = ca ? A.lx : 0.f, ay = ca ? A.ly : 0.f, az = ca ? A.lz : 0.f;
+                const float bx = cb ? B.lx : 0.f, by = cb ? B.ly : 0.f, bz = cb ? B.lz : 0.f;
+                v0 -= ax * da.x + bx * db.x;
+                v1 -= (ax * da.y + ay * da.x) + (bx * db.y + by * db.x);
+                v2 -= ay * da.y + by * db.y;
+                v3 -= (ax * da.z + az * da.x) + (bx * db.z + bz * db.x);
+                v4 -= (ay * da.z + az * da.y) + (by * db.z + bz * db.y);
+                v5 -= az * da.z + bz * db.z;
+            }
+            if (NEED_E) {
+                elj += (ca ? A.elj : 0.f) + (cb ? B.elj : 0.f);
+                ec += (ca ? A.ec : 0.f) + (cb ? B.ec : 0.f);
             }
         }
     }
@@ -199,6 +268,57 @@ __global__ void __launch_bounds__(256) k_lj_rows(
             v[0] = (float)dv0; v[1] = (float)dv1; v[2] = (float)dv2;
             v[3] = (float)dv3; v[4] = (float)dv4; v[5] = (float)dv5;
         }
+    }
+}
+
+template <bool ORTHO, bool NEED_E, bool NEED_V, bool FAST>
+__global__ void __launch_bounds__(512) k_lj_cells(
+    const int* __restrict__ start, const int* __restrict__ nranges, int max_ranges,
+    const int* __restrict__ g_rfirst, const int* __restrict__ g_roff,
+    const int* __restrict__ list_count, const unsigned short* __restrict__ list, int cap,
+    const float4* __restrict__ pos, const int* __restrict__ ptype, const int* __restrict__ pgid,
+    const float2* __restrict__ AB_sq, int T, LT6 c, LT6 r, float rc2_thr, float beta, int stage,
+    float4* __restrict__ facc, float2* __restrict__ eacc, float* __restrict__ vacc) {
+    extern __shared__ float4 cpos[];
+    int* ctyp = reinterpret_cast<int*>(cpos + stage);
+    int* cgid = ctyp + stage;
+    float2* cab = reinterpret_cast<float2*>(ctyp + ((NEED_E || NEED_V) ? 2 : 1) * stage);
+    __shared__ int r_first[kMaxRanges];
+    __shared__ int r_off[kMaxRanges + 1];
+    const int cell = blockIdx.x;
+    const int a0 = start[cell], a1 = start[cell + 1];
+    if (a0 == a1) return;
+    const int nr = nranges[cell];
+    for (int k = threadIdx.x; k < nr; k += blockDim.x) r_first[k] = g_rfirst[(size_t)cell * max_ranges + k];
+    for (int k = threadIdx.x; k <= nr; k += blockDim.x) r_off[k] = g_roff[(size_t)cell * (max_ranges + 1) + k];
+    for (int k = threadIdx.x; k < T * T; k += blockDim.x) cab[k] = AB_sq[k];
+    __syncthreads();
+    const int total = r_off[nr];
+    const bool staged = total <= stage;
+    if (staged) {
+        for (int t = threadIdx.x; t < total; t += blockDim.x) {
+            const int q = stream_to_sorted(t, nr, r_first, r_off);
+            cpos[t] = pos[q];
+            ctyp[t] = ptype[q];
+            if (NEED_E || NEED_V) cgid[t] = pgid[q];
+        }
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave, nwave = blockDim.x / kWave;
+    for (int p = a0 + wave; p < a1; p += nwave) {
+        const float2* trow = cab + ptype[p] * T;
+        const int gi = (NEED_E || NEED_V) ? pgid[p] : 0;
+        const int n = list_count[p];
+        const unsigned short* row = list + (size_t)p * cap;
+        if (staged)
+            lj_row<ORTHO, NEED_E, NEED_V, FAST, true>(p, lane, n, row, PairSrc{cpos, ctyp, cgid}, nr,
+                                                      r_first, r_off, pos, trow, gi, c, r, rc2_thr,
+                                                      beta, facc, eacc, vacc);
+        else
+            lj_row<ORTHO, NEED_E, NEED_V, FAST, false>(p, lane, n, row, PairSrc{pos, ptype, pgid}, nr,
+                                                       r_first, r_off, pos, trow, gi, c, r, rc2_thr,
+                                                       beta, facc, eacc, vacc);
     }
 }
 
@@ -262,18 +382,28 @@ __global__ void k_sum_f32(int n, const float* __restrict__ a, double* out) {
 }
 
 template <bool ORTHO>
-int launch_rows(sponge_lj* lj, int nrows, const int* cnt, const int* list, int cap, LT6 c, LT6 r,
-                float beta, int need_e, int need_v, hipStream_t s) {
-    dim3 grid(grid_for(nrows, 4)), block(256);
+int launch_rows(sponge_lj* lj, sponge_nl* nl, LT6 c, LT6 r, float beta, int need_e, int need_v,
+                hipStream_t s) {
     ScopedTimer timer("lj_pairs", s);
-#define SA_LJ_ROWS(E, V)                                                                        \
-    hipLaunchKernelGGL((k_lj_rows<ORTHO, E, V>), grid, block, 0, s, nrows, cnt, list, cap,      \
-                       lj->d_pos, lj->d_ptype, lj->d_pgid, lj->d_AB, c, r, lj->rc2_thr, beta,   \
-                       lj->d_facc, lj->d_eacc, lj->d_vacc)
-    if (need_e && need_v) SA_LJ_ROWS(true, true);
-    else if (need_e) SA_LJ_ROWS(true, false);
-    else if (need_v) SA_LJ_ROWS(false, true);
-    else SA_LJ_ROWS(false, false);
+    const size_t sh = (sizeof(float4) + ((need_e || need_v) ? 2 : 1) * sizeof(int)) * lj->stage +
+                      sizeof(float2) * lj->T * lj->T;
+    const bool fast = beta * lj->cutoff <= kErfcxMax;
+#define SA_LJ_ROWS(E, V, F)                                                                         \
+    hipLaunchKernelGGL((k_lj_cells<ORTHO, E, V, F>), dim3(nl->G), dim3(512), sh, s, nl->d_cell_start, \
+                       nl->d_nranges, nl->max_ranges, nl->d_rfirst, nl->d_roff, nl->d_list_count,    \
+                       nl->d_list, nl->cap, lj->d_pos, lj->d_ptype, lj->d_pgid, lj->d_AB_sq, lj->T,  \
+                       c, r, lj->rc2_thr, beta, lj->stage, lj->d_facc, lj->d_eacc, lj->d_vacc)
+#define SA_LJ_ROWS_EV(F)                        \
+    if (need_e && need_v) SA_LJ_ROWS(true, true, F);  \
+    else if (need_e) SA_LJ_ROWS(true, false, F);      \
+    else if (need_v) SA_LJ_ROWS(false, true, F);      \
+    else SA_LJ_ROWS(false, false, F)
+    if (fast) {
+        SA_LJ_ROWS_EV(true);
+    } else {
+        SA_LJ_ROWS_EV(false);
+    }
+#undef SA_LJ_ROWS_EV
 #undef SA_LJ_ROWS
     SA_LAUNCH_CHECK();
     return 0;
@@ -315,6 +445,7 @@ int finalize(sponge_lj* lj, int N, const int* sidx, int need_e, int need_v, floa
 
 void lj_free(sponge_lj* lj) {
     dfree(lj->d_AB);
+    dfree(lj->d_AB_sq);
     dfree(lj->d_type);
     dfree(lj->d_pos);
     dfree(lj->d_ptype);
@@ -335,6 +466,8 @@ extern "C" int sponge_lj_initial(sponge_lj** out, int atom_numbers, int atom_typ
         !(cutoff > 0))
         return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
     *out = nullptr;
+    // the pair kernel keeps the square (A, B) table of the types in LDS
+    if (atom_type_numbers > kMaxLdsTypes) return SPONGE_AMD_ERROR_NOT_IMPLEMENTED;
     const int N = atom_numbers, T = atom_type_numbers, P = T * (T + 1) / 2;
     for (int i = 0; i < N; ++i)
         if (atom_LJ_type[i] < 0 || atom_LJ_type[i] >= T) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
@@ -366,9 +499,15 @@ extern "C" int sponge_lj_initial(sponge_lj** out, int atom_numbers, int atom_typ
     lj->long_range_factor = f;
     std::vector<float2> ab(P);
     for (int k = 0; k < P; ++k) ab[k] = make_float2(LJ_A[k], LJ_B[k]);
+    std::vector<float2> ab_sq((size_t)T * T);
+    for (int a = 0; a < T; ++a)
+        for (int b = 0; b < T; ++b) ab_sq[(size_t)a * T + b] = ab[lj_pair_index(a, b)];
+    lj->T = T;
     int e = 0;
     e |= dmalloc(&lj->d_AB, P);
+    e |= dmalloc(&lj->d_AB_sq, (size_t)T * T);
     e |= dmalloc(&lj->d_type, N);
+    lj->stage = stage_entries(sizeof(float4) + 2 * sizeof(int), kPairStage, sizeof(float2) * T * T);
     e |= dmalloc(&lj->d_pos, N);
     e |= dmalloc(&lj->d_ptype, N);
     e |= dmalloc(&lj->d_pgid, N);
@@ -384,6 +523,9 @@ extern "C" int sponge_lj_initial(sponge_lj** out, int atom_numbers, int atom_typ
     }
     hipError_t he = hipMemcpyAsync(lj->d_AB, ab.data(), P * sizeof(float2), hipMemcpyHostToDevice,
                                    lj->stream);
+    if (he == hipSuccess)
+        he = hipMemcpyAsync(lj->d_AB_sq, ab_sq.data(), ab_sq.size() * sizeof(float2),
+                            hipMemcpyHostToDevice, lj->stream);
     if (he == hipSuccess)
         he = hipMemcpyAsync(lj->d_type, atom_LJ_type, N * sizeof(int), hipMemcpyHostToDevice,
                             lj->stream);
@@ -430,11 +572,9 @@ extern "C" int sponge_lj_pme_direct_force(sponge_lj* lj, sponge_nl* nl, int atom
     int rc = nl_pack_for_pairs(nl, crd, charge, lj->d_type, lj->d_pos, lj->d_ptype, lj->d_pgid, s);
     if (rc) return rc;
     if (lt_is_orthorhombic(c))
-        rc = launch_rows<true>(lj, lj->N, nl->d_list_count, nl->d_list, nl->cap, c, r, pme_beta,
-                               need_atom_energy, need_virial, s);
+        rc = launch_rows<true>(lj, nl, c, r, pme_beta, need_atom_energy, need_virial, s);
     else
-        rc = launch_rows<false>(lj, lj->N, nl->d_list_count, nl->d_list, nl->cap, c, r, pme_beta,
-                                need_atom_energy, need_virial, s);
+        rc = launch_rows<false>(lj, nl, c, r, pme_beta, need_atom_energy, need_virial, s);
     if (rc) return rc;
     return finalize(lj, lj->N, nl->d_sidx, need_atom_energy, need_virial, frc, atom_energy,
                     atom_virial, atom_direct_pme_energy, s);

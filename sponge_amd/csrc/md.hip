@@ -51,6 +51,7 @@ struct sponge_md {
     long long step = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    GraphTimers graph_timers;      // timer event nodes inside the captured step
     int use_graph = 0;
     int own_stream = 0;  // the driver created its stream (graph capture needs one)
 };
@@ -206,11 +207,16 @@ void triangle_geometry(double dAB, double dAC, double dBC, double mA, double mB,
     g[4] = (float)Y;
 }
 
-void md_free(sponge_md* m) {
+void drop_graph(sponge_md* m) {
     if (m->graph_exec) (void)hipGraphExecDestroy(m->graph_exec);
     if (m->graph) (void)hipGraphDestroy(m->graph);
     m->graph_exec = nullptr;
     m->graph = nullptr;
+    m->graph_timers = GraphTimers();
+}
+
+void md_free(sponge_md* m) {
+    drop_graph(m);
     sponge_nl_clear(m->nl);
     sponge_lj_clear(m->lj);
     sponge_pme_clear(m->pme);
@@ -414,6 +420,8 @@ extern "C" int sponge_md_run(sponge_md* m, int steps, int energy_interval, int u
                                m->nl->opt.refresh_interval <= 0 &&
                                m->pme->opt.update_interval == 1;
         if (graphable) {
+            // timers switched on or off since the capture: capture again
+            if (m->graph_exec && m->graph_timers.timing != g_timing) drop_graph(m);
             if (!m->graph_exec) {
                 SA_CHECK(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
                 int rc = md_step(m, st, 0);
@@ -425,7 +433,13 @@ extern "C" int sponge_md_run(sponge_md* m, int steps, int energy_interval, int u
                     return rc ? rc : (int)he;
                 }
                 m->graph = g;
+                rc = graph_timers_collect(g, m->graph_timers);
+                if (rc) return rc;
                 SA_CHECK(hipGraphInstantiate(&m->graph_exec, g, nullptr, nullptr, 0));
+            }
+            if (!m->graph_timers.nodes.empty()) {
+                int rc = graph_timers_arm(m->graph_exec, m->graph_timers);
+                if (rc) return rc;
             }
             SA_CHECK(hipGraphLaunch(m->graph_exec, m->stream));
         } else {

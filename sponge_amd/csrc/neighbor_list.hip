@@ -27,6 +27,15 @@
 
 using namespace sa;
 
+int sa::stage_entries(size_t bytes_per_entry, int want, size_t reserve) {
+    int dev = 0, lds = 65536;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+        lds = 65536;
+    const long fit = ((long)lds - 4096 - (long)reserve) / (long)bytes_per_entry;
+    return (int)(std::max(64L, std::min((long)want, fit)) & ~63L);
+}
+
 namespace {
 
 constexpr int kMaxGridNeighbors = 192;  // MAX_GRID_NEIGHBORS, neighbor_list.cpp:5
@@ -124,8 +133,9 @@ __global__ void __launch_bounds__(1024) k_nl_scan(const int* __restrict__ need, 
 __global__ void k_nl_scatter(const int* __restrict__ need, int N, const float* __restrict__ crd,
                              const int* __restrict__ atom_local,
                              const int* __restrict__ atom_cell, const int* __restrict__ atom_slot,
-                             const int* __restrict__ start, int* __restrict__ sidx,
-                             int* __restrict__ rank, float4* __restrict__ spos) {
+                             const int* __restrict__ start, const int2* __restrict__ gexw,
+                             int* __restrict__ sidx, int* __restrict__ rank,
+                             float4* __restrict__ spos, int2* __restrict__ sexw) {
     if (*need == 0) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
@@ -134,169 +144,222 @@ __global__ void k_nl_scatter(const int* __restrict__ need, int N, const float* _
     sidx[p] = i;
     rank[i] = p;
     spos[p] = make_float4(crd[3 * i], crd[3 * i + 1], crd[3 * i + 2], __int_as_float(gi));
+    sexw[p] = gexw ? gexw[gi] : make_int2(1, 0);
 }
 
-// Exclusion of the unordered pair {gi, gj} as the reference decides it for
-// the half-list row of the lower id: the higher id must lie inside
-// [list[first], list[last]] of the lower id's exclusion list and equal one of
-// its entries (Delete_Excluded_Atoms_Serial_In_Neighbor_List,
-// neighbor_list.cpp:736-807).
-__device__ __forceinline__ bool excluded_in(int lo_id, int hi_id, const int* __restrict__ ex_start,
-                                            const int* __restrict__ ex_list,
-                                            const int* __restrict__ ex_num) {
-    const int n = ex_num[lo_id];
+// Combined exclusion window of every atom: the smallest and largest id among
+// the atoms of its own exclusion list and the atoms whose lists hold it.  A
+// pair (i, j) with j outside i's combined window cannot be excluded, so the
+// search tests the reference's exclusion rule only inside it.
+__global__ void k_nl_exwin_init(int N, int2* __restrict__ gexw) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) gexw[i] = make_int2(0x7fffffff, -1);
+}
+
+__global__ void k_nl_exwin(int N, const int* __restrict__ ex_start, const int* __restrict__ ex_list,
+                           const int* __restrict__ ex_num, int2* __restrict__ gexw) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= N) return;
+    const int n = ex_num[a], s0 = ex_start[a];
+    for (int e = s0; e < s0 + n; ++e) {
+        const int b = ex_list[e];
+        if (b < 0 || b >= N) continue;
+        atomicMin(&gexw[a].x, b);
+        atomicMax(&gexw[a].y, b);
+        atomicMin(&gexw[b].x, a);
+        atomicMax(&gexw[b].y, a);
+    }
+}
+
+// The reference's exclusion rule for the unordered pair {a, b}, applied in the
+// half-list row of the lower id lo (Delete_Excluded_Atoms_Serial_In_Neighbor_
+// List, neighbor_list.cpp:736-807): the higher id must lie inside the window
+// [list[first], list[last]] of lo's exclusion list and be one of its entries.
+__device__ __noinline__ bool excluded_pair(int a, int b, const int* __restrict__ ex_start,
+                                           const int* __restrict__ ex_list,
+                                           const int* __restrict__ ex_num) {
+    const int lo = a < b ? a : b, hi = a < b ? b : a;
+    const int n = ex_num[lo];
     if (n <= 0) return false;
-    const int s = ex_start[lo_id];
-    if (hi_id < ex_list[s] || hi_id > ex_list[s + n - 1]) return false;
-    for (int e = s; e < s + n; ++e)
-        if (ex_list[e] == hi_id) return true;
+    const int s0 = ex_start[lo];
+    if (hi < ex_list[s0] || hi > ex_list[s0 + n - 1]) return false;
+    for (int e = s0; e < s0 + n; ++e)
+        if (ex_list[e] == hi) return true;
     return false;
 }
 
-// One workgroup per cell.  The cell's stencil -- runs of consecutive cells,
-// hence contiguous ranges of the sorted array -- is staged once into LDS
-// (up to kStage candidates; any excess is read from L2), then each wavefront
-// takes one i atom of the cell at a time and sweeps the staged candidates 64
-// per pass.  Acceptance is compacted by ballot + mbcnt into the i row.
-// Rows hold BOTH directions (full list), so the pair kernel can own its rows
-// without atomics; the reference half list is the subset global(j) >
-// global(i) (k_nl_export).
-constexpr int kStage = 3072;
-constexpr int kMaxRangesLds = 256;
+// Candidate stream of a cell: its stencil runs of consecutive cells, i.e.
+// contiguous ranges [first_k, first_k + len_k) of the sorted array,
+// concatenated.  Built in LDS by the block (loads in parallel, scan by one
+// thread over at most kMaxRanges runs).
+
+__device__ __forceinline__ int cell_stream(int cell, const int* __restrict__ start,
+                                           const int2* __restrict__ ranges,
+                                           const int* __restrict__ nranges, int max_ranges,
+                                           int* r_first, int* r_off) {
+    const int nr = nranges[cell];
+    for (int k = threadIdx.x; k < nr; k += blockDim.x) {
+        const int2 cr = ranges[(size_t)cell * max_ranges + k];
+        const int f = start[cr.x];
+        r_first[k] = f;
+        r_off[k + 1] = start[cr.y] - f;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        r_off[0] = 0;
+        for (int k = 0; k < nr; ++k) r_off[k + 1] += r_off[k];
+    }
+    __syncthreads();
+    return nr;
+}
+
+// One workgroup (8 wavefronts) per cell.  The cell's candidate stream (sorted
+// positions with the global id in w) is staged into LDS once; each wavefront
+// then takes up to kSearchRows i atoms of the cell and sweeps the stream 64
+// candidates per pass, testing every staged candidate against all of its
+// rows (one LDS read feeds kSearchRows independent tests).  The pair test is
+// the reference's Find_Neighbors_Gridly criterion (Get_Displacement, r^2 <
+// cutoff_skin_square) and, inside the row's combined exclusion window, its
+// exclusion rule.  Rows hold both directions as 16-bit stream positions.
+constexpr int kSearchStage = 3584;  // covers the stencil streams of water at 12 A list radius
+constexpr int kSearchRows = 4;
 
 template <bool ORTHO>
-__global__ void __launch_bounds__(256) k_nl_search_cells(
-    const int* __restrict__ need, const float4* __restrict__ spos,
+__global__ void __launch_bounds__(512) k_nl_search_cells(
+    const int* __restrict__ need, const float4* __restrict__ spos, const int2* __restrict__ sexw,
     const int* __restrict__ start, const int2* __restrict__ ranges,
     const int* __restrict__ nranges, int max_ranges, LT6 c, LT6 r, float thr,
     const int* __restrict__ ex_start, const int* __restrict__ ex_list,
-    const int* __restrict__ ex_num, int cap, int* __restrict__ list_count,
-    int* __restrict__ list, int* __restrict__ flags) {
+    const int* __restrict__ ex_num, int cap, int stage, int* __restrict__ list_count,
+    unsigned short* __restrict__ list, int* __restrict__ g_rfirst, int* __restrict__ g_roff,
+    int* __restrict__ flags) {
     if (*need == 0) return;
     extern __shared__ float4 cand[];
-    __shared__ int r_first[kMaxRangesLds];  // sorted index of the run's first atom
-    __shared__ int r_off[kMaxRangesLds + 1];  // offset of the run in the staged stream
+    __shared__ int r_first[kMaxRanges];
+    __shared__ int r_off[kMaxRanges + 1];
     const int cell = blockIdx.x;
+    const int nr = cell_stream(cell, start, ranges, nranges, max_ranges, r_first, r_off);
+    const int total = r_off[nr];
+    // the stream tables are kept for the pair and export kernels
+    for (int k = threadIdx.x; k < nr; k += blockDim.x) g_rfirst[(size_t)cell * max_ranges + k] = r_first[k];
+    for (int k = threadIdx.x; k <= nr; k += blockDim.x) g_roff[(size_t)cell * (max_ranges + 1) + k] = r_off[k];
     const int a0 = start[cell], a1 = start[cell + 1];
-    if (a0 == a1) return;
-    const int nr = nranges[cell];
-    if (threadIdx.x == 0) {
-        int off = 0;
-        for (int k = 0; k < nr; ++k) {
-            const int2 cr = ranges[(size_t)cell * max_ranges + k];
-            r_first[k] = start[cr.x];
-            r_off[k] = off;
-            off += start[cr.y] - start[cr.x];
-        }
-        r_off[nr] = off;
+    if (total > 65535) {
+        // positions no longer fit 16 bits: empty rows and a raised flag
+        for (int p = a0 + (int)threadIdx.x; p < a1; p += blockDim.x) list_count[p] = 0;
+        if (threadIdx.x == 0) atomicOr(flags, SPONGE_AMD_NL_GRID_OVERFLOW | SPONGE_AMD_NL_LIST_OVERFLOW);
+        return;
     }
-    __syncthreads();
-    for (int k = 0; k < nr; ++k) {
-        const int len = r_off[k + 1] - r_off[k];
-        for (int u = threadIdx.x; u < len; u += blockDim.x) {
-            const int t = r_off[k] + u;
-            if (t < kStage) cand[t] = spos[r_first[k] + u];
-        }
-    }
+    const int staged = total < stage ? total : stage;
+    for (int t = threadIdx.x; t < staged; t += blockDim.x)
+        cand[t] = spos[stream_to_sorted(t, nr, r_first, r_off)];
     __syncthreads();
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave, nwave = blockDim.x / kWave;
-    for (int p = a0 + wave; p < a1; p += nwave) {
-        const float4 pi4 = spos[p];
-        const float3 pi = make_float3(pi4.x, pi4.y, pi4.z);
-        const int gi = __float_as_int(pi4.w);
-        int ex_n = 0, ex_s = 0, ex_lo = 0, ex_hi = -1;
-        if (ex_num) {
-            ex_n = ex_num[gi];
-            if (ex_n > 0) {
-                ex_s = ex_start[gi];
-                ex_lo = ex_list[ex_s];
-                ex_hi = ex_list[ex_s + ex_n - 1];
-            }
+    const int rows = a1 - a0;
+    for (int k0 = 0; wave + k0 * nwave < rows; k0 += kSearchRows) {
+        float3 pi[kSearchRows];
+        int gi[kSearchRows], cnt[kSearchRows];
+        int2 wi[kSearchRows];
+        bool act[kSearchRows];
+#pragma unroll
+        for (int u = 0; u < kSearchRows; ++u) {
+            const int row = wave + (k0 + u) * nwave;
+            act[u] = row < rows;
+            const int p = a0 + (act[u] ? row : 0);
+            const float4 v = spos[p];
+            pi[u] = make_float3(v.x, v.y, v.z);
+            gi[u] = __float_as_int(v.w);
+            wi[u] = sexw[p];
+            cnt[u] = 0;
         }
-        int* out = list + (size_t)p * cap;
-        int count = 0;
-        for (int k = 0; k < nr; ++k) {
-            const int off = r_off[k], len = r_off[k + 1] - off, first = r_first[k];
-            for (int base = 0; base < len; base += kWave) {
-                const int u = base + lane;
-                bool ok = false;
-                if (u < len) {
-                    const int t = off + u;
-                    const float4 pj4 = t < kStage ? cand[t] : spos[first + u];
-                    const int gj = __float_as_int(pj4.w);
-                    if (gj != gi) {
-                        // Get_Displacement(crd_i, crd_j) as in Find_Neighbors_Gridly
-                        const float3 d = disp_pbc<ORTHO>(pi, make_float3(pj4.x, pj4.y, pj4.z), c, r);
-                        if (norm2_exact(d) < thr) {
-                            ok = true;
-                            if (ex_num) {
-                                if (gj > gi) {
-                                    if (ex_n > 0 && gj >= ex_lo && gj <= ex_hi) {
-                                        for (int e = ex_s; e < ex_s + ex_n; ++e)
-                                            if (ex_list[e] == gj) ok = false;
-                                    }
-                                } else if (excluded_in(gj, gi, ex_start, ex_list, ex_num)) {
-                                    ok = false;
-                                }
-                            }
-                        }
-                    }
-                }
+        for (int base = 0; base < total; base += kWave) {
+            const int t = base + lane;
+            const bool in = t < total;
+            float4 pj4 = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
+            if (t < staged) pj4 = cand[t];
+            else if (in) pj4 = spos[stream_to_sorted(t, nr, r_first, r_off)];
+            const float3 pj = make_float3(pj4.x, pj4.y, pj4.z);
+            const int gj = __float_as_int(pj4.w);
+#pragma unroll
+            for (int u = 0; u < kSearchRows; ++u) {
+                if (!act[u]) continue;
+                // Get_Displacement(crd_i, crd_j) as in Find_Neighbors_Gridly
+                const float3 d = disp_pbc<ORTHO>(pi[u], pj, c, r);
+                bool ok = in && gj != gi[u] && norm2_exact(d) < thr;
+                if (ok && gj >= wi[u].x && gj <= wi[u].y)
+                    ok = !excluded_pair(gi[u], gj, ex_start, ex_list, ex_num);
                 const unsigned long long m = __ballot(ok);
                 if (ok) {
-                    const int pos = count + lanes_below(m);
-                    if (pos < cap) out[pos] = first + u;
+                    const int pos = cnt[u] + lanes_below(m);
+                    if (pos < cap) list[(size_t)(a0 + wave + (k0 + u) * nwave) * cap + pos] = (unsigned short)t;
                 }
-                count += __popcll(m);
+                cnt[u] += __popcll(m);
             }
         }
         if (lane == 0) {
-            list_count[p] = count < cap ? count : cap;
-            if (count > cap) atomicOr(flags, SPONGE_AMD_NL_LIST_OVERFLOW);
+#pragma unroll
+            for (int u = 0; u < kSearchRows; ++u) {
+                if (!act[u]) continue;
+                const int p = a0 + wave + (k0 + u) * nwave;
+                list_count[p] = cnt[u] < cap ? cnt[u] : cap;
+                if (cnt[u] > cap) atomicOr(flags, SPONGE_AMD_NL_LIST_OVERFLOW);
+            }
         }
     }
 }
 
+
 // Reference-format export: the half rows (global(j) > global(i)) in local
-// ids, rows by local atom; one wavefront per sorted row, ballot compaction.
-__global__ void k_nl_export(int N, const int* __restrict__ sidx, const float4* __restrict__ spos,
-                            const int* __restrict__ list_count, const int* __restrict__ list,
-                            int cap, int* __restrict__ counts, int* __restrict__ serial,
-                            int max_out, sa_atom_group* group, int* __restrict__ flags) {
+// ids, rows by local atom; one workgroup per cell, one wavefront per row,
+// ballot compaction.
+__global__ void __launch_bounds__(256) k_nl_export(
+    const int* __restrict__ sidx, const float4* __restrict__ spos, const int* __restrict__ start,
+    const int* __restrict__ nranges, int max_ranges, const int* __restrict__ g_rfirst,
+    const int* __restrict__ g_roff, const int* __restrict__ list_count,
+    const unsigned short* __restrict__ list, int cap, int* __restrict__ counts,
+    int* __restrict__ serial, int max_out, sa_atom_group* group, int* __restrict__ flags) {
+    __shared__ int r_first[kMaxRanges];
+    __shared__ int r_off[kMaxRanges + 1];
+    const int cell = blockIdx.x;
+    const int nr = nranges[cell];
+    for (int k = threadIdx.x; k < nr; k += blockDim.x) r_first[k] = g_rfirst[(size_t)cell * max_ranges + k];
+    for (int k = threadIdx.x; k <= nr; k += blockDim.x) r_off[k] = g_roff[(size_t)cell * (max_ranges + 1) + k];
+    __syncthreads();
     const int lane = threadIdx.x & (kWave - 1);
-    const int p = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
-    if (p >= N) return;
-    const int i = sidx[p];
-    const int gi = __float_as_int(spos[p].w);
-    const int n = list_count[p];
-    const int* in = list + (size_t)p * cap;
-    int* o = serial + (size_t)i * max_out;
-    int count = 0;
-    for (int base = 0; base < n; base += kWave) {
-        const int k = base + lane;
-        int q = 0;
-        bool keep = false;
-        if (k < n) {
-            q = in[k];
-            keep = __float_as_int(spos[q].w) > gi;
+    const int wave = threadIdx.x / kWave, nwave = blockDim.x / kWave;
+    for (int p = start[cell] + wave; p < start[cell + 1]; p += nwave) {
+        const int i = sidx[p];
+        const int gi = __float_as_int(spos[p].w);
+        const int n = list_count[p];
+        const unsigned short* in = list + (size_t)p * cap;
+        int* o = serial + (size_t)i * max_out;
+        int count = 0;
+        for (int base = 0; base < n; base += kWave) {
+            const int k = base + lane;
+            int q = 0;
+            bool keep = false;
+            if (k < n) {
+                q = stream_to_sorted(in[k], nr, r_first, r_off);
+                keep = __float_as_int(spos[q].w) > gi;
+            }
+            const unsigned long long m = __ballot(keep);
+            if (keep) {
+                const int pos = count + lanes_below(m);
+                if (pos < max_out) o[pos] = sidx[q];
+            }
+            count += __popcll(m);
         }
-        const unsigned long long m = __ballot(keep);
-        if (keep) {
-            const int pos = count + lanes_below(m);
-            if (pos < max_out) o[pos] = sidx[q];
+        if (lane == 0) {
+            const int nout = count < max_out ? count : max_out;
+            counts[i] = nout;
+            if (group) {
+                group[i].atom_numbers = nout;
+                group[i].ghost_numbers = 0;
+                group[i].atom_serial = o;
+            }
+            if (count > max_out) atomicOr(flags, SPONGE_AMD_NL_LIST_OVERFLOW);
         }
-        count += __popcll(m);
-    }
-    if (lane == 0) {
-        const int nout = count < max_out ? count : max_out;
-        counts[i] = nout;
-        if (group) {
-            group[i].atom_numbers = nout;
-            group[i].ghost_numbers = 0;
-            group[i].atom_serial = o;
-        }
-        if (count > max_out) atomicOr(flags, SPONGE_AMD_NL_LIST_OVERFLOW);
     }
 }
 
@@ -376,6 +439,10 @@ void nl_free(sponge_nl* nl) {
     dfree(nl->d_nranges);
     dfree(nl->d_list_count);
     dfree(nl->d_list);
+    dfree(nl->d_sexw);
+    dfree(nl->d_gexw);
+    dfree(nl->d_rfirst);
+    dfree(nl->d_roff);
     dfree(nl->d_exp_serial);
     dfree(nl->d_exp_count);
     dfree(nl->d_atom_group);
@@ -432,7 +499,8 @@ extern "C" int sponge_nl_initial(sponge_nl** out, int atom_numbers, float cutoff
     const int N = nl->N, G = nl->G;
     // rows hold both directions: twice the reference's per-atom half capacity
     nl->cap = 2 * nl->opt.max_neighbor_numbers;
-    if (nl->max_ranges > 256) {
+    nl->stage = stage_entries(sizeof(float4), kSearchStage);
+    if (nl->max_ranges > kMaxRanges) {
         delete nl;
         return SPONGE_AMD_ERROR_OVERFLOW;
     }
@@ -452,6 +520,10 @@ extern "C" int sponge_nl_initial(sponge_nl** out, int atom_numbers, float cutoff
     e |= dmalloc(&nl->d_nranges, G);
     e |= dmalloc(&nl->d_list_count, N);
     e |= dmalloc(&nl->d_list, (size_t)N * nl->cap);
+    e |= dmalloc(&nl->d_sexw, N);
+    e |= dmalloc(&nl->d_gexw, N);
+    e |= dmalloc(&nl->d_rfirst, (size_t)G * nl->max_ranges);
+    e |= dmalloc(&nl->d_roff, (size_t)G * (nl->max_ranges + 1));
     if (e) {
         nl_free(nl);
         delete nl;
@@ -522,25 +594,40 @@ extern "C" int sponge_nl_update(sponge_nl* nl, const int* atom_local, int local_
     hipLaunchKernelGGL(k_nl_scan, dim3(1), dim3(1024), 0, s, nl->d_need, nl->G, nl->d_cell_count,
                        nl->d_cell_start, nl->opt.max_atom_in_grid_numbers, nl->d_flags, nl->d_rebuilds);
     SA_LAUNCH_CHECK();
+    // combined exclusion windows: the exclusion lists are topology, so the
+    // windows are derived once per set of list pointers (and again on a
+    // forced update)
+    const bool have_ex = excluded_numbers != nullptr;
+    if (have_ex && (update == SPONGE_AMD_NL_FORCED || nl->exw_src[0] != excluded_list_start ||
+                    nl->exw_src[1] != excluded_list || nl->exw_src[2] != excluded_numbers)) {
+        hipLaunchKernelGGL(k_nl_exwin_init, dim3(grid_for(N, 256)), dim3(256), 0, s, N, nl->d_gexw);
+        hipLaunchKernelGGL(k_nl_exwin, dim3(grid_for(N, 256)), dim3(256), 0, s, N,
+                           excluded_list_start, excluded_list, excluded_numbers, nl->d_gexw);
+        SA_LAUNCH_CHECK();
+        nl->exw_src[0] = excluded_list_start;
+        nl->exw_src[1] = excluded_list;
+        nl->exw_src[2] = excluded_numbers;
+    }
     hipLaunchKernelGGL(k_nl_scatter, dim3(grid_for(N, 256)), dim3(256), 0, s, nl->d_need, N, crd,
-                       atom_local, nl->d_atom_cell, nl->d_atom_slot, nl->d_cell_start, nl->d_sidx,
-                       nl->d_rank, nl->d_spos);
+                       atom_local, nl->d_atom_cell, nl->d_atom_slot, nl->d_cell_start,
+                       have_ex ? nl->d_gexw : nullptr, nl->d_sidx, nl->d_rank, nl->d_spos,
+                       nl->d_sexw);
     SA_LAUNCH_CHECK();
     {
         ScopedTimer timer("nl_search", s);
-        const size_t sh = sizeof(float4) * kStage;
+        const size_t sh = sizeof(float4) * nl->stage;
         if (nl->ortho)
-            hipLaunchKernelGGL(k_nl_search_cells<true>, dim3(nl->G), dim3(256), sh, s, nl->d_need,
-                               nl->d_spos, nl->d_cell_start, nl->d_ranges, nl->d_nranges,
-                               nl->max_ranges, c, r, nl->thr, excluded_list_start, excluded_list,
-                               excluded_numbers, nl->cap, nl->d_list_count, nl->d_list,
-                               nl->d_flags);
+            hipLaunchKernelGGL(k_nl_search_cells<true>, dim3(nl->G), dim3(512), sh, s, nl->d_need,
+                               nl->d_spos, nl->d_sexw, nl->d_cell_start, nl->d_ranges,
+                               nl->d_nranges, nl->max_ranges, c, r, nl->thr, excluded_list_start,
+                               excluded_list, excluded_numbers, nl->cap, nl->stage, nl->d_list_count,
+                               nl->d_list, nl->d_rfirst, nl->d_roff, nl->d_flags);
         else
-            hipLaunchKernelGGL(k_nl_search_cells<false>, dim3(nl->G), dim3(256), sh, s, nl->d_need,
-                               nl->d_spos, nl->d_cell_start, nl->d_ranges, nl->d_nranges,
-                               nl->max_ranges, c, r, nl->thr, excluded_list_start, excluded_list,
-                               excluded_numbers, nl->cap, nl->d_list_count, nl->d_list,
-                               nl->d_flags);
+            hipLaunchKernelGGL(k_nl_search_cells<false>, dim3(nl->G), dim3(512), sh, s, nl->d_need,
+                               nl->d_spos, nl->d_sexw, nl->d_cell_start, nl->d_ranges,
+                               nl->d_nranges, nl->max_ranges, c, r, nl->thr, excluded_list_start,
+                               excluded_list, excluded_numbers, nl->cap, nl->stage, nl->d_list_count,
+                               nl->d_list, nl->d_rfirst, nl->d_roff, nl->d_flags);
         SA_LAUNCH_CHECK();
     }
     nl->built = 1;
@@ -565,8 +652,8 @@ extern "C" int sponge_nl_export_half_list(sponge_nl* nl, int* counts, int* seria
                                           int max_numbers) {
     if (!nl || !counts || !serial || max_numbers <= 0) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
     if (!nl->built) return SPONGE_AMD_ERROR_MISSING_COMMAND;
-    dim3 grid(grid_for(nl->N, 4)), block(256);
-    hipLaunchKernelGGL(k_nl_export, grid, block, 0, nl->stream, nl->N, nl->d_sidx, nl->d_spos,
+    hipLaunchKernelGGL(k_nl_export, dim3(nl->G), dim3(256), 0, nl->stream, nl->d_sidx, nl->d_spos,
+                       nl->d_cell_start, nl->d_nranges, nl->max_ranges, nl->d_rfirst, nl->d_roff,
                        nl->d_list_count, nl->d_list, nl->cap, counts, serial, max_numbers,
                        (sa_atom_group*)nullptr, nl->d_flags);
     SA_LAUNCH_CHECK();
@@ -583,8 +670,8 @@ extern "C" int sponge_nl_atom_group(sponge_nl* nl, void** d_nl) {
         e |= dmalloc(&nl->d_atom_group, nl->N);
         if (e) return SPONGE_AMD_ERROR_MALLOC_FAILED;
     }
-    dim3 grid(grid_for(nl->N, 4)), block(256);
-    hipLaunchKernelGGL(k_nl_export, grid, block, 0, nl->stream, nl->N, nl->d_sidx, nl->d_spos,
+    hipLaunchKernelGGL(k_nl_export, dim3(nl->G), dim3(256), 0, nl->stream, nl->d_sidx, nl->d_spos,
+                       nl->d_cell_start, nl->d_nranges, nl->max_ranges, nl->d_rfirst, nl->d_roff,
                        nl->d_list_count, nl->d_list, nl->cap, nl->d_exp_count, nl->d_exp_serial, M,
                        nl->d_atom_group, nl->d_flags);
     SA_LAUNCH_CHECK();

@@ -73,3 +73,27 @@ def test_graph_replay_matches_eager(dev):
     # bit equality
     assert np.abs(ca - cb).max() < 1e-3
     assert np.abs(va - vb).max() < 1e-2 * np.abs(va).max()
+
+
+def test_kernel_timers_inside_graph(dev):
+    """Timers recorded during capture become graph event nodes that are
+    re-armed on every replay: one timed pair-kernel launch per step."""
+    from sponge_amd import abi
+    from sponge_amd import system as S
+    s = S.config(1)
+    md = abi.MDSystem(s)
+    md.run(2, use_graph=False)
+    abi.set_kernel_timing_filter(["lj_pairs"])
+    try:
+        abi.set_kernel_timing(True)
+        md.run(1, use_graph=True)
+        abi.reset_kernel_timing()
+        md.run(7, use_graph=True)
+        ms, n = abi.kernel_timing("lj_pairs")
+        assert n == 7 and ms > 0.0, (ms, n)
+        # a filtered-out timer records nothing
+        assert abi.kernel_timing("pme_spread")[1] == 0
+    finally:
+        abi.set_kernel_timing(False)
+        abi.set_kernel_timing_filter(None)
+        abi.reset_kernel_timing()
