@@ -125,10 +125,11 @@ __global__ void __launch_bounds__(256) k_lj_pairs(
 // virials are counted in the row of the lower global id only, the
 // reference's half-list row.
 //
-// Arithmetic per in-cutoff pair: 1/r from the hardware rsqrt, erfc(beta r)
-// as exp(-(beta r)^2) erfcx(beta r) with the polynomial erfcx (FAST, used
-// when beta * cutoff <= kErfcxMax) so that one exponential serves the
-// energy and the force bracket; otherwise the library erfcf.
+// Arithmetic per in-cutoff pair on force-only steps (FAST, when beta *
+// cutoff <= kErfcxMax): 1/r from the hardware rsqrt and erfc(beta r) as
+// exp(-(beta r)^2) erfcx(beta r) with the polynomial erfcx, so one
+// exponential serves erfc and the force bracket.  Steps that need energies or
+// the virial use the reference's sqrtf, division and erfcf.
 constexpr int kPairStage = 3584;
 constexpr int kMaxLdsTypes = 64;  // the square (A, B) table lives in LDS: up to 64 LJ types
 
@@ -158,8 +159,17 @@ struct PairTerm {
     __device__ __forceinline__ PairTerm(const float3& d, float r2, float qi, float qj, float2 ab,
                                         float rc2_thr, float beta) {
         in = r2 < rc2_thr;
-        const float rinv = __builtin_amdgcn_rsqf(r2);
-        const float rabs = r2 * rinv;
+        // FAST: hardware rsqrt (1 ulp); otherwise the reference's sqrtf and
+        // division, which energies and virials need: the LJ energy is a
+        // difference of the r^-12 and r^-6 terms and amplifies a biased 1/r
+        float rinv, rabs;
+        if (FAST) {
+            rinv = __builtin_amdgcn_rsqf(r2);
+            rabs = r2 * rinv;
+        } else {
+            rabs = sqrtf(r2);
+            rinv = 1.0f / rabs;
+        }
         const float r2inv = rinv * rinv;
         const float r6inv = r2inv * r2inv * r2inv;
         const float qq = qi * qj;
@@ -189,8 +199,10 @@ struct PairTerm {
     }
 };
 
-// One row (wavefront-wide): two partner entries per lane and pass, k and
-// k + 64, loaded together so that their latencies and arithmetic overlap.
+// One row (wavefront-wide): two adjacent partner entries per lane and pass
+// (one 32-bit load: rows start 4-byte aligned since the row capacity is
+// even), with the next pass's pair loaded before the current one is used so
+// that the list's memory latency overlaps the arithmetic.
 template <bool ORTHO, bool NEED_E, bool NEED_V, bool FAST, bool STAGED>
 __device__ __forceinline__ void lj_row(int p, int lane, int n, const unsigned short* __restrict__ row,
                                        PairSrc src, int nr, const int* r_first, const int* r_off,
@@ -203,11 +215,17 @@ __device__ __forceinline__ void lj_row(int p, int lane, int n, const unsigned sh
     float fx = 0.f, fy = 0.f, fz = 0.f;
     float elj = 0.f, ec = 0.f;
     float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f, v5 = 0.f;
-    for (int k = lane; k < n; k += 2 * kWave) {
-        const int kb = k + kWave;
-        const bool has_b = kb < n;
-        const int ta = row[k];
-        const int tb = row[has_b ? kb : k];
+    const unsigned* row2 = reinterpret_cast<const unsigned*>(row);
+    const int npairs = (n + 1) >> 1;  // 32-bit words holding the row
+    const int last = npairs > 0 ? npairs - 1 : 0;
+    unsigned cur = row2[lane < last ? lane : last];
+    for (int w = lane; w - lane < npairs; w += kWave) {
+        const int wn = w + kWave;
+        unsigned nxt = row2[wn < last ? wn : last];
+        const bool has_a = w < npairs;
+        const bool has_b = 2 * w + 1 < n;
+        const int ta = (int)(cur & 0xffffu);
+        const int tb = has_b ? (int)(cur >> 16) : ta;
         const int qa = src_index<STAGED>(ta, nr, r_first, r_off);
         const int qb = src_index<STAGED>(tb, nr, r_first, r_off);
         const float4 pa = src.pos[qa];
@@ -216,8 +234,8 @@ __device__ __forceinline__ void lj_row(int p, int lane, int n, const unsigned sh
         const float2 abb = trow[src.typ[qb]];
         const float3 da = disp_pbc<ORTHO>(make_float3(pa.x, pa.y, pa.z), pi, c, r);
         const float3 db = disp_pbc<ORTHO>(make_float3(pb.x, pb.y, pb.z), pi, c, r);
-        // an absent second entry is given r2 = cutoff^2 threshold: out of range
-        const float r2a = norm2_exact(da);
+        // an absent entry is given r2 = the cutoff threshold: out of range
+        const float r2a = has_a ? norm2_exact(da) : rc2_thr;
         const float r2b = has_b ? norm2_exact(db) : rc2_thr;
         const PairTerm<NEED_E, NEED_V, FAST> A(da, r2a, pi4.w, pa.w, aba, rc2_thr, beta);
         const PairTerm<NEED_E, NEED_V, FAST> B(db, r2b, pi4.w, pb.w, abb, rc2_thr, beta);
@@ -243,6 +261,10 @@ __device__ __forceinline__ void lj_row(int p, int lane, int n, const unsigned sh
                 ec += (ca ? A.ec : 0.f) + (cb ? B.ec : 0.f);
             }
         }
+        // the loaded word is opaque from here: the compiler cannot fold the
+        // prefetch back into a load at the top of the next pass
+        asm volatile("" : "+v"(nxt));
+        cur = nxt;
     }
     fx = wave_sum(fx);
     fy = wave_sum(fy);
@@ -387,23 +409,17 @@ int launch_rows(sponge_lj* lj, sponge_nl* nl, LT6 c, LT6 r, float beta, int need
     ScopedTimer timer("lj_pairs", s);
     const size_t sh = (sizeof(float4) + ((need_e || need_v) ? 2 : 1) * sizeof(int)) * lj->stage +
                       sizeof(float2) * lj->T * lj->T;
-    const bool fast = beta * lj->cutoff <= kErfcxMax;
+    const bool fast = beta * lj->cutoff <= kErfcxMax && !need_e && !need_v;
 #define SA_LJ_ROWS(E, V, F)                                                                         \
     hipLaunchKernelGGL((k_lj_cells<ORTHO, E, V, F>), dim3(nl->G), dim3(512), sh, s, nl->d_cell_start, \
                        nl->d_nranges, nl->max_ranges, nl->d_rfirst, nl->d_roff, nl->d_list_count,    \
                        nl->d_list, nl->cap, lj->d_pos, lj->d_ptype, lj->d_pgid, lj->d_AB_sq, lj->T,  \
                        c, r, lj->rc2_thr, beta, lj->stage, lj->d_facc, lj->d_eacc, lj->d_vacc)
-#define SA_LJ_ROWS_EV(F)                        \
-    if (need_e && need_v) SA_LJ_ROWS(true, true, F);  \
-    else if (need_e) SA_LJ_ROWS(true, false, F);      \
-    else if (need_v) SA_LJ_ROWS(false, true, F);      \
-    else SA_LJ_ROWS(false, false, F)
-    if (fast) {
-        SA_LJ_ROWS_EV(true);
-    } else {
-        SA_LJ_ROWS_EV(false);
-    }
-#undef SA_LJ_ROWS_EV
+    if (fast) SA_LJ_ROWS(false, false, true);
+    else if (need_e && need_v) SA_LJ_ROWS(true, true, false);
+    else if (need_e) SA_LJ_ROWS(true, false, false);
+    else if (need_v) SA_LJ_ROWS(false, true, false);
+    else SA_LJ_ROWS(false, false, false);
 #undef SA_LJ_ROWS
     SA_LAUNCH_CHECK();
     return 0;

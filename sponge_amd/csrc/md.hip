@@ -49,9 +49,7 @@ struct sponge_md {
     float mO = 0, mH1 = 0, mH2 = 0;
     double* d_ek = nullptr;
     long long step = 0;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
-    GraphTimers graph_timers;      // timer event nodes inside the captured step
+    SegmentedCapture capture;      // the captured step (one graph, or segments between timers)
     int use_graph = 0;
     int own_stream = 0;  // the driver created its stream (graph capture needs one)
 };
@@ -207,16 +205,8 @@ void triangle_geometry(double dAB, double dAC, double dBC, double mA, double mB,
     g[4] = (float)Y;
 }
 
-void drop_graph(sponge_md* m) {
-    if (m->graph_exec) (void)hipGraphExecDestroy(m->graph_exec);
-    if (m->graph) (void)hipGraphDestroy(m->graph);
-    m->graph_exec = nullptr;
-    m->graph = nullptr;
-    m->graph_timers = GraphTimers();
-}
-
 void md_free(sponge_md* m) {
-    drop_graph(m);
+    capture_destroy(m->capture);
     sponge_nl_clear(m->nl);
     sponge_lj_clear(m->lj);
     sponge_pme_clear(m->pme);
@@ -421,27 +411,21 @@ extern "C" int sponge_md_run(sponge_md* m, int steps, int energy_interval, int u
                                m->pme->opt.update_interval == 1;
         if (graphable) {
             // timers switched on or off since the capture: capture again
-            if (m->graph_exec && m->graph_timers.timing != g_timing) drop_graph(m);
-            if (!m->graph_exec) {
-                SA_CHECK(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
-                int rc = md_step(m, st, 0);
-                hipGraph_t g = nullptr;
-                hipError_t he = hipStreamEndCapture(m->stream, &g);
-                if (rc || he != hipSuccess) {
-                    if (g) (void)hipGraphDestroy(g);
+            if (!m->capture.items.empty() && m->capture.timing != g_timing)
+                capture_destroy(m->capture);
+            if (m->capture.items.empty()) {
+                const int rc0 = capture_begin(m->capture, m->stream);
+                if (rc0) return rc0;
+                const int rc = md_step(m, st, 0);
+                const int rc2 = capture_end(m->capture, m->stream);
+                if (rc || rc2) {
+                    capture_destroy(m->capture);
                     (void)hipGetLastError();
-                    return rc ? rc : (int)he;
+                    return rc ? rc : rc2;
                 }
-                m->graph = g;
-                rc = graph_timers_collect(g, m->graph_timers);
-                if (rc) return rc;
-                SA_CHECK(hipGraphInstantiate(&m->graph_exec, g, nullptr, nullptr, 0));
             }
-            if (!m->graph_timers.nodes.empty()) {
-                int rc = graph_timers_arm(m->graph_exec, m->graph_timers);
-                if (rc) return rc;
-            }
-            SA_CHECK(hipGraphLaunch(m->graph_exec, m->stream));
+            const int rc = capture_launch(m->capture, m->stream);
+            if (rc) return rc;
         } else {
             int rc = md_step(m, st, need_e);
             if (rc) return rc;

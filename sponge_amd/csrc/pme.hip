@@ -359,7 +359,12 @@ struct Near {
     float fx, fy, fz;
 };
 
+// FMA contraction stays off here: fused, frac * n - t would subtract the
+// truncated base from the unrounded product, and an atom on a grid plane
+// (frac * n rounding up to an integer) would get f = 1 - eps on the base
+// below instead of f = 0 on its own plane: a charge one grid step away.
 __device__ __forceinline__ void near_axis(float frac, int n, int& u, float& f) {
+#pragma clang fp contract(off)
     frac *= n;
     int t = (int)frac;
     t = t < 0 ? 0 : (t < n ? t : n - 1);
@@ -370,6 +375,7 @@ __device__ __forceinline__ void near_axis(float frac, int n, int& u, float& f) {
 
 __device__ __forceinline__ Near atom_near(const float* __restrict__ crd, int i, const LT6& rc,
                                           int fx, int fy, int fz) {
+#pragma clang fp contract(off)
     const float x = crd[3 * i], y = crd[3 * i + 1], z = crd[3 * i + 2];
     // VECTOR * LTMatrix3 then frac - floorf(frac)
     float sx = x * rc.a11 + y * rc.a21 + z * rc.a31;
@@ -407,22 +413,178 @@ __device__ __forceinline__ int stencil_index(const Near& nr, int kx, int ky, int
     return (x * fy + y) * fz + z;
 }
 
-// One wavefront per atom, one lane per stencil point (lane = 16 kx + 4 ky + kz
-// as in PME_Q_Spread's k loop, :787-812).
-__global__ void __launch_bounds__(256) k_pme_spread(int N, const float* __restrict__ crd,
-                                                    const float* __restrict__ charge, LT6 rc,
-                                                    int fx, int fy, int fz,
-                                                    float* __restrict__ Q) {
-    const int lane = threadIdx.x & (kWave - 1);
-    const int kx = lane >> 4, ky = (lane >> 2) & 3, kz = lane & 3;
-    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-    const int nwaves = gridDim.x * blockDim.x / kWave;
-    for (int i = wave; i < N; i += nwaves) {
+// Brick spread.  The charge grid is tiled by bricks of Bx x By x Bz points;
+// one workgroup owns one brick, accumulates it in LDS and stores it with plain
+// writes, so the grid needs neither atomics nor a clearing pass.  An atom
+// spreads onto the points u - k (k = 0..3 per axis) below its base point u,
+// which lie in the brick of u and the brick below it on each axis; so a brick
+// receives from the atoms binned in it and in its upper neighbours (8 bins),
+// plus the overflow list.  Same weights and order as PME_Q_Spread
+// (PM_force.cpp:787-812): q Mx(kx) My(ky) Mz(kz).
+
+// Bin the atoms by the brick of their base point (and clear the FFT energy /
+// virial accumulators of this evaluation).  Slots are taken with one atomic
+// per distinct bin of a wavefront (neighbouring atoms share bins).
+__global__ void __launch_bounds__(256) k_pme_bin(int N, const float* __restrict__ crd,
+                                                 const float* __restrict__ charge, LT6 rc, int fx,
+                                                 int fy, int fz, int bx, int by, int bz, int nby,
+                                                 int nbz, int nb, int bcap, int* __restrict__ bcount,
+                                                 float4* __restrict__ bw, int* __restrict__ bu,
+                                                 float4* __restrict__ ow, int* __restrict__ ou,
+                                                 double* __restrict__ acc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 7 && blockIdx.x == 0) acc[i] = 0.0;
+    const bool live = i < N;
+    int h = -1, u = 0;
+    float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (live) {
         const Near nr = atom_near(crd, i, rc, fx, fy, fz);
-        float tq = charge[i] * bsp_w(kx, nr.fx);
-        tq = tq * bsp_w(ky, nr.fy);
-        tq = tq * bsp_w(kz, nr.fz);
-        unsafeAtomicAdd(&Q[stencil_index(nr, kx, ky, kz, fx, fy, fz)], tq);
+        h = ((nr.ux / bx) * nby + nr.uy / by) * nbz + nr.uz / bz;
+        w = make_float4(nr.fx, nr.fy, nr.fz, charge[i]);
+        u = nr.ux | (nr.uy << 10) | (nr.uz << 20);
+    }
+    unsigned long long todo = __ballot(live);
+    int slot = 0;
+    while (todo) {
+        const int leader = __builtin_ctzll(todo);
+        const int hb = __builtin_amdgcn_readlane(h, leader);
+        const unsigned long long peers = __ballot(h == hb);
+        int base = 0;
+        if (__lane_id() == leader) base = atomicAdd(&bcount[hb], __popcll(peers));
+        base = __builtin_amdgcn_readlane(base, leader);
+        if (h == hb) slot = base + lanes_below(peers);
+        todo &= ~peers;
+    }
+    if (!live) return;
+    if (slot < bcap) {
+        bw[(size_t)h * bcap + slot] = w;
+        bu[(size_t)h * bcap + slot] = u;
+    } else {
+        const int o = atomicAdd(&bcount[nb], 1);
+        ow[o] = w;
+        ou[o] = u;
+    }
+}
+
+// Does the stencil below base point u (one axis) reach brick b of size B?
+// (general form, for overflow entries whose bin is not known)
+__device__ __forceinline__ bool reaches(int u, int b, int B, int nbk) {
+    const int h = u / B;
+    if (h == b) return true;
+    const int up = b + 1 == nbk ? 0 : b + 1;
+    return h == up && u - h * B < 3;
+}
+
+__device__ __forceinline__ float lane_f(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+__global__ void __launch_bounds__(256) k_pme_spread_bricks(
+    int fx, int fy, int fz, int bx, int by, int bz, int nbx, int nby, int nbz, int nb, int bcap,
+    const int* __restrict__ bcount, const float4* __restrict__ bw, const int* __restrict__ bu,
+    const float4* __restrict__ ow, const int* __restrict__ ou, float* __restrict__ Q) {
+    extern __shared__ float acc[];
+    __shared__ int s_first[9], s_off[10], s_dx[9], s_sx[9];
+    const int b = blockIdx.x;
+    const int cz = b % nbz, cy = (b / nbz) % nby, cx = b / (nbz * nby);
+    const int X0 = cx * bx, Y0 = cy * by, Z0 = cz * bz;
+    const int npts = bx * by * bz;
+    for (int k = threadIdx.x; k < npts; k += blockDim.x) acc[k] = 0.f;
+    // sources: bins (cx + dx, cy + dy, cz + dz), d in {0, 1}, each bin once,
+    // then the overflow list; flattened so that all wavefronts share the work
+    if (threadIdx.x == 0) {
+        int ns = 0, off = 0;
+        for (int src = 0; src < 9; ++src) {
+            int first, n, d = 0, sxyz = 0;
+            if (src < 8) {
+                const int dx = src >> 2, dy = (src >> 1) & 1, dz = src & 1;
+                if ((dx && nbx == 1) || (dy && nby == 1) || (dz && nbz == 1)) continue;
+                const int sx = cx + dx == nbx ? 0 : cx + dx;
+                const int sy = cy + dy == nby ? 0 : cy + dy;
+                const int sz = cz + dz == nbz ? 0 : cz + dz;
+                const int sb = (sx * nby + sy) * nbz + sz;
+                first = sb * bcap;
+                n = min(bcount[sb], bcap);
+                d = src;
+                sxyz = sx | (sy << 10) | (sz << 20);
+            } else {
+                first = -1;  // overflow list
+                n = bcount[nb];
+                d = 8;
+            }
+            if (n <= 0) continue;
+            s_first[ns] = first;
+            s_off[ns] = off;
+            s_dx[ns] = d;
+            s_sx[ns] = sxyz;
+            off += n;
+            ++ns;
+        }
+        s_off[ns] = off;
+        for (int k = ns + 1; k < 10; ++k) s_off[k] = off;
+    }
+    __syncthreads();
+    const int total = s_off[9];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave, nwave = blockDim.x / kWave;
+    const int kx = lane >> 4, ky = (lane >> 2) & 3, kz = lane & 3;
+    for (int base = wave * kWave; base < total; base += nwave * kWave) {
+        const int g = base + lane;
+        float4 wj = make_float4(0.f, 0.f, 0.f, 0.f);
+        int uj = 0;
+        bool hit = false;
+        if (g < total) {
+            int k = 0;
+            while (s_off[k + 1] <= g) ++k;
+            const int j = g - s_off[k];
+            const int d = s_dx[k];
+            if (d < 8) {
+                const int e = s_first[k] + j;
+                wj = bw[e];
+                uj = bu[e];
+                // the bin's own brick coordinates fix the home brick of the atom
+                const int sxyz = s_sx[k];
+                const int ux = uj & 1023, uy = (uj >> 10) & 1023, uz = uj >> 20;
+                hit = (!(d & 4) || ux - (sxyz & 1023) * bx < 3) &&
+                      (!(d & 2) || uy - ((sxyz >> 10) & 1023) * by < 3) &&
+                      (!(d & 1) || uz - (sxyz >> 20) * bz < 3);
+            } else {
+                wj = ow[j];
+                uj = ou[j];
+                hit = reaches(uj & 1023, cx, bx, nbx) && reaches((uj >> 10) & 1023, cy, by, nby) &&
+                      reaches(uj >> 20, cz, bz, nbz);
+            }
+        }
+        unsigned long long m = __ballot(hit);
+        while (m) {
+            const int src_lane = __builtin_ctzll(m);
+            m &= m - 1;
+            const int u = __builtin_amdgcn_readlane(uj, src_lane);
+            const float wfx = lane_f(wj.x, src_lane);
+            const float wfy = lane_f(wj.y, src_lane);
+            const float wfz = lane_f(wj.z, src_lane);
+            const float q = lane_f(wj.w, src_lane);
+            int x = (u & 1023) - kx;
+            if (x < 0) x += fx;
+            int y = ((u >> 10) & 1023) - ky;
+            if (y < 0) y += fy;
+            int z = (u >> 20) - kz;
+            if (z < 0) z += fz;
+            x -= X0;
+            y -= Y0;
+            z -= Z0;
+            if (x >= 0 && x < bx && y >= 0 && y < by && z >= 0 && z < bz) {
+                float tq = q * bsp_w(kx, wfx);
+                tq = tq * bsp_w(ky, wfy);
+                tq = tq * bsp_w(kz, wfz);
+                atomicAdd(&acc[(x * by + y) * bz + z], tq);
+            }
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < npts; k += blockDim.x) {
+        const int z = k % bz, y = (k / bz) % by, x = k / (bz * by);
+        Q[((size_t)(X0 + x) * fy + (Y0 + y)) * fz + (Z0 + z)] = acc[k];
     }
 }
 
@@ -433,7 +595,11 @@ __global__ void __launch_bounds__(256) k_pme_gather(int N, const float* __restri
                                                     int fx, int fy, int fz,
                                                     const float* __restrict__ phi,
                                                     float update_interval,
-                                                    float* __restrict__ frc) {
+                                                    float* __restrict__ frc,
+                                                    int* __restrict__ bcount, int nb) {
+    // the spread of this evaluation is complete: clear its bins for the next
+    if (blockIdx.x == 0)
+        for (int k = threadIdx.x; k <= nb; k += blockDim.x) bcount[k] = 0;
     const int lane = threadIdx.x & (kWave - 1);
     const int kx = lane >> 4, ky = (lane >> 2) & 3, kz = lane & 3;
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
@@ -657,6 +823,27 @@ void pme_free(sponge_pme* p) {
     dfree(p->d_direct_atom_energy);
     dfree(p->d_correction_atom_energy);
     dfree(p->d_sum);
+    dfree(p->d_bcount);
+    dfree(p->d_bw);
+    dfree(p->d_bu);
+    dfree(p->d_ow);
+    dfree(p->d_ou);
+}
+
+// Brick edge on one axis: the divisor of the grid size closest to 8 among
+// 3..16 (the stencil then spans at most two bricks per axis), else the whole
+// axis.
+int brick_edge(int f) {
+    int best = f, score = 1 << 30;
+    for (int d = 3; d <= 16 && d <= f; ++d) {
+        if (f % d) continue;
+        const int sc = d > 8 ? d - 8 : 8 - d;
+        if (sc < score) {
+            score = sc;
+            best = d;
+        }
+    }
+    return best;
 }
 
 }  // namespace
@@ -805,6 +992,24 @@ extern "C" int sponge_pme_initial(sponge_pme** out, int atom_numbers, const floa
     e |= dmalloc(&p->d_direct_atom_energy, p->N);
     e |= dmalloc(&p->d_correction_atom_energy, p->N);
     e |= dmalloc(&p->d_sum, 2);
+    p->bx = brick_edge(p->fx);
+    p->by = brick_edge(p->fy);
+    p->bz = brick_edge(p->fz);
+    p->nbx = p->fx / p->bx;
+    p->nby = p->fy / p->by;
+    p->nbz = p->fz / p->bz;
+    p->nb = p->nbx * p->nby * p->nbz;
+    {
+        // four times the mean occupancy plus a margin; denser bins go to the
+        // overflow list
+        const long long mean = ((long long)p->N + p->nb - 1) / p->nb;
+        p->bcap = (int)(((4 * mean + 64) + 63) / 64 * 64);
+    }
+    e |= dmalloc(&p->d_bcount, (size_t)p->nb + 1);
+    e |= dmalloc(&p->d_bw, (size_t)p->nb * p->bcap);
+    e |= dmalloc(&p->d_bu, (size_t)p->nb * p->bcap);
+    e |= dmalloc(&p->d_ow, (size_t)p->N);
+    e |= dmalloc(&p->d_ou, (size_t)p->N);
     if (e) {
         pme_free(p);
         delete p;
@@ -828,6 +1033,7 @@ extern "C" int sponge_pme_initial(sponge_pme** out, int atom_numbers, const floa
     int fe = 0;
     if (he == hipSuccess) {
         fe |= fill_async(p->d_acc, 0.0, 9, s);
+        fe |= fill_async(p->d_bcount, 0, p->nb + 1, s);
         fe |= fill_async(p->d_ene, 0.f, 2, s);
         fe |= fill_async(p->d_Q, 0.f, p->nall, s);
         fe |= fill_async(p->d_FBCFQ, 0.f, p->nall, s);
@@ -859,11 +1065,10 @@ extern "C" float* sponge_pme_direct_atom_energy(sponge_pme* p) {
     return p ? p->d_direct_atom_energy : nullptr;
 }
 
+// (the energy / virial accumulators acc[0..6] are cleared by k_pme_bin)
 static int pme_fft_convolve(sponge_pme* p, int need_virial) {
     hipStream_t s = p->stream;
     const int fx = p->fx, fy = p->fy, fz = p->fz, fzc = p->fzc;
-    int rc = fill_async(p->d_acc, 0.0, 7, s);
-    if (rc) return rc;
     {
         const int TL = p->tl_z;
         const size_t sh = (size_t)2 * ((TL + 1) / 2) * fz * sizeof(float2);
@@ -922,14 +1127,18 @@ extern "C" int sponge_pme_reciprocal_force(sponge_pme* p, const float* crd, cons
     const int N = p->N;
     int rc;
     if (step % p->opt.update_interval == 0) {
-        rc = fill_async(p->d_Q, 0.f, p->nall, s);
-        if (rc) return rc;
         int blocks = grid_for(N, 4);
         if (blocks > 16384) blocks = 16384;
         {
             ScopedTimer timer("pme_spread", s);
-            hipLaunchKernelGGL(k_pme_spread, dim3(blocks), dim3(256), 0, s, N, crd, charge, r,
-                               p->fx, p->fy, p->fz, p->d_Q);
+            hipLaunchKernelGGL(k_pme_bin, dim3(grid_for(N, 256)), dim3(256), 0, s, N, crd, charge,
+                               r, p->fx, p->fy, p->fz, p->bx, p->by, p->bz, p->nby, p->nbz, p->nb,
+                               p->bcap, p->d_bcount, p->d_bw, p->d_bu, p->d_ow, p->d_ou, p->d_acc);
+            SA_LAUNCH_CHECK();
+            const size_t sh = sizeof(float) * p->bx * p->by * p->bz;
+            hipLaunchKernelGGL(k_pme_spread_bricks, dim3(p->nb), dim3(256), sh, s, p->fx, p->fy,
+                               p->fz, p->bx, p->by, p->bz, p->nbx, p->nby, p->nbz, p->nb, p->bcap,
+                               p->d_bcount, p->d_bw, p->d_bu, p->d_ow, p->d_ou, p->d_Q);
             SA_LAUNCH_CHECK();
         }
         rc = pme_fft_convolve(p, need_virial);
@@ -942,7 +1151,7 @@ extern "C" int sponge_pme_reciprocal_force(sponge_pme* p, const float* crd, cons
             ScopedTimer timer("pme_gather", s);
             hipLaunchKernelGGL(k_pme_gather, dim3(blocks), dim3(256), 0, s, N, crd, charge, r,
                                p->fx, p->fy, p->fz, p->d_FBCFQ, (float)p->opt.update_interval,
-                               frc);
+                               frc, p->d_bcount, p->nb);
             SA_LAUNCH_CHECK();
         }
     }

@@ -11,24 +11,26 @@ extern bool g_timing;
 void timer_begin(const char* name, hipStream_t s);
 void timer_end(const char* name, hipStream_t s);
 
-// Timers inside HIP graphs: the events a timer records while its stream is
-// capturing become event-record nodes of the graph.  The graph's owner takes
-// the (name, begin event, end event) triples after capture, finds the nodes,
-// and before every launch points them at fresh events, so each replay of the
-// graph is timed like an eager launch.
-struct GraphTimer {
-    std::string name;
-    hipGraphNode_t begin = nullptr, end = nullptr;
+// Timers inside a captured sequence.  While a SegmentedCapture is open on a
+// stream, each timer boundary ends the capture there and starts a new one, so
+// the captured work becomes a list of graphs with the timer marks between
+// them.  Replaying the list launches the graphs in order on the stream and
+// records ordinary timer events at the marks: every replay is timed like an
+// eager launch, and with timers off the list is a single graph.
+struct CaptureItem {
+    hipGraph_t graph = nullptr;        // a segment, or
+    hipGraphExec_t exec = nullptr;
+    std::string timer;                 // a timer mark
+    bool begin = false;
 };
-struct GraphTimers {
-    std::vector<GraphTimer> nodes;
-    bool timing = false;  // whether timers were on when the graph was captured
+struct SegmentedCapture {
+    std::vector<CaptureItem> items;
+    bool timing = false;               // timers were on during the capture
 };
-// after hipStreamEndCapture: find the event-record nodes of the timers
-// recorded during the capture; clears the capture record
-int graph_timers_collect(hipGraph_t g, GraphTimers& out);
-// before hipGraphLaunch: fresh events into the nodes, registered as pending
-int graph_timers_arm(hipGraphExec_t exec, const GraphTimers& t);
+int capture_begin(SegmentedCapture& c, hipStream_t s);
+int capture_end(SegmentedCapture& c, hipStream_t s);  // also instantiates
+int capture_launch(const SegmentedCapture& c, hipStream_t s);
+void capture_destroy(SegmentedCapture& c);
 
 // Records events around a launch when timing is on.
 struct ScopedTimer {

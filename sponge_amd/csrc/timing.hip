@@ -23,12 +23,9 @@ std::map<std::string, Timer> g_timers;
 std::vector<hipEvent_t> g_pool;
 std::vector<std::string> g_filter;  // names timed; empty = all
 
-struct Captured {
-    std::string name;
-    hipEvent_t begin, end;
-};
-std::vector<Captured> g_captured;  // timers recorded into a capturing stream
-std::map<std::string, hipEvent_t> g_capture_open;
+// the segmented capture open on this thread, if any
+thread_local sa::SegmentedCapture* t_capture = nullptr;
+thread_local hipStream_t t_capture_stream = nullptr;
 
 bool selected(const char* name) {
     if (g_filter.empty()) return true;
@@ -37,11 +34,6 @@ bool selected(const char* name) {
     return false;
 }
 
-bool capturing(hipStream_t s) {
-    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-    if (hipStreamIsCapturing(s, &cs) != hipSuccess) return false;
-    return cs == hipStreamCaptureStatusActive;
-}
 
 hipEvent_t take_event() {
     if (!g_pool.empty()) {
@@ -70,115 +62,123 @@ int drain(Timer& t) {
     return 0;
 }
 
+// A timer boundary inside the segmented capture: close the current segment
+// and open the next; the mark goes between them.
+void capture_cut(const char* name, bool begin) {
+    hipGraph_t g = nullptr;
+    if (hipStreamEndCapture(t_capture_stream, &g) != hipSuccess) return;
+    t_capture->items.push_back(sa::CaptureItem{g, nullptr, std::string(), false});
+    t_capture->items.push_back(sa::CaptureItem{nullptr, nullptr, std::string(name), begin});
+    (void)hipStreamBeginCapture(t_capture_stream, hipStreamCaptureModeThreadLocal);
+}
+
+void record_begin(const char* name, hipStream_t s) {
+    hipEvent_t e = take_event();
+    if (!e) return;
+    if (hipEventRecord(e, s) != hipSuccess) {
+        g_pool.push_back(e);
+        return;
+    }
+    g_timers[name].open = e;
+}
+
+void record_end(const char* name, hipStream_t s) {
+    Timer& t = g_timers[name];
+    if (!t.open) return;
+    hipEvent_t e = take_event();
+    if (!e) return;
+    if (hipEventRecord(e, s) != hipSuccess) {
+        g_pool.push_back(e);
+        return;
+    }
+    t.pending.emplace_back(t.open, e);
+    t.open = nullptr;
+    // bound the number of outstanding events
+    if (t.pending.size() > 4096) drain(t);
+}
+
 }  // namespace
 
 namespace sa {
 
 bool g_timing = false;
 
-// inside a capture the record must be an external event-record node (a plain
-// record there only marks a dependency point of the capture)
-hipError_t record(hipEvent_t e, hipStream_t s, bool cap) {
-    return cap ? hipEventRecordWithFlags(e, s, hipEventRecordExternal) : hipEventRecord(e, s);
-}
-
 void timer_begin(const char* name, hipStream_t s) {
     std::lock_guard<std::mutex> lk(g_mu);
     if (!selected(name)) return;
-    const bool cap = capturing(s);
-    hipEvent_t e = take_event();
-    if (!e) return;
-    if (record(e, s, cap) != hipSuccess) {
-        g_pool.push_back(e);
+    if (t_capture && s == t_capture_stream) {
+        capture_cut(name, true);
         return;
     }
-    if (cap) g_capture_open[name] = e;
-    else g_timers[name].open = e;
+    record_begin(name, s);
 }
 
 void timer_end(const char* name, hipStream_t s) {
     std::lock_guard<std::mutex> lk(g_mu);
     if (!selected(name)) return;
-    const bool cap = capturing(s);
-    hipEvent_t b = nullptr;
-    if (cap) {
-        auto it = g_capture_open.find(name);
-        if (it == g_capture_open.end()) return;
-        b = it->second;
-        g_capture_open.erase(it);
-    } else {
-        b = g_timers[name].open;
-        if (!b) return;
-    }
-    hipEvent_t e = take_event();
-    if (!e) return;
-    if (record(e, s, cap) != hipSuccess) {
-        g_pool.push_back(e);
+    if (t_capture && s == t_capture_stream) {
+        capture_cut(name, false);
         return;
     }
-    if (cap) {
-        g_captured.push_back({name, b, e});
-        return;
-    }
-    Timer& t = g_timers[name];
-    t.pending.emplace_back(b, e);
-    t.open = nullptr;
-    // bound the number of outstanding events
-    if (t.pending.size() > 4096) drain(t);
+    record_end(name, s);
 }
 
-int graph_timers_collect(hipGraph_t g, GraphTimers& out) {
-    std::lock_guard<std::mutex> lk(g_mu);
-    out.nodes.clear();
-    out.timing = g_timing;
-    std::vector<Captured> cap;
-    cap.swap(g_captured);
-    g_capture_open.clear();
-    if (cap.empty()) return 0;
-    size_t n = 0;
-    hipError_t e = hipGraphGetNodes(g, nullptr, &n);
-    if (e != hipSuccess) return (int)e;
-    std::vector<hipGraphNode_t> nodes(n);
-    e = hipGraphGetNodes(g, nodes.data(), &n);
-    if (e != hipSuccess) return (int)e;
-    for (const auto& c : cap) {
-        GraphTimer gt;
-        gt.name = c.name;
-        for (auto nd : nodes) {
-            hipGraphNodeType ty;
-            if (hipGraphNodeGetType(nd, &ty) != hipSuccess || ty != hipGraphNodeTypeEventRecord) continue;
-            hipEvent_t ev = nullptr;
-            if (hipGraphEventRecordNodeGetEvent(nd, &ev) != hipSuccess) continue;
-            if (ev == c.begin) gt.begin = nd;
-            if (ev == c.end) gt.end = nd;
+int capture_begin(SegmentedCapture& c, hipStream_t s) {
+    capture_destroy(c);
+    c.timing = g_timing;
+    SA_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    t_capture = &c;
+    t_capture_stream = s;
+    return 0;
+}
+
+int capture_end(SegmentedCapture& c, hipStream_t s) {
+    t_capture = nullptr;
+    t_capture_stream = nullptr;
+    hipGraph_t g = nullptr;
+    SA_CHECK(hipStreamEndCapture(s, &g));
+    c.items.push_back(CaptureItem{g, nullptr, std::string(), false});
+    // drop empty segments, instantiate the others
+    std::vector<CaptureItem> kept;
+    for (auto& it : c.items) {
+        if (!it.graph) {
+            kept.push_back(it);
+            continue;
         }
-        // the capture events stay referenced by the graph: not returned to the pool
-        if (gt.begin && gt.end) out.nodes.push_back(gt);
+        size_t n = 0;
+        SA_CHECK(hipGraphGetNodes(it.graph, nullptr, &n));
+        if (n == 0) {
+            (void)hipGraphDestroy(it.graph);
+            continue;
+        }
+        SA_CHECK(hipGraphInstantiate(&it.exec, it.graph, nullptr, nullptr, 0));
+        kept.push_back(it);
+    }
+    c.items.swap(kept);
+    return 0;
+}
+
+int capture_launch(const SegmentedCapture& c, hipStream_t s) {
+    for (const auto& it : c.items) {
+        if (it.exec) {
+            SA_CHECK(hipGraphLaunch(it.exec, s));
+        } else {
+            std::lock_guard<std::mutex> lk(g_mu);
+            if (it.begin) record_begin(it.timer.c_str(), s);
+            else record_end(it.timer.c_str(), s);
+        }
     }
     return 0;
 }
 
-int graph_timers_arm(hipGraphExec_t exec, const GraphTimers& t) {
-    std::lock_guard<std::mutex> lk(g_mu);
-    for (const auto& gt : t.nodes) {
-        hipEvent_t b = take_event(), e = take_event();
-        if (!b || !e) return (int)hipErrorOutOfMemory;
-        hipError_t he = hipGraphExecEventRecordNodeSetEvent(exec, gt.begin, b);
-        if (he == hipSuccess) he = hipGraphExecEventRecordNodeSetEvent(exec, gt.end, e);
-        if (he != hipSuccess) {
-            g_pool.push_back(b);
-            g_pool.push_back(e);
-            return (int)he;
-        }
-        Timer& tm = g_timers[gt.name];
-        tm.pending.emplace_back(b, e);
-        if (tm.pending.size() > 4096) {
-            int rc = drain(tm);
-            if (rc) return rc;
-        }
+void capture_destroy(SegmentedCapture& c) {
+    for (auto& it : c.items) {
+        if (it.exec) (void)hipGraphExecDestroy(it.exec);
+        if (it.graph) (void)hipGraphDestroy(it.graph);
     }
-    return 0;
+    c.items.clear();
 }
+
 
 }  // namespace sa
 
