@@ -132,6 +132,21 @@ __device__ __forceinline__ int stream_to_sorted(int t, int nr, const int* r_firs
     return r_first[lo] + (t - r_off[lo]);
 }
 
+// Stage the first `staged` positions of a cell's candidate stream into LDS:
+// one wavefront per run (a run is contiguous in sorted order, so the loads
+// coalesce); copy(t, q) moves sorted entry q to stream position t.
+template <typename Copy>
+__device__ __forceinline__ void stage_stream(int nr, const int* r_first, const int* r_off, int staged,
+                                             Copy copy) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+    for (int k = wave; k < nr; k += nwave) {
+        const int off = r_off[k];
+        if (off >= staged) continue;
+        const int len = min(r_off[k + 1], staged) - off, f = r_first[k];
+        for (int u = lane; u < len; u += 64) copy(off + u, f + u);
+    }
+}
+
 // Candidate-stream entries a kernel stages in LDS, given the bytes per entry:
 // as many as the device's per-workgroup LDS holds beside 4 KiB of tables and
 // `reserve` further bytes, at most `want`, a multiple of 64.

@@ -317,14 +317,12 @@ __global__ void __launch_bounds__(512) k_lj_cells(
     __syncthreads();
     const int total = r_off[nr];
     const bool staged = total <= stage;
-    if (staged) {
-        for (int t = threadIdx.x; t < total; t += blockDim.x) {
-            const int q = stream_to_sorted(t, nr, r_first, r_off);
+    if (staged)
+        stage_stream(nr, r_first, r_off, total, [&](int t, int q) {
             cpos[t] = pos[q];
             ctyp[t] = ptype[q];
             if (NEED_E || NEED_V) cgid[t] = pgid[q];
-        }
-    }
+        });
     __syncthreads();
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave, nwave = blockDim.x / kWave;

@@ -44,7 +44,8 @@ struct sponge_md {
     int* d_excl_num = nullptr;
     int* d_excl_list = nullptr;
     int4* d_water = nullptr;       // O, H1, H2 per water
-    float4* d_old = nullptr;       // [2*n_water] previous B-A and C-A
+    int n_free = 0;                // atoms outside any water
+    int* d_free = nullptr;
     float geom[5] = {0, 0, 0, 0, 0};  // canonical ya, xb, yb, xc, yc
     float mO = 0, mH1 = 0, mH2 = 0;
     double* d_ek = nullptr;
@@ -73,49 +74,33 @@ __device__ __forceinline__ float3 f3cross(float3 a, float3 b) {
 }
 __device__ __forceinline__ float3 f3unit(float3 a) { return f3scale(rsqrtf(f3dot(a, a)), a); }
 
-// previous geometry of each water: B - A and C - A (minimum image)
-__global__ void k_settle_remember(int n, const int4* __restrict__ w, const float* __restrict__ crd,
-                                  LT6 c, LT6 r, float4* __restrict__ old) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const int4 t = w[k];
-    const float3 a = ld3(crd, t.x);
-    const float3 b = disp_pbc<true>(ld3(crd, t.y), a, c, r);
-    const float3 cc = disp_pbc<true>(ld3(crd, t.z), a, c, r);
-    old[2 * k] = make_float4(b.x, b.y, b.z, 0.f);
-    old[2 * k + 1] = make_float4(cc.x, cc.y, cc.z, 0.f);
-}
+// Canonical geometry of the rigid triangle (COM at the origin, molecule in the
+// xy plane): A = (0, ya), B = (xb, yb), C = (xc, yc), masses mA, mB, mC.
+struct Triangle {
+    float ya, xb, yb, xc, yc, mA, mB, mC;
+};
 
-__global__ void k_leap_frog(int N, float* __restrict__ vel, float* __restrict__ crd,
-                            const float* __restrict__ frc, const float* __restrict__ inv_mass,
-                            float dt) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    const float im = inv_mass[i];
-    float3 v = ld3(vel, i);
-    const float3 f = ld3(frc, i);
-    v = f3add(v, f3scale(dt, f3scale(im, f)));
-    st3(vel, i, v);
-    st3(crd, i, f3add(ld3(crd, i), f3scale(dt, v)));
-}
-
-// SETTLE for one rigid triangle per thread.  Canonical geometry (COM at the
-// origin, molecule in the xy plane): A = (0, ya), B = (xb, yb), C = (xc, yc).
-__global__ void k_settle(int n, const int4* __restrict__ w, float* __restrict__ crd,
-                         float* __restrict__ vel, const float4* __restrict__ old, LT6 c, LT6 r,
-                         float ya, float xb, float yb, float xc, float yc, float mA, float mB,
-                         float mC, float inv_dt) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const int4 t = w[k];
-    const float3 A1 = ld3(crd, t.x);
-    const float3 B1 = f3add(A1, disp_pbc<true>(ld3(crd, t.y), A1, c, r));
-    const float3 C1 = f3add(A1, disp_pbc<true>(ld3(crd, t.z), A1, c, r));
+// SETTLE of one water: A1 is the unconstrained position of A, rB and rC the
+// unconstrained B - A and C - A (minimum image), B0, C0 the previous B - A and
+// C - A.  Writes the constrained positions and adds (x_constrained - x1) / dt
+// to the velocities va, vb, vc.
+//
+// The COM-relative coordinates are formed from the relative vectors only,
+// a1 = -(mB rB + mC rC) / M, as the reference does (settle.cpp:234-240):
+// taken from absolute positions (~100 A in FP32, ulp 7.6e-6 A) they carry a
+// shape error that the velocity correction divides by dt every step, and a
+// rigid water then heats up steadily (+12% kinetic energy in 600 steps of
+// 2 fs at 99 A, force-free).
+__device__ __forceinline__ void settle_one(const int4 t, float3 A1, float3 rB, float3 rC, float3 B0,
+                                           float3 C0, float3 va, float3 vb, float3 vc,
+                                           const Triangle& g, float inv_dt, float* __restrict__ crd,
+                                           float* __restrict__ vel) {
+    const float ya = g.ya, xb = g.xb, yb = g.yb, xc = g.xc, yc = g.yc;
+    const float mA = g.mA, mB = g.mB, mC = g.mC;
     const float M = mA + mB + mC;
-    const float3 com = f3scale(1.0f / M, f3add(f3scale(mA, A1), f3add(f3scale(mB, B1), f3scale(mC, C1))));
-    const float3 a1 = f3sub(A1, com), b1 = f3sub(B1, com), c1 = f3sub(C1, com);
-    const float4 o0 = old[2 * k], o1 = old[2 * k + 1];
-    const float3 B0 = make_float3(o0.x, o0.y, o0.z), C0 = make_float3(o1.x, o1.y, o1.z);
+    const float3 a1 = f3scale(-1.0f / M, f3add(f3scale(mB, rB), f3scale(mC, rC)));
+    const float3 b1 = f3add(rB, a1), c1 = f3add(rC, a1);
+    const float3 com = f3sub(A1, a1);
     // frame: ez normal to the old plane, ex normal to ez and the new A
     const float3 ez = f3unit(f3cross(B0, C0));
     const float3 ex = f3unit(f3cross(a1, ez));
@@ -149,9 +134,45 @@ __global__ void k_settle(int n, const int4* __restrict__ w, float* __restrict__ 
     st3(crd, t.x, f3add(com, a3));
     st3(crd, t.y, f3add(com, b3));
     st3(crd, t.z, f3add(com, c3));
-    st3(vel, t.x, f3add(ld3(vel, t.x), f3scale(inv_dt, f3sub(a3, a1))));
-    st3(vel, t.y, f3add(ld3(vel, t.y), f3scale(inv_dt, f3sub(b3, b1))));
-    st3(vel, t.z, f3add(ld3(vel, t.z), f3scale(inv_dt, f3sub(c3, c1))));
+    st3(vel, t.x, f3add(va, f3scale(inv_dt, f3sub(a3, a1))));
+    st3(vel, t.y, f3add(vb, f3scale(inv_dt, f3sub(b3, b1))));
+    st3(vel, t.z, f3add(vc, f3scale(inv_dt, f3sub(c3, c1))));
+}
+
+// One MD integration step in one launch: a thread per water remembers its
+// geometry, advances its three atoms by leap-frog and applies SETTLE; a
+// thread per atom outside any water advances it by leap-frog.  The same
+// arithmetic as k_settle_remember + k_leap_frog + k_settle.
+__device__ __forceinline__ void leap(float3& x, float3& v, float3 f, float im, float dt) {
+    v = f3add(v, f3scale(dt, f3scale(im, f)));
+    x = f3add(x, f3scale(dt, v));
+}
+
+__global__ void __launch_bounds__(256) k_integrate(int n_water, const int4* __restrict__ w,
+                                                   int n_free, const int* __restrict__ free_atoms,
+                                                   float* __restrict__ crd, float* __restrict__ vel,
+                                                   const float* __restrict__ frc,
+                                                   const float* __restrict__ inv_mass, float dt,
+                                                   LT6 c, LT6 r, Triangle g, float inv_dt) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n_water) {
+        const int4 t = w[k];
+        float3 xa = ld3(crd, t.x), xb = ld3(crd, t.y), xc = ld3(crd, t.z);
+        const float3 B0 = disp_pbc<true>(xb, xa, c, r);
+        const float3 C0 = disp_pbc<true>(xc, xa, c, r);
+        float3 va = ld3(vel, t.x), vb = ld3(vel, t.y), vc = ld3(vel, t.z);
+        leap(xa, va, ld3(frc, t.x), inv_mass[t.x], dt);
+        leap(xb, vb, ld3(frc, t.y), inv_mass[t.y], dt);
+        leap(xc, vc, ld3(frc, t.z), inv_mass[t.z], dt);
+        settle_one(t, xa, disp_pbc<true>(xb, xa, c, r), disp_pbc<true>(xc, xa, c, r), B0, C0, va,
+                   vb, vc, g, inv_dt, crd, vel);
+    } else if (k - n_water < n_free) {
+        const int i = free_atoms[k - n_water];
+        float3 x = ld3(crd, i), v = ld3(vel, i);
+        leap(x, v, ld3(frc, i), inv_mass[i], dt);
+        st3(vel, i, v);
+        st3(crd, i, x);
+    }
 }
 
 __global__ void k_kinetic(int N, const float* __restrict__ vel, const float* __restrict__ mass,
@@ -227,7 +248,7 @@ void md_free(sponge_md* m) {
     dfree(m->d_excl_num);
     dfree(m->d_excl_list);
     dfree(m->d_water);
-    dfree(m->d_old);
+    dfree(m->d_free);
     dfree(m->d_ek);
     if (m->own_stream && m->stream) (void)hipStreamDestroy(m->stream);
     m->stream = nullptr;
@@ -264,20 +285,14 @@ int md_step(sponge_md* m, long long step, int need_energy) {
                                      need_energy, nullptr, need_energy ? m->d_potential : nullptr,
                                      (int)step);
     if (rc) return rc;
-    // integration (Main_Iteration, SPONGE/main.cpp:1702-1774)
-    if (m->n_water) {
-        hipLaunchKernelGGL(k_settle_remember, dim3(grid_for(m->n_water, 256)), dim3(256), 0, s,
-                           m->n_water, m->d_water, m->d_crd, m->cell, m->rcell, m->d_old);
-        SA_LAUNCH_CHECK();
-    }
-    hipLaunchKernelGGL(k_leap_frog, dim3(grid_for(N, 256)), dim3(256), 0, s, N, m->d_vel, m->d_crd,
-                       m->d_frc, m->d_inv_mass, m->dt);
-    SA_LAUNCH_CHECK();
-    if (m->n_water) {
-        hipLaunchKernelGGL(k_settle, dim3(grid_for(m->n_water, 256)), dim3(256), 0, s, m->n_water,
-                           m->d_water, m->d_crd, m->d_vel, m->d_old, m->cell, m->rcell, m->geom[0],
-                           m->geom[1], m->geom[2], m->geom[3], m->geom[4], m->mO, m->mH1, m->mH2,
-                           1.0f / m->dt);
+    // integration (Main_Iteration, SPONGE/main.cpp:1702-1774): leap-frog and
+    // SETTLE in one launch
+    const Triangle g{m->geom[0], m->geom[1], m->geom[2], m->geom[3], m->geom[4], m->mO, m->mH1, m->mH2};
+    const int threads = m->n_water + m->n_free;
+    if (threads > 0) {
+        hipLaunchKernelGGL(k_integrate, dim3(grid_for(threads, 256)), dim3(256), 0, s, m->n_water,
+                           m->d_water, m->n_free, m->d_free, m->d_crd, m->d_vel, m->d_frc,
+                           m->d_inv_mass, m->dt, m->cell, m->rcell, g, 1.0f / m->dt);
         SA_LAUNCH_CHECK();
     }
     return 0;
@@ -349,6 +364,23 @@ extern "C" int sponge_md_initial(sponge_md** out, int atom_numbers, const float*
         m->mH2 = mass[c];
         triangle_geometry(dist(a, b), dist(a, c), dist(b, c), mass[a], mass[b], mass[c], m->geom);
     }
+    // atoms outside any water are advanced by leap-frog alone; a water atom
+    // listed twice would be advanced twice
+    std::vector<int> in_water(N, 0);
+    for (int k = 0; k < n_water; ++k) {
+        in_water[water[k].x]++;
+        in_water[water[k].y]++;
+        in_water[water[k].z]++;
+    }
+    std::vector<int> free_atoms;
+    for (int i = 0; i < N; ++i) {
+        if (in_water[i] > 1) {
+            delete m;
+            return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+        }
+        if (!in_water[i]) free_atoms.push_back(i);
+    }
+    m->n_free = (int)free_atoms.size();
     int e = 0;
     e |= dmalloc(&m->d_crd, (size_t)3 * N);
     e |= dmalloc(&m->d_vel, (size_t)3 * N);
@@ -364,7 +396,7 @@ extern "C" int sponge_md_initial(sponge_md** out, int atom_numbers, const float*
     e |= dmalloc(&m->d_excl_num, N);
     e |= dmalloc(&m->d_excl_list, E > 0 ? E : 1);
     e |= dmalloc(&m->d_water, n_water > 0 ? n_water : 1);
-    e |= dmalloc(&m->d_old, (size_t)2 * (n_water > 0 ? n_water : 1));
+    e |= dmalloc(&m->d_free, free_atoms.size());
     e |= dmalloc(&m->d_ek, 2);
     if (e) {
         md_free(m);
@@ -385,6 +417,7 @@ extern "C" int sponge_md_initial(sponge_md** out, int atom_numbers, const float*
     cp(m->d_excl_num, excluded_numbers, sizeof(int) * N);
     if (E > 0) cp(m->d_excl_list, excluded_list, sizeof(int) * E);
     if (n_water > 0) cp(m->d_water, water.data(), sizeof(int4) * n_water);
+    if (m->n_free > 0) cp(m->d_free, free_atoms.data(), sizeof(int) * m->n_free);
     if (he == hipSuccess) he = hipStreamSynchronize(s);
     int rc = he != hipSuccess ? (int)he : 0;
     if (!rc) rc = sponge_nl_initial(&m->nl, N, cutoff, skin, &m->cell.a11, &m->rcell.a11, nl_opt, stream);

@@ -251,8 +251,7 @@ __global__ void __launch_bounds__(512) k_nl_search_cells(
         return;
     }
     const int staged = total < stage ? total : stage;
-    for (int t = threadIdx.x; t < staged; t += blockDim.x)
-        cand[t] = spos[stream_to_sorted(t, nr, r_first, r_off)];
+    stage_stream(nr, r_first, r_off, staged, [&](int t, int q) { cand[t] = spos[q]; });
     __syncthreads();
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave, nwave = blockDim.x / kWave;

@@ -392,14 +392,23 @@ __device__ __forceinline__ Near atom_near(const float* __restrict__ crd, int i, 
     return o;
 }
 
-__device__ __forceinline__ float bsp_w(int k, float f) {
-    const float f2 = f * f;
-    return cMa[k] * f * f2 + cMb[k] * f2 + cMc[k] * f + cMd[k];
-}
-__device__ __forceinline__ float bsp_dw(int k, float f) {
-    const float f2 = f * f;
-    return cdMa[k] * f2 + cdMb[k] * f + cdMc[k];
-}
+// The weight polynomials of one stencil index k, held in registers by a
+// lane for its whole kernel (the lane's k never changes), so the hot loops
+// do not reload the coefficient tables.
+struct BspCoef {
+    float a, b, c, d;     // M: a f^3 + b f^2 + c f + d
+    float da, db, dc;     // dM: da f^2 + db f + dc
+    __device__ __forceinline__ explicit BspCoef(int k)
+        : a(cMa[k]), b(cMb[k]), c(cMc[k]), d(cMd[k]), da(cdMa[k]), db(cdMb[k]), dc(cdMc[k]) {}
+    __device__ __forceinline__ float w(float f) const {
+        const float f2 = f * f;
+        return a * f * f2 + b * f2 + c * f + d;
+    }
+    __device__ __forceinline__ float dw(float f) const {
+        const float f2 = f * f;
+        return da * f2 + db * f + dc;
+    }
+};
 
 // grid index of stencil point (kx, ky, kz) below u (PM_force.cpp:737-759)
 __device__ __forceinline__ int stencil_index(const Near& nr, int kx, int ky, int kz, int fx,
@@ -528,6 +537,7 @@ __global__ void __launch_bounds__(256) k_pme_spread_bricks(
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave, nwave = blockDim.x / kWave;
     const int kx = lane >> 4, ky = (lane >> 2) & 3, kz = lane & 3;
+    const BspCoef Cx(kx), Cy(ky), Cz(kz);
     for (int base = wave * kWave; base < total; base += nwave * kWave) {
         const int g = base + lane;
         float4 wj = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -574,9 +584,9 @@ __global__ void __launch_bounds__(256) k_pme_spread_bricks(
             y -= Y0;
             z -= Z0;
             if (x >= 0 && x < bx && y >= 0 && y < by && z >= 0 && z < bz) {
-                float tq = q * bsp_w(kx, wfx);
-                tq = tq * bsp_w(ky, wfy);
-                tq = tq * bsp_w(kz, wfz);
+                float tq = q * Cx.w(wfx);
+                tq = tq * Cy.w(wfy);
+                tq = tq * Cz.w(wfz);
                 atomicAdd(&acc[(x * by + y) * bz + z], tq);
             }
         }
@@ -602,13 +612,14 @@ __global__ void __launch_bounds__(256) k_pme_gather(int N, const float* __restri
         for (int k = threadIdx.x; k <= nb; k += blockDim.x) bcount[k] = 0;
     const int lane = threadIdx.x & (kWave - 1);
     const int kx = lane >> 4, ky = (lane >> 2) & 3, kz = lane & 3;
+    const BspCoef Cx(kx), Cy(ky), Cz(kz);
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const int nwaves = gridDim.x * blockDim.x / kWave;
     for (int i = wave; i < N; i += nwaves) {
         const Near nr = atom_near(crd, i, rc, fx, fy, fz);
         const float dqf = -phi[stencil_index(nr, kx, ky, kz, fx, fy, fz)] * charge[i];
-        const float wx = bsp_w(kx, nr.fx), wy = bsp_w(ky, nr.fy), wz = bsp_w(kz, nr.fz);
-        const float dx = bsp_dw(kx, nr.fx), dy = bsp_dw(ky, nr.fy), dz = bsp_dw(kz, nr.fz);
+        const float wx = Cx.w(nr.fx), wy = Cy.w(nr.fy), wz = Cz.w(nr.fz);
+        const float dx = Cx.dw(nr.fx), dy = Cy.dw(nr.fy), dz = Cz.dw(nr.fz);
         const float gx = dx * wy * wz * fx, gy = dy * wx * wz * fy, gz = dz * wx * wy * fz;
         // MultiplyTranspose(g, rcell) = {gx a11, gx a21 + gy a22, gx a31 + gy a32 + gz a33}
         float ox = dqf * (gx * rc.a11);

@@ -60,6 +60,8 @@ def export_pairs(nl, N, dev, max_numbers=1200):
     from oracle import oracle as O
     counts = torch.zeros(N, dtype=torch.int32, device=dev)
     serial = torch.zeros(N * max_numbers, dtype=torch.int32, device=dev)
+    # the list's stream is not torch's: the zero fill must be done first
+    torch.cuda.synchronize()
     nl.export_half_list(counts, serial, max_numbers)
     torch.cuda.synchronize()
     c = counts.cpu().numpy()

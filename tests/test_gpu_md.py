@@ -56,6 +56,8 @@ def test_settle_and_energy_conservation(dev):
     totals = np.array(totals)
     kin = e_first["kinetic"]
     # leap-frog with half-step kinetic energy: fluctuation, no runaway drift
+    # (the lattice start still relaxes here; the equilibrated 95k-atom test
+    # in test_gpu_fullsize.py holds the tight bound)
     assert np.abs(totals - totals[0]).max() < 0.05 * abs(kin), totals
 
 
