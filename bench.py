@@ -32,6 +32,10 @@ METRIC = "ns/day @96k-atom water+ions; learned-vs-plain speedup at ≤1% recip R
 FP32_PEAK_TFLOPS = 157.3          # MI355X FP32 vector, dual-issue packed
 FP32_PEAK_TFLOPS_UNPACKED = 78.6  # without packed FP32 instructions
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per force-only pair-kernel launch on this workload: rocprofv3
+# FETCH_SIZE and WRITE_SIZE runs scaled by a 256 MiB copy (2.0 x FETCH_SIZE,
+# 1.0 x WRITE_SIZE); commands and table in profiles/traffic_water_ions_95k.md
+PAIR_KERNEL_TRAFFIC_BYTES = 159.9e6
 TIMERS = ("lj_pairs", "nl_search", "pme_spread", "pme_gather", "pme_fft_z_r2c", "pme_fft_y",
           "pme_fft_x_conv", "pme_ifft_y", "pme_fft_z_c2r")
 
@@ -169,8 +173,10 @@ def main():
     lj_avg_s = lj_ms * 1e-3 / max(lj_n, 1)
     flops = pair_kernel_flops(n_in, n_list)
     achieved = flops / lj_avg_s / 1e12
+    traffic = PAIR_KERNEL_TRAFFIC_BYTES if args.config == 3 else None
     roofline = dict(bound="valu", achieved=achieved, peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
-                    frac=achieved / FP32_PEAK_TFLOPS, traffic=None,
+                    frac=achieved / FP32_PEAK_TFLOPS, traffic=traffic,
+                    traffic_note="HBM bytes per launch from rocprofv3 PMC runs (copy-calibrated)",
                     kernel="lj_pairs", flop_per_launch=flops, pairs_in_cutoff=n_in,
                     list_entries=n_list, avg_launch_us=lj_avg_s * 1e6,
                     frac_of_unpacked_peak=achieved / FP32_PEAK_TFLOPS_UNPACKED,

@@ -152,6 +152,16 @@ __device__ __forceinline__ void stage_stream(int nr, const int* r_first, const i
 // `reserve` further bytes, at most `want`, a multiple of 64.
 int stage_entries(size_t bytes_per_entry, int want, size_t reserve = 0);
 
+// MD-driver forms of two module calls (see their definitions)
+int lj_direct_force_md(sponge_lj* lj, sponge_nl* nl, const float* crd, const float* charge,
+                       float* frc, const float* cell, const float* rcell, float pme_beta,
+                       int need_atom_energy, float* atom_energy, float* atom_direct_pme_energy,
+                       int* reset_flag);
+int nl_update_impl(sponge_nl* nl, const int* atom_local, int local_atom_numbers, int ghost_numbers,
+                   const float* crd, const float* cell, const float* rcell, int step, int update,
+                   const int* excluded_list_start, const int* excluded_list,
+                   const int* excluded_numbers, bool need_cleared);
+
 int nl_pack_for_pairs(sponge_nl* nl, const float* crd, const float* charge, const int* type,
                       float4* pos, int* ptype, int* pgid, hipStream_t s);
 }

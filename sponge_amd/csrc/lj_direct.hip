@@ -351,14 +351,22 @@ __global__ void k_lj_finalize(int N, const int* __restrict__ sidx, const float4*
                               const float2* __restrict__ eacc, const float* __restrict__ vacc,
                               float* __restrict__ frc, float* __restrict__ atom_energy,
                               float* __restrict__ lj_energy, float* __restrict__ direct_energy,
-                              float* __restrict__ atom_virial) {
+                              float* __restrict__ atom_virial, int assign,
+                              int* __restrict__ reset_flag) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p == 0 && reset_flag) *reset_flag = 0;
     if (p >= N) return;
     const int i = sidx ? sidx[p] : p;
     const float4 f = facc[p];
-    frc[3 * i] += f.x;
-    frc[3 * i + 1] += f.y;
-    frc[3 * i + 2] += f.z;
+    if (assign) {
+        frc[3 * i] = f.x;
+        frc[3 * i + 1] = f.y;
+        frc[3 * i + 2] = f.z;
+    } else {
+        frc[3 * i] += f.x;
+        frc[3 * i + 1] += f.y;
+        frc[3 * i + 2] += f.z;
+    }
     if (NEED_E) {
         const float2 e = eacc[p];
         atom_energy[i] += e.x + e.y;
@@ -442,12 +450,13 @@ int launch_pairs(sponge_lj* lj, int nrows, const int* cnt, const int* list, int 
 }
 
 int finalize(sponge_lj* lj, int N, const int* sidx, int need_e, int need_v, float* frc,
-             float* atom_energy, float* atom_virial, float* direct, hipStream_t s) {
+             float* atom_energy, float* atom_virial, float* direct, hipStream_t s, int assign = 0,
+             int* reset_flag = nullptr) {
     dim3 grid(grid_for(N, 256)), block(256);
 #define SA_FIN(E, V)                                                                            \
     hipLaunchKernelGGL((k_lj_finalize<E, V>), grid, block, 0, s, N, sidx, lj->d_facc,           \
                        lj->d_eacc, lj->d_vacc, frc, atom_energy, lj->d_LJ_energy_atom, direct, \
-                       atom_virial)
+                       atom_virial, assign, reset_flag)
     if (need_e && need_v) SA_FIN(true, true);
     else if (need_e) SA_FIN(true, false);
     else if (need_v) SA_FIN(false, true);
@@ -563,6 +572,27 @@ extern "C" int sponge_lj_long_range_factor(sponge_lj* lj, float* factor) {
     if (!lj || !factor) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
     *factor = lj->long_range_factor;
     return 0;
+}
+
+// The MD driver's form: forces assigned (the driver's first force term of a
+// step, so its force array needs no clearing) and the neighbor list's rebuild
+// flag cleared once the pair kernel has consumed the list.
+int sa::lj_direct_force_md(sponge_lj* lj, sponge_nl* nl, const float* crd, const float* charge,
+                           float* frc, const float* cell, const float* rcell, float pme_beta,
+                           int need_atom_energy, float* atom_energy, float* atom_direct_pme_energy,
+                           int* reset_flag) {
+    if (!nl->built) return SPONGE_AMD_ERROR_MISSING_COMMAND;
+    hipStream_t s = lj->stream;
+    const LT6 c = lt_from(cell), r = lt_from(rcell);
+    int rc = nl_pack_for_pairs(nl, crd, charge, lj->d_type, lj->d_pos, lj->d_ptype, lj->d_pgid, s);
+    if (rc) return rc;
+    if (lt_is_orthorhombic(c))
+        rc = launch_rows<true>(lj, nl, c, r, pme_beta, need_atom_energy, 0, s);
+    else
+        rc = launch_rows<false>(lj, nl, c, r, pme_beta, need_atom_energy, 0, s);
+    if (rc) return rc;
+    return finalize(lj, lj->N, nl->d_sidx, need_atom_energy, 0, frc, atom_energy, nullptr,
+                    atom_direct_pme_energy, s, 1, reset_flag);
 }
 
 extern "C" int sponge_lj_pme_direct_force(sponge_lj* lj, sponge_nl* nl, int atom_numbers,

@@ -51,6 +51,7 @@ struct sponge_md {
     double* d_ek = nullptr;
     long long step = 0;
     SegmentedCapture capture;      // the captured step (one graph, or segments between timers)
+    bool need_cleared = false;     // the pair kernel of the last step cleared the rebuild flag
     int use_graph = 0;
     int own_stream = 0;  // the driver created its stream (graph capture needs one)
 };
@@ -258,24 +259,30 @@ void md_free(sponge_md* m) {
 int md_step(sponge_md* m, long long step, int need_energy) {
     hipStream_t s = m->stream;
     const int N = m->N;
-    int rc = fill_async(m->d_frc, 0.f, (long long)3 * N, s);
-    if (!rc && need_energy) {
+    int rc = 0;
+    if (need_energy) {
         rc |= fill_async(m->d_atom_energy, 0.f, N, s);
         rc |= fill_async(m->d_potential, 0.f, 1, s);
     }
     if (rc) return rc;
     const float* cell = &m->cell.a11;
     const float* rcell = &m->rcell.a11;
-    rc = sponge_nl_update(m->nl, nullptr, N, 0, m->d_crd, cell, rcell, (int)step,
-                          SPONGE_AMD_NL_CONDITIONAL, m->d_excl_start, m->d_excl_list, m->d_excl_num);
+    rc = nl_update_impl(m->nl, nullptr, N, 0, m->d_crd, cell, rcell, (int)step,
+                        SPONGE_AMD_NL_CONDITIONAL, m->d_excl_start, m->d_excl_list, m->d_excl_num,
+                        m->need_cleared);
     if (rc) return rc;
+    // LJ + PME direct first: it assigns the force array (no clearing pass) and
+    // clears the list's rebuild flag once the list has been used this step.
+    // The terms are summed in another order than Main_Calculate_Force's, which
+    // changes only the rounding of the total force.
+    rc = lj_direct_force_md(m->lj, m->nl, m->d_crd, m->d_charge, m->d_frc, cell, rcell,
+                            m->pme->beta, need_energy, m->d_atom_energy,
+                            m->pme->d_direct_atom_energy, m->nl->d_need);
+    if (rc) return rc;
+    m->need_cleared = true;
     rc = sponge_pme_excluded_force(m->pme, N, nullptr, nullptr, m->d_crd, cell, rcell, m->d_charge,
                                    m->d_excl_start, m->d_excl_list, m->d_excl_num, m->d_frc,
                                    need_energy, m->d_atom_energy, 0, nullptr);
-    if (rc) return rc;
-    rc = sponge_lj_pme_direct_force(m->lj, m->nl, N, N, 0, 0, m->d_crd, m->d_charge, m->d_frc, cell,
-                                    rcell, m->pme->beta, need_energy, m->d_atom_energy, 0, nullptr,
-                                    m->pme->d_direct_atom_energy);
     if (rc) return rc;
     if (need_energy) {
         rc = sponge_lj_long_range_correction(m->lj, 0, nullptr, 1, m->d_potential, m->volume);

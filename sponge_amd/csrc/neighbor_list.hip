@@ -556,6 +556,17 @@ extern "C" int sponge_nl_update(sponge_nl* nl, const int* atom_local, int local_
                                 const float* rcell, int step, int update,
                                 const int* excluded_list_start, const int* excluded_list,
                                 const int* excluded_numbers) {
+    return nl_update_impl(nl, atom_local, local_atom_numbers, ghost_numbers, crd, cell, rcell, step,
+                          update, excluded_list_start, excluded_list, excluded_numbers, false);
+}
+
+// need_cleared: the caller guarantees the rebuild flag is already 0 (the MD
+// driver clears it after the pair kernel), so the automatic-refresh check
+// only has to raise it.
+int sa::nl_update_impl(sponge_nl* nl, const int* atom_local, int local_atom_numbers,
+                       int ghost_numbers, const float* crd, const float* cell, const float* rcell,
+                       int step, int update, const int* excluded_list_start,
+                       const int* excluded_list, const int* excluded_numbers, bool need_cleared) {
     if (!nl || !crd || !cell || !rcell) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
     if (ghost_numbers != 0) return SPONGE_AMD_ERROR_NOT_IMPLEMENTED;
     if (local_atom_numbers <= 0) return 0;  // :980
@@ -573,7 +584,7 @@ extern "C" int sponge_nl_update(sponge_nl* nl, const int* atom_local, int local_
     if (update == SPONGE_AMD_NL_FORCED || !nl->built) {
         hipLaunchKernelGGL(k_nl_set, dim3(1), dim3(1), 0, s, nl->d_need, 1);
     } else if (auto_refresh) {
-        hipLaunchKernelGGL(k_nl_set, dim3(1), dim3(1), 0, s, nl->d_need, 0);
+        if (!need_cleared) hipLaunchKernelGGL(k_nl_set, dim3(1), dim3(1), 0, s, nl->d_need, 0);
         const float permit2 = nl->skin * nl->skin * nl->opt.skin_permit * nl->opt.skin_permit;
         if (nl->ortho)
             hipLaunchKernelGGL(k_nl_check<true>, dim3(grid_for(N, 256)), dim3(256), 0, s, N, crd,

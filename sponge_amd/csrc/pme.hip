@@ -432,8 +432,13 @@ __device__ __forceinline__ int stencil_index(const Near& nr, int kx, int ky, int
 // (PM_force.cpp:787-812): q Mx(kx) My(ky) Mz(kz).
 
 // Bin the atoms by the brick of their base point (and clear the FFT energy /
-// virial accumulators of this evaluation).  Slots are taken with one atomic
-// per distinct bin of a wavefront (neighbouring atoms share bins).
+// virial accumulators of this evaluation).  Slots: an LDS histogram of the
+// block's atoms per brick, then one global atomic per distinct brick of the
+// block, all issued at once (one round trip instead of a chain); for grids
+// with more bricks than the LDS table holds, one atomic per distinct brick of
+// each wavefront.
+constexpr int kBinLds = 4096;  // bricks tracked in LDS by the bin kernel
+
 __global__ void __launch_bounds__(256) k_pme_bin(int N, const float* __restrict__ crd,
                                                  const float* __restrict__ charge, LT6 rc, int fx,
                                                  int fy, int fz, int bx, int by, int bz, int nby,
@@ -441,6 +446,8 @@ __global__ void __launch_bounds__(256) k_pme_bin(int N, const float* __restrict_
                                                  float4* __restrict__ bw, int* __restrict__ bu,
                                                  float4* __restrict__ ow, int* __restrict__ ou,
                                                  double* __restrict__ acc) {
+    __shared__ int lcount[kBinLds];
+    __shared__ int lbase[kBinLds];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < 7 && blockIdx.x == 0) acc[i] = 0.0;
     const bool live = i < N;
@@ -452,17 +459,28 @@ __global__ void __launch_bounds__(256) k_pme_bin(int N, const float* __restrict_
         w = make_float4(nr.fx, nr.fy, nr.fz, charge[i]);
         u = nr.ux | (nr.uy << 10) | (nr.uz << 20);
     }
-    unsigned long long todo = __ballot(live);
     int slot = 0;
-    while (todo) {
-        const int leader = __builtin_ctzll(todo);
-        const int hb = __builtin_amdgcn_readlane(h, leader);
-        const unsigned long long peers = __ballot(h == hb);
-        int base = 0;
-        if (__lane_id() == leader) base = atomicAdd(&bcount[hb], __popcll(peers));
-        base = __builtin_amdgcn_readlane(base, leader);
-        if (h == hb) slot = base + lanes_below(peers);
-        todo &= ~peers;
+    if (nb <= kBinLds) {
+        if (live) lcount[h] = 0;
+        __syncthreads();
+        int local = 0;
+        if (live) local = atomicAdd(&lcount[h], 1);
+        __syncthreads();
+        if (live && local == 0) lbase[h] = atomicAdd(&bcount[h], lcount[h]);
+        __syncthreads();
+        if (live) slot = lbase[h] + local;
+    } else {
+        unsigned long long todo = __ballot(live);
+        while (todo) {
+            const int leader = __builtin_ctzll(todo);
+            const int hb = __builtin_amdgcn_readlane(h, leader);
+            const unsigned long long peers = __ballot(h == hb);
+            int base = 0;
+            if (__lane_id() == leader) base = atomicAdd(&bcount[hb], __popcll(peers));
+            base = __builtin_amdgcn_readlane(base, leader);
+            if (h == hb) slot = base + lanes_below(peers);
+            todo &= ~peers;
+        }
     }
     if (!live) return;
     if (slot < bcap) {
