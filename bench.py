@@ -47,6 +47,24 @@ def pair_kernel_flops(n_in, n_list):
     return 45.0 * n_in + 17.0 * (n_list - n_in)
 
 
+def pair_counts(crd, counts, serial, box, cutoff, chunk=1 << 22):
+    """(pairs within the cutoff, entries) of a reference-format half list:
+    FP32 minimum-image distances of every entry."""
+    N = serial.shape[0]
+    i_idx = np.repeat(np.arange(N), counts)
+    j_idx = serial[np.arange(serial.shape[1])[None, :] < counts[:, None]]
+    crd = np.asarray(crd, np.float32)
+    box = np.asarray(box, np.float32)
+    rbox = np.float32(1.0) / box
+    rc2 = np.float32(cutoff) * np.float32(cutoff)
+    n_in = 0
+    for c in range(0, len(i_idx), chunk):
+        d = crd[j_idx[c:c + chunk]] - crd[i_idx[c:c + chunk]]
+        d -= np.floor(d * rbox + np.float32(0.5)) * box
+        n_in += int(((d * d).sum(1) < rc2).sum())
+    return n_in, int(len(i_idx))
+
+
 def cpu_baseline(s, gpu_rebuild_interval, budget_s=20.0):
     """The oracle's baseline build (the reference CPU path restated with its
     OpenMP + atomic structure, -O3 -ffast-math) on the same coordinates:
@@ -81,6 +99,93 @@ def cpu_baseline(s, gpu_rebuild_interval, budget_s=20.0):
                 ms_per_step=per_step * 1e3)
 
 
+def learned_form(md, s, torch, abi, n_train=40, n_heldout=16, stride=5, epochs=30, lr=2e-3,
+                 period=4, check_interval=8, bound=0.5, timed_steps=400, drift_steps=1000):
+    """The learned multiple-time-step form on the same system: training
+    frames from the plain engine (stride steps apart) and consecutive
+    held-out frames, the surrogate trained on the GPU, its reciprocal-force
+    error 1-3 steps after an exact PME step (the schedule's lags), the graph-
+    replayed ms/step of both schedules and the NVE energy drift of both from
+    the same state."""
+    beta = abi.load().sponge_amd_get_beta(float(s.cutoff), float(s.tolerance))
+    rs = abi.ReciprocalSurrogate(s.n_atoms, beta, feature_cutoff=s.cutoff,
+                                 max_frames=n_train + n_heldout)
+    t0 = time.perf_counter()
+    md.collect_frames(rs, n_train, stride)
+    md.collect_frames(rs, n_heldout, 1)
+    t_collect = time.perf_counter() - t0
+    rs.fit_normalization(n_train)
+    t0 = time.perf_counter()
+    losses = rs.train(n_train, epochs, lr)
+    t_train = time.perf_counter() - t0
+    ev = {lag: rs.evaluate(n_train + 3, n_train + n_heldout, lag) for lag in (1, 2, 3)}
+    crd0, vel0, _ = md.state()
+
+    def timed(steps):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        md.run(steps, use_graph=True)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) * 1e3 / steps
+
+    md.set_surrogate(None)
+    md.run(8, use_graph=True)
+    plain_ms = timed(timed_steps)
+    md.set_surrogate(rs, period, check_interval, bound)
+    md.run(2 * period * check_interval, use_graph=True)
+    learned_ms = timed(timed_steps)
+    stats = md.surrogate_stats()
+    # kernel breakdown of the learned schedule (eager, timers on)
+    abi.set_kernel_timing_filter(["rsur_features", "rsur_mlp"])
+    abi.reset_kernel_timing()
+    abi.set_kernel_timing(True)
+    md.run(4 * period, use_graph=False)
+    torch.cuda.synchronize()
+    abi.set_kernel_timing(False)
+    kt = {k: abi.kernel_timing(k) for k in ("rsur_features", "rsur_mlp")}
+    abi.set_kernel_timing_filter(None)
+
+    def drift(use):
+        md.set_state(crd0, vel0)
+        md.set_surrogate(rs if use else None, period, check_interval, bound)
+        e = []
+        every = drift_steps // 10
+        for k in range(11):
+            md.run(1, energy_interval=1)
+            en = md.energies()
+            e.append(en["potential"] + en["kinetic"])
+            if k < 10:
+                md.run(every - 1, use_graph=True)
+        x = np.arange(11) * every
+        slope = float(np.polyfit(x, np.asarray(e), 1)[0])
+        return slope * 1000.0, e
+
+    d_plain, e_plain = drift(False)
+    d_learned, e_learned = drift(True)
+    md.set_surrogate(None)
+    rmse = {f"{lag}_step": ev[lag]["model_anchored"] for lag in (1, 2, 3)}
+    return dict(
+        speedup_vs_plain=plain_ms / learned_ms, ms_per_step_learned=learned_ms,
+        ms_per_step_plain=plain_ms,
+        recip_rel_rmse_anchored=rmse,
+        recip_rel_rmse_schedule_mean=float(np.mean(list(rmse.values()))),
+        meets_1pct_rmse=bool(max(rmse.values()) <= 0.01),
+        ablation={f"{lag}_step": dict(stale_pme=ev[lag]["stale"],
+                                      baseline_anchored=ev[lag]["baseline_anchored"],
+                                      network_anchored=ev[lag]["model_anchored"])
+                  for lag in (1, 2, 3)},
+        network_alone_rel_rmse=ev[1]["model"], baseline_alone_rel_rmse=ev[1]["baseline"],
+        energy_drift_kcal_mol_per_1000_steps=dict(plain=d_plain, learned=d_learned),
+        checks=stats["checks"], fallbacks=stats["fallbacks"],
+        kernel_avg_us={k: (v[0] * 1e3 / v[1] if v[1] else None) for k, v in kt.items()},
+        training=dict(frames=n_train, heldout=n_heldout, stride_steps=stride, epochs=epochs,
+                      lr=lr, loss_first_epoch=losses[0], loss_last_epoch=losses[-1],
+                      collect_s=t_collect, train_s=t_train),
+        schedule=f"PME every {period}th step (and on energy steps), the surrogate anchored on the "
+                 f"last PME step in between; every {check_interval}th period one surrogate step "
+                 f"is checked against PME, fallback above {bound} relative RMS error")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +196,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--eager", action="store_true", help="time eager launches instead of graph replays")
     ap.add_argument("--breakdown-steps", type=int, default=100)
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned multiple-time-step form")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -156,8 +263,7 @@ def main():
     rebuilds = rb1 - rb0
     rebuild_interval = args.steps / max(rebuilds, 1)
 
-    # pair statistics of the current list, counted by the oracle
-    from oracle import oracle as O
+    # pair statistics of the current list
     N = s.n_atoms
     M = 1200
     counts = torch.zeros(N, dtype=torch.int32, device="cuda")
@@ -167,7 +273,8 @@ def main():
     assert rc == 0
     crd, _, _ = md.state()
     torch.cuda.synchronize()
-    n_in, n_list = O.pair_counts(s, counts.cpu().numpy(), serial.view(N, M).cpu().numpy(), crd=crd)
+    n_in, n_list = pair_counts(crd, counts.cpu().numpy(), serial.view(N, M).cpu().numpy(), s.box,
+                               s.cutoff)
 
     assert lj_n == args.steps, (lj_n, args.steps)
     lj_avg_s = lj_ms * 1e-3 / max(lj_n, 1)
@@ -197,6 +304,8 @@ def main():
                launch="hip graph replay" if use_graph else "eager",
                kernel_avg_us_eager_pass={k: (v[0] * 1e3 / v[1] if v[1] else None)
                                          for k, v in kernel.items()})
+    if not args.no_learned:
+        out["learned"] = learned_form(md, s, torch, abi)
     if not args.no_cpu_baseline:
         import dataclasses
         s_now = dataclasses.replace(s, crd=crd)

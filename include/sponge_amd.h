@@ -271,6 +271,85 @@ int sponge_pme_grids(sponge_pme* pme, float* Q, float* FBCFQ);
 int sponge_pme_clear(sponge_pme* pme);
 
 /* ------------------------------------------------------------------------- */
+/* Learned reciprocal-force surrogate                                         */
+/* ------------------------------------------------------------------------- */
+
+/* A per-atom MLP (48 -> 64 -> 64 -> 3, SiLU, BF16 MFMA with FP32
+ * accumulation) over charge-environment features of the neighbor-list
+ * partners within feature_cutoff predicts the PME reciprocal force as
+ * q_i (c_0 U_0 + c_1 U_1 + c_2 U_2): U_0 the reciprocal pair field of the
+ * list partners (smooth cutoff), U_1 that of the excluded partners, U_2 the
+ * charge dipole of the environment.  No counterpart in the reference: it
+ * stands beside Particle_Mesh::PME_Reciprocal_Force_With_Energy_And_Virial
+ * (PM_force.cpp:1070-1169), which stays the fallback; its impulse
+ * multiple-time-step switch (PME update_interval, PM_force.cpp:1082,
+ * 1135-1140) is the method the surrogate is compared with.  Training frames
+ * (features, basis, target force) stay resident in device memory. */
+typedef struct sponge_rsur sponge_rsur;
+
+typedef struct sponge_rsur_options {
+    float feature_cutoff; /* Rf in A, at most the neighbor list's cutoff (default 10) */
+    int max_frames;       /* training frames kept on the device (default 0) */
+    int seed;             /* initial weights and frame shuffling (default 1) */
+} sponge_rsur_options;
+
+void sponge_rsur_default_options(sponge_rsur_options* opt);
+
+/* Floats in a model blob: parameters (W1[64][48], b1[64], W2[64][64], b2[64],
+ * W3[3][64], b3[3]) followed by the feature mean[48] and inverse std[48]. */
+int sponge_rsur_model_size(void);
+
+/* beta is the Ewald splitting parameter of the PME module it stands in for. */
+int sponge_rsur_initial(sponge_rsur** out, int atom_numbers, float beta,
+                        const sponge_rsur_options* opt, void* stream);
+
+/* Features and basis of the current coordinates over the neighbor list
+ * (which must have been built; its partners within the cutoff are complete)
+ * and the exclusion CSR (device arrays, may be NULL). */
+int sponge_rsur_features(sponge_rsur* h, sponge_nl* nl, const float* crd, const float* cell,
+                         const float* rcell, const float* charge, const int* excluded_list_start,
+                         const int* excluded_list, const int* excluded_numbers);
+
+/* The last features [N][48] and basis [N][3][3] in atom order (device arrays;
+ * either may be NULL). */
+int sponge_rsur_export_features(sponge_rsur* h, float* feat, float* basis);
+
+/* frc += predicted reciprocal force of the last features (device VECTOR[N]). */
+int sponge_rsur_predict(sponge_rsur* h, float* frc);
+
+/* Model blob of sponge_rsur_model_size() floats, host memory; synchronise. */
+int sponge_rsur_get_model(sponge_rsur* h, float* model);
+int sponge_rsur_set_model(sponge_rsur* h, const float* model);
+
+/* Stores the last features and basis with the target rec_frc - mean(rec_frc)
+ * (device VECTOR[N], the PME reciprocal force of the same coordinates). */
+int sponge_rsur_add_frame(sponge_rsur* h, const float* rec_frc);
+int sponge_rsur_frame_count(sponge_rsur* h, int* n);
+
+/* Feature mean and inverse standard deviation from the first n stored frames. */
+int sponge_rsur_fit_normalization(sponge_rsur* h, int n_frames);
+
+/* Loss (mean squared force error per component) and its gradient on one
+ * stored frame (host arrays of model-parameter length / 1 double; either may
+ * be NULL); synchronises. */
+int sponge_rsur_gradient(sponge_rsur* h, int frame, float* grad, double* loss);
+
+/* Adam over the first n_train_frames stored frames, one step per frame,
+ * `epochs` passes in shuffled order; epoch_loss[epochs] (host, may be NULL)
+ * gets the mean loss of each pass.  Synchronises. */
+int sponge_rsur_train(sponge_rsur* h, int n_train_frames, int epochs, float lr, double* epoch_loss);
+
+/* Relative RMS force errors over stored frames [begin, end) (frames are
+ * kept in atom order): out[0] the model, out[1] the physics baseline
+ * c = (1, 1, 0); with lag > 0 each frame f is also predicted anchored on
+ * frame f - lag (prediction + that frame's target - its prediction):
+ * out[2] model anchored, out[3] baseline anchored, out[4] the stale target
+ * of frame f - lag itself.  Synchronises. */
+int sponge_rsur_evaluate(sponge_rsur* h, int frame_begin, int frame_end, int lag, double* out);
+
+int sponge_rsur_clear(sponge_rsur* h);
+
+/* ------------------------------------------------------------------------- */
 /* NVE step driver (the caller side of the path)                              */
 /* ------------------------------------------------------------------------- */
 
@@ -315,6 +394,28 @@ int sponge_md_set_state(sponge_md* md, const float* crd, const float* vel);
 
 /* The module handles the driver owns (for timing and list export). */
 int sponge_md_handles(sponge_md* md, sponge_nl** nl, sponge_lj** lj, sponge_pme** pme);
+
+/* Learned multiple-time-step schedule (no counterpart in the reference; the
+ * reference's own way to skip reciprocal steps is PME update_interval,
+ * PM_force.cpp:1082, 1135-1140): with rsur set, the steps with step % period
+ * != 0 that evaluate no energy take the surrogate's predicted reciprocal force
+ * instead of PME's.  The first surrogate step after the PME step of every
+ * check_interval-th period (0 = never) is also evaluated with PME; when the
+ * prediction's relative RMS error exceeds bound, that step uses PME's force
+ * (the fallback).  The surrogate runs on the driver's stream from then on;
+ * its feature cutoff must not exceed the driver's cutoff.  rsur = NULL
+ * restores plain PME.  Graph replay captures whole cycles of the schedule. */
+int sponge_md_set_surrogate(sponge_md* md, sponge_rsur* rsur, int period, int check_interval,
+                            float bound);
+
+/* out[3] = {surrogate steps, checked steps, fallbacks}; *last_error = the
+ * relative RMS error of the last checked step (either may be NULL). */
+int sponge_md_surrogate_stats(sponge_md* md, long long* out, double* last_error);
+
+/* Runs n_frames * stride plain steps (eager); the last step of every stride
+ * stores a training frame (features and PME reciprocal force of that step's
+ * coordinates) into rsur.  Synchronises. */
+int sponge_md_collect_frames(sponge_md* md, sponge_rsur* rsur, int n_frames, int stride);
 
 int sponge_md_clear(sponge_md* md);
 

@@ -97,6 +97,24 @@ SIGNATURES = {
                                C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), _I, C.POINTER(_I),
                                _FP, _F, _F, _F, C.POINTER(NLOptions), C.POINTER(PMEOptions), _P]),
     "sponge_md_run": (_I, [_P, _I, _I, _I]),
+    "sponge_md_set_surrogate": (_I, [_P, _P, _I, _I, _F]),
+    "sponge_md_surrogate_stats": (_I, [_P, C.POINTER(C.c_longlong), C.POINTER(C.c_double)]),
+    "sponge_md_collect_frames": (_I, [_P, _P, _I, _I]),
+    "sponge_rsur_default_options": (None, [C.c_void_p]),
+    "sponge_rsur_model_size": (_I, []),
+    "sponge_rsur_initial": (_I, [C.POINTER(_P), _I, _F, C.c_void_p, _P]),
+    "sponge_rsur_features": (_I, [_P, _P, _P, _FP, _FP, _P, _P, _P, _P]),
+    "sponge_rsur_export_features": (_I, [_P, _P, _P]),
+    "sponge_rsur_predict": (_I, [_P, _P]),
+    "sponge_rsur_get_model": (_I, [_P, _FP]),
+    "sponge_rsur_set_model": (_I, [_P, _FP]),
+    "sponge_rsur_add_frame": (_I, [_P, _P]),
+    "sponge_rsur_frame_count": (_I, [_P, C.POINTER(_I)]),
+    "sponge_rsur_fit_normalization": (_I, [_P, _I]),
+    "sponge_rsur_gradient": (_I, [_P, _I, _FP, C.POINTER(C.c_double)]),
+    "sponge_rsur_train": (_I, [_P, _I, _I, _F, C.POINTER(C.c_double)]),
+    "sponge_rsur_evaluate": (_I, [_P, _I, _I, _I, C.POINTER(C.c_double)]),
+    "sponge_rsur_clear": (_I, [_P]),
     "sponge_md_energies": (_I, [_P, C.POINTER(C.c_double)]),
     "sponge_md_get_state": (_I, [_P, _FP, _FP, _FP]),
     "sponge_md_set_state": (_I, [_P, _FP, _FP]),
@@ -346,6 +364,95 @@ class ParticleMesh:
     __del__ = clear
 
 
+class RsurOptions(C.Structure):
+    _fields_ = [("feature_cutoff", C.c_float), ("max_frames", C.c_int), ("seed", C.c_int)]
+
+
+class ReciprocalSurrogate:
+    """Learned reciprocal-force surrogate (no counterpart in the reference;
+    it stands beside Particle_Mesh's reciprocal force, PM_force.cpp:1070-1169)."""
+
+    N_FEAT = 48
+
+    def __init__(self, atom_numbers, beta, stream=None, **opts):
+        self.lib = load()
+        o = RsurOptions()
+        self.lib.sponge_rsur_default_options(C.byref(o))
+        for k, v in opts.items():
+            setattr(o, k, v)
+        self.opt = o
+        h = C.c_void_p()
+        _check(self.lib.sponge_rsur_initial(C.byref(h), int(atom_numbers), float(beta), C.byref(o),
+                                            _stream_ptr(stream)), "sponge_rsur_initial")
+        self.h = h
+        self.N = int(atom_numbers)
+        self.model_size = self.lib.sponge_rsur_model_size()
+
+    def features(self, nl, crd, charge, box, excl=None):
+        es, el, en = excl if excl is not None else (None, None, None)
+        _check(self.lib.sponge_rsur_features(self.h, nl.h, _ptr(crd), *box.args(), _ptr(charge),
+                                             _ptr(es), _ptr(el), _ptr(en)), "sponge_rsur_features")
+
+    def export_features(self, feat=None, basis=None):
+        _check(self.lib.sponge_rsur_export_features(self.h, _ptr(feat), _ptr(basis)),
+               "sponge_rsur_export_features")
+
+    def predict(self, frc):
+        _check(self.lib.sponge_rsur_predict(self.h, _ptr(frc)), "sponge_rsur_predict")
+
+    def get_model(self):
+        m = np.zeros(self.model_size, np.float32)
+        _check(self.lib.sponge_rsur_get_model(self.h, m.ctypes.data_as(_FP)), "sponge_rsur_get_model")
+        return m
+
+    def set_model(self, model):
+        m = np.ascontiguousarray(model, np.float32)
+        assert m.size == self.model_size
+        _check(self.lib.sponge_rsur_set_model(self.h, m.ctypes.data_as(_FP)), "sponge_rsur_set_model")
+
+    def add_frame(self, rec_frc):
+        _check(self.lib.sponge_rsur_add_frame(self.h, _ptr(rec_frc)), "sponge_rsur_add_frame")
+
+    def frame_count(self):
+        n = C.c_int()
+        _check(self.lib.sponge_rsur_frame_count(self.h, C.byref(n)), "sponge_rsur_frame_count")
+        return n.value
+
+    def fit_normalization(self, n_frames):
+        _check(self.lib.sponge_rsur_fit_normalization(self.h, int(n_frames)),
+               "sponge_rsur_fit_normalization")
+
+    def gradient(self, frame):
+        g = np.zeros(self.model_size - 2 * self.N_FEAT, np.float32)
+        loss = C.c_double()
+        _check(self.lib.sponge_rsur_gradient(self.h, int(frame), g.ctypes.data_as(_FP),
+                                             C.byref(loss)), "sponge_rsur_gradient")
+        return loss.value, g
+
+    def train(self, n_train_frames, epochs, lr):
+        out = (C.c_double * max(1, epochs))()
+        _check(self.lib.sponge_rsur_train(self.h, int(n_train_frames), int(epochs), float(lr), out),
+               "sponge_rsur_train")
+        return list(out)[:epochs]
+
+    def evaluate(self, begin, end, lag=0):
+        """Relative RMS errors over stored frames [begin, end): model,
+        baseline and, with lag > 0, model / baseline anchored on frame f - lag
+        and the stale force of frame f - lag."""
+        out = (C.c_double * 5)()
+        _check(self.lib.sponge_rsur_evaluate(self.h, int(begin), int(end), int(lag), out),
+               "sponge_rsur_evaluate")
+        keys = ("model", "baseline", "model_anchored", "baseline_anchored", "stale")
+        return dict(zip(keys, list(out)))
+
+    def clear(self):
+        if self.h:
+            self.lib.sponge_rsur_clear(self.h)
+            self.h = None
+
+    __del__ = clear
+
+
 class MDSystem:
     """NVE step driver over a System (sponge_amd.system.System)."""
 
@@ -424,6 +531,22 @@ class MDSystem:
             t = 2 * ek / (dof * KB)
             if t > 0:
                 self.set_state(vel=vel * np.float32(np.sqrt(temperature / t)))
+
+    def set_surrogate(self, rsur, period=4, check_interval=8, bound=0.05):
+        _check(self.lib.sponge_md_set_surrogate(self.h, rsur.h if rsur is not None else None,
+                                                int(period), int(check_interval), float(bound)),
+               "sponge_md_set_surrogate")
+
+    def surrogate_stats(self):
+        out = (C.c_longlong * 3)()
+        err = C.c_double()
+        _check(self.lib.sponge_md_surrogate_stats(self.h, out, C.byref(err)),
+               "sponge_md_surrogate_stats")
+        return dict(surrogate_steps=out[0], checks=out[1], fallbacks=out[2], last_error=err.value)
+
+    def collect_frames(self, rsur, n_frames, stride):
+        _check(self.lib.sponge_md_collect_frames(self.h, rsur.h, int(n_frames), int(stride)),
+               "sponge_md_collect_frames")
 
     def handles(self):
         nl, lj, pm = C.c_void_p(), C.c_void_p(), C.c_void_p()

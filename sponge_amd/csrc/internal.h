@@ -162,6 +162,15 @@ int nl_update_impl(sponge_nl* nl, const int* atom_local, int local_atom_numbers,
                    const int* excluded_list_start, const int* excluded_list,
                    const int* excluded_numbers, bool need_cleared);
 
+// surrogate forms driven by the MD step (sorted positions + charges given)
+int rsur_features_sorted(sponge_rsur* h, sponge_nl* nl, const float4* pos, const float* crd,
+                         const float* charge, const float* cell, const float* rcell,
+                         const int* ex_start, const int* ex_list, const int* ex_num);
+int rsur_predict_sorted(sponge_rsur* h, float* frc, int assign, const float* anchor);
+void rsur_set_stream(sponge_rsur* h, hipStream_t s);
+int rsur_atoms(const sponge_rsur* h);
+float rsur_cutoff(const sponge_rsur* h);
+
 int nl_pack_for_pairs(sponge_nl* nl, const float* crd, const float* charge, const int* type,
                       float4* pos, int* ptype, int* pgid, hipStream_t s);
 }

@@ -12,6 +12,7 @@
 // constraint displacements are consistent with the previous geometry; the
 // velocity correction is (x_constrained - x_unconstrained) / dt, as the
 // reference applies it (SPONGE/constrain/settle.cpp:109-214, 216-258).
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -19,6 +20,9 @@
 #include "timing.h"
 
 using namespace sa;
+
+// reciprocal part of a step: PME, the surrogate, or the surrogate checked against PME
+enum class StepKind { Pme, Surrogate, Checked };
 
 struct sponge_md {
     int N = 0, n_water = 0;
@@ -50,8 +54,23 @@ struct sponge_md {
     float mO = 0, mH1 = 0, mH2 = 0;
     double* d_ek = nullptr;
     long long step = 0;
-    SegmentedCapture capture;      // the captured step (one graph, or segments between timers)
+    SegmentedCapture capture;      // the captured steps (one graph, or segments between timers)
+    int capture_len = 0;           // steps in the captured cycle
+    std::vector<StepKind> capture_kinds;  // reciprocal part of each captured step
     bool need_cleared = false;     // the pair kernel of the last step cleared the rebuild flag
+    // learned multiple-time-step schedule (surrogate between PME steps)
+    sponge_rsur* rsur = nullptr;
+    int sur_period = 0, sur_check = 0;
+    float sur_bound = 0;
+    long long sur_steps = 0;
+    int collect = 0;               // store a training frame on this step
+    sponge_rsur* collect_into = nullptr;
+    float* d_rec = nullptr;        // [3N] PME reciprocal force of a checked / collected step
+    float* d_sur = nullptr;        // [3N] surrogate force of a checked step
+    float* d_anchor = nullptr;     // [3N] PME minus surrogate force of the last PME step
+    bool anchor_valid = false;
+    double* d_sacc = nullptr;      // [0..2] sum of rec, [3] error^2, [4] reference^2, [5] last error
+    unsigned long long* d_scount = nullptr;  // [0] checks, [1] fallbacks
     int use_graph = 0;
     int own_stream = 0;  // the driver created its stream (graph capture needs one)
 };
@@ -176,6 +195,66 @@ __global__ void __launch_bounds__(256) k_integrate(int n_water, const int4* __re
     }
 }
 
+__global__ void k_add_vec(int n, const float* __restrict__ a, float* __restrict__ frc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) frc[i] += a[i];
+}
+
+// check of a surrogate step against PME: acc[3] += |sur - (rec - mean)|^2,
+// acc[4] += |rec - mean|^2 (mean from acc[0..2], the sums of rec)
+__global__ void __launch_bounds__(256) k_sur_check(int N, const float* __restrict__ sur,
+                                                   const float* __restrict__ rec,
+                                                   double* __restrict__ acc) {
+    __shared__ double s[2][256];
+    const double mx = acc[0] / N, my = acc[1] / N, mz = acc[2] / N;
+    double e = 0, r = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        const double px = rec[3 * i] - mx, py = rec[3 * i + 1] - my, pz = rec[3 * i + 2] - mz;
+        const double dx = sur[3 * i] - px, dy = sur[3 * i + 1] - py, dz = sur[3 * i + 2] - pz;
+        e += dx * dx + dy * dy + dz * dz;
+        r += px * px + py * py + pz * pz;
+    }
+    s[0][threadIdx.x] = e;
+    s[1][threadIdx.x] = r;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            s[0][threadIdx.x] += s[0][threadIdx.x + w];
+            s[1][threadIdx.x] += s[1][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomicAdd(&acc[3], s[0][0]);
+        atomicAdd(&acc[4], s[1][0]);
+    }
+}
+
+// fallback decision of a checked step: PME's force when the relative RMS
+// error exceeds the bound (or is not a number), the surrogate's otherwise
+__global__ void k_sur_select(int N, const float* __restrict__ sur, const float* __restrict__ rec,
+                             double* __restrict__ acc, float bound, float* __restrict__ frc,
+                             unsigned long long* __restrict__ count) {
+    const double err = acc[4] > 0 ? sqrt(acc[3] / acc[4]) : 0.0;
+    const bool use_pme = !(err <= (double)bound);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 3 * N) frc[i] += use_pme ? rec[i] : sur[i];
+    if (i == 0) {
+        count[0] += 1;
+        count[1] += use_pme ? 1 : 0;
+        acc[5] = err;
+    }
+}
+
+// anchor = PME - surrogate of this step; the force takes PME's
+__global__ void k_anchor(int n, const float* __restrict__ rec, const float* __restrict__ sur,
+                         float* __restrict__ anchor, float* __restrict__ frc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    anchor[i] = rec[i] - sur[i];
+    frc[i] += rec[i];
+}
+
 __global__ void k_kinetic(int N, const float* __restrict__ vel, const float* __restrict__ mass,
                           double* out) {
     double s = 0;
@@ -251,12 +330,62 @@ void md_free(sponge_md* m) {
     dfree(m->d_water);
     dfree(m->d_free);
     dfree(m->d_ek);
+    dfree(m->d_rec);
+    dfree(m->d_sur);
+    dfree(m->d_anchor);
+    dfree(m->d_sacc);
+    dfree(m->d_scount);
     if (m->own_stream && m->stream) (void)hipStreamDestroy(m->stream);
     m->stream = nullptr;
 }
 
+// FP64 sums of a VECTOR array's components into out[0..2]
+__global__ void __launch_bounds__(256) k_sum_vec3(int N, const float* __restrict__ f,
+                                                  double* __restrict__ out) {
+    __shared__ double s[3][256];
+    double a = 0, b = 0, c = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        a += f[3 * i];
+        b += f[3 * i + 1];
+        c += f[3 * i + 2];
+    }
+    s[0][threadIdx.x] = a;
+    s[1][threadIdx.x] = b;
+    s[2][threadIdx.x] = c;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            for (int k = 0; k < 3; ++k) s[k][threadIdx.x] += s[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; ++k) atomicAdd(&out[k], s[k][0]);
+}
+
+// Reciprocal part of a step under the learned schedule: PME on every
+// period-th step and on energy steps; the surrogate on the others, of which
+// the first after the PME step of every check-th period is also evaluated
+// with PME (the fallback check).  The surrogate's prediction is anchored on
+// the last PME step: F(t) = F_net(t) + (F_PME(t0) - F_net(t0)), so the part
+// of the reciprocal force its local features cannot see (the far field) is
+// carried over from the exact evaluation and only its change is learned.
+StepKind step_kind(const sponge_md* m, long long step, int need_energy) {
+    if (!m->rsur || need_energy || m->sur_period < 2 || step % m->sur_period == 0 ||
+        !m->anchor_valid)
+        return StepKind::Pme;
+    if (m->sur_check > 0 && step % m->sur_period == 1 && (step / m->sur_period) % m->sur_check == 0)
+        return StepKind::Checked;
+    return StepKind::Surrogate;
+}
+
+// Steps in one period of the schedule (the unit a graph captures).
+int schedule_cycle(const sponge_md* m) {
+    if (!m->rsur || m->sur_period < 2) return 1;
+    return m->sur_check > 0 ? m->sur_period * m->sur_check : m->sur_period;
+}
+
 // One MD step (force evaluation + integration), all on m->stream.
-int md_step(sponge_md* m, long long step, int need_energy) {
+int md_step(sponge_md* m, long long step, int need_energy, StepKind* kind_out = nullptr) {
     hipStream_t s = m->stream;
     const int N = m->N;
     int rc = 0;
@@ -288,10 +417,71 @@ int md_step(sponge_md* m, long long step, int need_energy) {
         rc = sponge_lj_long_range_correction(m->lj, 0, nullptr, 1, m->d_potential, m->volume);
         if (rc) return rc;
     }
-    rc = sponge_pme_reciprocal_force(m->pme, m->d_crd, cell, rcell, m->d_charge, m->d_frc, 0,
-                                     need_energy, nullptr, need_energy ? m->d_potential : nullptr,
-                                     (int)step);
-    if (rc) return rc;
+    const StepKind kind = step_kind(m, step, need_energy);
+    if (kind_out) *kind_out = kind;
+    const int N3 = 3 * N;
+    if (kind == StepKind::Pme && m->rsur && !m->collect) {
+        // exact step of the learned schedule: the anchor of the next steps
+        rc = fill_async(m->d_rec, 0.f, N3, s);
+        if (!rc)
+            rc = sponge_pme_reciprocal_force(m->pme, m->d_crd, cell, rcell, m->d_charge, m->d_rec, 0,
+                                             need_energy, nullptr,
+                                             need_energy ? m->d_potential : nullptr, (int)step);
+        if (!rc)
+            rc = rsur_features_sorted(m->rsur, m->nl, m->lj->d_pos, m->d_crd, m->d_charge, cell,
+                                      rcell, m->d_excl_start, m->d_excl_list, m->d_excl_num);
+        if (!rc) rc = rsur_predict_sorted(m->rsur, m->d_sur, 1, nullptr);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_anchor, dim3(grid_for(N3, 256)), dim3(256), 0, s, N3, m->d_rec, m->d_sur,
+                           m->d_anchor, m->d_frc);
+        SA_LAUNCH_CHECK();
+        m->anchor_valid = true;
+    } else if (kind == StepKind::Pme && !m->collect) {
+        rc = sponge_pme_reciprocal_force(m->pme, m->d_crd, cell, rcell, m->d_charge, m->d_frc, 0,
+                                         need_energy, nullptr, need_energy ? m->d_potential : nullptr,
+                                         (int)step);
+        if (rc) return rc;
+    } else if (kind == StepKind::Pme) {
+        // a training frame: the reciprocal force alone, then the features of
+        // the same coordinates
+        rc = fill_async(m->d_rec, 0.f, N3, s);
+        if (!rc)
+            rc = sponge_pme_reciprocal_force(m->pme, m->d_crd, cell, rcell, m->d_charge, m->d_rec, 0,
+                                             need_energy, nullptr,
+                                             need_energy ? m->d_potential : nullptr, (int)step);
+        if (!rc)
+            rc = rsur_features_sorted(m->collect_into, m->nl, m->lj->d_pos, m->d_crd, m->d_charge,
+                                      cell, rcell, m->d_excl_start, m->d_excl_list, m->d_excl_num);
+        if (!rc) rc = sponge_rsur_add_frame(m->collect_into, m->d_rec);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_add_vec, dim3(grid_for(N3, 256)), dim3(256), 0, s, N3, m->d_rec, m->d_frc);
+        SA_LAUNCH_CHECK();
+    } else {
+        // the surrogate stands in for PME; the list's sorted positions and
+        // charges were packed by the pair kernel of this step
+        rc = rsur_features_sorted(m->rsur, m->nl, m->lj->d_pos, m->d_crd, m->d_charge, cell, rcell,
+                                  m->d_excl_start, m->d_excl_list, m->d_excl_num);
+        if (rc) return rc;
+        if (kind == StepKind::Surrogate) {
+            rc = rsur_predict_sorted(m->rsur, m->d_frc, 0, m->d_anchor);
+            if (rc) return rc;
+        } else {
+            rc = rsur_predict_sorted(m->rsur, m->d_sur, 1, m->d_anchor);
+            if (!rc) rc = fill_async(m->d_rec, 0.f, N3, s);
+            if (!rc) rc = fill_async(m->d_sacc, 0.0, 5, s);
+            if (!rc)
+                rc = sponge_pme_reciprocal_force(m->pme, m->d_crd, cell, rcell, m->d_charge, m->d_rec,
+                                                 0, 0, nullptr, nullptr, (int)step);
+            if (rc) return rc;
+            const int blocks = std::min(grid_for(N, 256), 256);
+            hipLaunchKernelGGL(k_sum_vec3, dim3(blocks), dim3(256), 0, s, N, m->d_rec, m->d_sacc);
+            hipLaunchKernelGGL(k_sur_check, dim3(blocks), dim3(256), 0, s, N, m->d_sur, m->d_rec,
+                               m->d_sacc);
+            hipLaunchKernelGGL(k_sur_select, dim3(grid_for(N3, 256)), dim3(256), 0, s, N, m->d_sur,
+                               m->d_rec, m->d_sacc, m->sur_bound, m->d_frc, m->d_scount);
+            SA_LAUNCH_CHECK();
+        }
+    }
     // integration (Main_Iteration, SPONGE/main.cpp:1702-1774): leap-frog and
     // SETTLE in one launch
     const Triangle g{m->geom[0], m->geom[1], m->geom[2], m->geom[3], m->geom[4], m->mO, m->mH1, m->mH2};
@@ -441,38 +631,147 @@ extern "C" int sponge_md_initial(sponge_md** out, int atom_numbers, const float*
 
 extern "C" int sponge_md_run(sponge_md* m, int steps, int energy_interval, int use_graph) {
     if (!m || steps < 0) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
-    for (int k = 0; k < steps; ++k) {
+    const int cyc = schedule_cycle(m);
+    int k = 0;
+    while (k < steps) {
         const long long st = m->step;
-        const int need_e = energy_interval > 0 && st % energy_interval == 0;
-        // graph replay needs the fixed default schedule: automatic list
-        // refresh and a reciprocal part on every step
-        const bool graphable = use_graph && !need_e && m->nl->built &&
-                               m->nl->opt.refresh_interval <= 0 &&
-                               m->pme->opt.update_interval == 1;
-        if (graphable) {
-            // timers switched on or off since the capture: capture again
-            if (!m->capture.items.empty() && m->capture.timing != g_timing)
+        // graph replay needs the fixed default schedule (automatic list
+        // refresh, no impulse interval on the reciprocal part) and replays
+        // whole cycles of the learned schedule that hold no energy step
+        bool graph = use_graph && m->nl->built && m->nl->opt.refresh_interval <= 0 &&
+                     m->pme->opt.update_interval == 1 && st % cyc == 0 && steps - k >= cyc;
+        for (long long t = st; graph && energy_interval > 0 && t < st + cyc; ++t)
+            if (t % energy_interval == 0) graph = false;
+        if (graph) {
+            // timers switched on or off, or another cycle, since the capture
+            if (!m->capture.items.empty() &&
+                (m->capture.timing != g_timing || m->capture_len != cyc))
                 capture_destroy(m->capture);
             if (m->capture.items.empty()) {
                 const int rc0 = capture_begin(m->capture, m->stream);
                 if (rc0) return rc0;
-                const int rc = md_step(m, st, 0);
+                int rc = 0;
+                m->capture_kinds.clear();
+                for (int c = 0; c < cyc && !rc; ++c) {
+                    StepKind kd = StepKind::Pme;
+                    rc = md_step(m, st + c, 0, &kd);
+                    m->capture_kinds.push_back(kd);
+                }
                 const int rc2 = capture_end(m->capture, m->stream);
                 if (rc || rc2) {
                     capture_destroy(m->capture);
                     (void)hipGetLastError();
                     return rc ? rc : rc2;
                 }
+                m->capture_len = cyc;
             }
             const int rc = capture_launch(m->capture, m->stream);
             if (rc) return rc;
+            for (StepKind kd : m->capture_kinds)
+                if (kd != StepKind::Pme) m->sur_steps++;
+            m->step += cyc;
+            k += cyc;
         } else {
-            int rc = md_step(m, st, need_e);
+            const int need_e = energy_interval > 0 && st % energy_interval == 0;
+            StepKind kd = StepKind::Pme;
+            const int rc = md_step(m, st, need_e, &kd);
             if (rc) return rc;
+            if (kd != StepKind::Pme) m->sur_steps++;
+            m->step++;
+            k++;
         }
-        m->step++;
     }
     return 0;
+}
+
+namespace {
+
+int ensure_surrogate_buffers(sponge_md* m) {
+    int e = 0;
+    if (!m->d_rec) e |= dmalloc(&m->d_rec, (size_t)3 * m->N);
+    if (!m->d_sur) e |= dmalloc(&m->d_sur, (size_t)3 * m->N);
+    if (!m->d_anchor) e |= dmalloc(&m->d_anchor, (size_t)3 * m->N);
+    if (!m->d_sacc) e |= dmalloc(&m->d_sacc, 8);
+    if (!m->d_scount) {
+        e |= dmalloc(&m->d_scount, 2);
+        if (!e) e = fill_async(m->d_scount, 0ull, 2, m->stream);
+        if (!e) e = fill_async(m->d_sacc, 0.0, 8, m->stream);
+    }
+    return e;
+}
+
+int check_surrogate(const sponge_md* m, const sponge_rsur* rsur) {
+    if (rsur_atoms(rsur) != m->N) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    // every partner within the feature cutoff must be on the list
+    if (rsur_cutoff(rsur) > m->cutoff) return SPONGE_AMD_ERROR_CONFLICTING_COMMAND;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int sponge_md_set_surrogate(sponge_md* m, sponge_rsur* rsur, int period,
+                                       int check_interval, float bound) {
+    if (!m) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    capture_destroy(m->capture);
+    m->capture_len = 0;
+    m->anchor_valid = false;
+    if (!rsur) {
+        m->rsur = nullptr;
+        return 0;
+    }
+    if (period < 1 || check_interval < 0 || !(bound > 0.f)) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    int rc = check_surrogate(m, rsur);
+    if (!rc) rc = ensure_surrogate_buffers(m);
+    if (rc) return rc;
+    rsur_set_stream(rsur, m->stream);
+    m->rsur = rsur;
+    m->anchor_valid = false;
+    m->sur_period = period;
+    m->sur_check = check_interval;
+    m->sur_bound = bound;
+    return 0;
+}
+
+extern "C" int sponge_md_surrogate_stats(sponge_md* m, long long* out, double* last_error) {
+    if (!m) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    unsigned long long cnt[2] = {0, 0};
+    double err = 0;
+    if (m->d_scount) {
+        SA_CHECK(hipMemcpyAsync(cnt, m->d_scount, sizeof(cnt), hipMemcpyDeviceToHost, m->stream));
+        SA_CHECK(hipMemcpyAsync(&err, m->d_sacc + 5, sizeof(double), hipMemcpyDeviceToHost, m->stream));
+        SA_CHECK(hipStreamSynchronize(m->stream));
+    }
+    if (out) {
+        out[0] = m->sur_steps;
+        out[1] = (long long)cnt[0];
+        out[2] = (long long)cnt[1];
+    }
+    if (last_error) *last_error = err;
+    return 0;
+}
+
+extern "C" int sponge_md_collect_frames(sponge_md* m, sponge_rsur* rsur, int n_frames, int stride) {
+    if (!m || !rsur || n_frames < 0 || stride < 1) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    int rc = check_surrogate(m, rsur);
+    if (!rc) rc = ensure_surrogate_buffers(m);
+    if (rc) return rc;
+    rsur_set_stream(rsur, m->stream);
+    // plain PME steps while collecting
+    sponge_rsur* saved = m->rsur;
+    m->rsur = nullptr;
+    m->collect_into = rsur;
+    for (int f = 0; f < n_frames && !rc; ++f) {
+        for (int k = 0; k < stride && !rc; ++k) {
+            m->collect = k == stride - 1;
+            rc = md_step(m, m->step, 0);
+            if (!rc) m->step++;
+        }
+    }
+    m->collect = 0;
+    m->collect_into = nullptr;
+    m->rsur = saved;
+    if (!rc) SA_CHECK(hipStreamSynchronize(m->stream));
+    return rc;
 }
 
 extern "C" int sponge_md_energies(sponge_md* m, double* out) {
