@@ -186,6 +186,58 @@ def learned_form(md, s, torch, abi, n_train=40, n_heldout=16, stride=5, epochs=3
                  f"is checked against PME, fallback above {bound} relative RMS error")
 
 
+def batched_form(torch, abi, S, steps=500, steps_per_launch=10, n_boxes=48):
+    """The batched form: n_boxes independent water boxes (500-6000 waters,
+    log-uniform, 1.0 g/cm^3, cutoff 9 A, each with its own ~1 A PME grid and
+    autotuned FFT tiling) advanced by one graph launch per steps_per_launch
+    steps, against the same systems run one after another (each replaying
+    its own step graph)."""
+    t0 = time.perf_counter()
+    boxes = S.batched_boxes(n_boxes)
+    t_build = time.perf_counter() - t0
+    ms = [abi.MDSystem(s) for s in boxes]
+    for m in ms:
+        m.equilibrate(100, temperature=300.0, every=10)
+    b = abi.Batch(ms)
+    b.run(1)
+    tiles = b.autotune(3)
+    b.run(2 * steps_per_launch, steps_per_launch)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.run(steps, steps_per_launch)
+    b.synchronize()
+    t_batch = time.perf_counter() - t0
+    # the same systems one after another, each replaying its own step graph
+    for m in ms:
+        m.run(4, use_graph=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for m in ms:
+        m.run(steps, use_graph=True)
+    for m in ms:
+        m.energies()  # synchronises each system's stream
+    t_seq = time.perf_counter() - t0
+    atoms = int(sum(s.n_atoms for s in boxes))
+    dt_ps = boxes[0].dt_ps
+    ms_step = t_batch * 1e3 / steps
+    out = dict(workload=f"{n_boxes} water boxes, {min(s.n_waters for s in boxes)}-"
+                        f"{max(s.n_waters for s in boxes)} waters (log-uniform, seeded), "
+                        f"1.0 g/cm^3, cutoff 9 A, skin 2 A, PME ~1 A order 4, dt 2 fs, SETTLE",
+               systems=n_boxes, atoms=atoms, steps=steps, steps_per_launch=steps_per_launch,
+               ms_per_step_all_systems=ms_step,
+               ns_day_per_system=86.4 * dt_ps / (ms_step * 1e-3),
+               aggregate_ns_day=n_boxes * 86.4 * dt_ps / (ms_step * 1e-3),
+               atom_steps_per_s=atoms * steps / t_batch,
+               ms_per_step_sequential=t_seq * 1e3 / steps,
+               speedup_vs_sequential=t_seq / t_batch,
+               fft_tilings=sorted({str(t) for t in tiles}),
+               build_s=t_build)
+    b.clear()
+    for m in ms:
+        m.clear()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -198,6 +250,7 @@ def main():
     ap.add_argument("--breakdown-steps", type=int, default=100)
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned multiple-time-step form")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched form")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -306,6 +359,8 @@ def main():
                                          for k, v in kernel.items()})
     if not args.no_learned:
         out["learned"] = learned_form(md, s, torch, abi)
+    if not args.no_batched:
+        out["batched"] = batched_form(torch, abi, S)
     if not args.no_cpu_baseline:
         import dataclasses
         s_now = dataclasses.replace(s, crd=crd)

@@ -268,6 +268,17 @@ int sponge_pme_step_print(sponge_pme* pme, float* PM, float* direct, float* reci
  * arrays of fftx*ffty*fftz floats; either may be NULL. */
 int sponge_pme_grids(sponge_pme* pme, float* Q, float* FBCFQ);
 
+/* Autotuner of the FFT passes (no counterpart in the reference, which calls
+ * cuFFT / FFTW plans): lines per workgroup of the z, y and x passes, each the
+ * fastest of {2, 4, 8, 16} by event timing of `reps` reciprocal-force calls on
+ * crd / charge (device arrays) on the module's stream; chosen[3] (host, may
+ * be NULL) gets the choice.  Forces do not depend on it.  Synchronises. */
+int sponge_pme_autotune(sponge_pme* pme, const float* crd, const float* charge, int reps,
+                        int* chosen);
+
+/* Sets the FFT tiling (host int[3]: z, y, x lines per workgroup; even, 2..16). */
+int sponge_pme_set_fft_tiling(sponge_pme* pme, const int* tiles);
+
 int sponge_pme_clear(sponge_pme* pme);
 
 /* ------------------------------------------------------------------------- */
@@ -418,6 +429,34 @@ int sponge_md_surrogate_stats(sponge_md* md, long long* out, double* last_error)
 int sponge_md_collect_frames(sponge_md* md, sponge_rsur* rsur, int n_frames, int stride);
 
 int sponge_md_clear(sponge_md* md);
+
+/* ------------------------------------------------------------------------- */
+/* Batched driver                                                              */
+/* ------------------------------------------------------------------------- */
+
+/* Many independent systems (each a sponge_md on its own stream, with its own
+ * neighbor list and PME grid) advanced together: one HIP graph forks every
+ * system's stream from one origin stream, records steps_per_launch MD steps of
+ * each and joins them, so one launch advances all systems and their kernels
+ * run concurrently.  No counterpart in the reference (one process runs one
+ * system there).  The systems stay owned by the caller. */
+typedef struct sponge_batch sponge_batch;
+
+int sponge_batch_initial(sponge_batch** out, int n_systems, sponge_md* const* systems);
+
+/* sponge_pme_autotune for every distinct grid shape of the batch (the same
+ * choice for systems that share a shape); tiles[3 * n_systems] (host, may be
+ * NULL) gets each system's choice.  Synchronises. */
+int sponge_batch_autotune(sponge_batch* b, int reps, int* tiles);
+
+/* Advances every system by `steps` steps: whole launches of the graph of
+ * steps_per_launch steps, the remainder step by step per system.  Systems on
+ * a schedule a graph cannot hold (list not built yet, fixed refresh or PME
+ * intervals, a surrogate) all run eagerly instead.  Kernel timers must be off
+ * (CONFLICTING_COMMAND otherwise).  Asynchronous. */
+int sponge_batch_run(sponge_batch* b, int steps, int steps_per_launch);
+int sponge_batch_synchronize(sponge_batch* b);
+int sponge_batch_clear(sponge_batch* b);
 
 #ifdef __cplusplus
 }

@@ -100,6 +100,13 @@ SIGNATURES = {
     "sponge_md_set_surrogate": (_I, [_P, _P, _I, _I, _F]),
     "sponge_md_surrogate_stats": (_I, [_P, C.POINTER(C.c_longlong), C.POINTER(C.c_double)]),
     "sponge_md_collect_frames": (_I, [_P, _P, _I, _I]),
+    "sponge_pme_autotune": (_I, [_P, _P, _P, _I, C.POINTER(_I)]),
+    "sponge_pme_set_fft_tiling": (_I, [_P, C.POINTER(_I)]),
+    "sponge_batch_initial": (_I, [C.POINTER(_P), _I, C.POINTER(_P)]),
+    "sponge_batch_autotune": (_I, [_P, _I, C.POINTER(_I)]),
+    "sponge_batch_run": (_I, [_P, _I, _I]),
+    "sponge_batch_synchronize": (_I, [_P]),
+    "sponge_batch_clear": (_I, [_P]),
     "sponge_rsur_default_options": (None, [C.c_void_p]),
     "sponge_rsur_model_size": (_I, []),
     "sponge_rsur_initial": (_I, [C.POINTER(_P), _I, _F, C.c_void_p, _P]),
@@ -359,6 +366,38 @@ class ParticleMesh:
     def clear(self):
         if self.h:
             self.lib.sponge_pme_clear(self.h)
+            self.h = None
+
+    __del__ = clear
+
+
+class Batch:
+    """Batched driver over MDSystem instances (one graph launch advances all)."""
+
+    def __init__(self, systems):
+        self.lib = load()
+        self.systems = list(systems)
+        arr = (_P * len(self.systems))(*[m.h for m in self.systems])
+        h = C.c_void_p()
+        _check(self.lib.sponge_batch_initial(C.byref(h), len(self.systems), arr),
+               "sponge_batch_initial")
+        self.h = h
+
+    def autotune(self, reps=5):
+        t = (C.c_int * (3 * len(self.systems)))()
+        _check(self.lib.sponge_batch_autotune(self.h, int(reps), t), "sponge_batch_autotune")
+        return [tuple(t[3 * i:3 * i + 3]) for i in range(len(self.systems))]
+
+    def run(self, steps, steps_per_launch=8):
+        _check(self.lib.sponge_batch_run(self.h, int(steps), int(steps_per_launch)),
+               "sponge_batch_run")
+
+    def synchronize(self):
+        _check(self.lib.sponge_batch_synchronize(self.h), "sponge_batch_synchronize")
+
+    def clear(self):
+        if self.h:
+            self.lib.sponge_batch_clear(self.h)
             self.h = None
 
     __del__ = clear

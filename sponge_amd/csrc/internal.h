@@ -171,6 +171,13 @@ void rsur_set_stream(sponge_rsur* h, hipStream_t s);
 int rsur_atoms(const sponge_rsur* h);
 float rsur_cutoff(const sponge_rsur* h);
 
+// MD-driver hooks of the batched driver
+hipStream_t md_stream(const sponge_md* m);
+bool md_graphable(const sponge_md* m);
+int md_record_steps(sponge_md* m, int k);
+void md_advance(sponge_md* m, int k);
+int md_state_arrays(sponge_md* m, float** crd, float** charge);
+
 int nl_pack_for_pairs(sponge_nl* nl, const float* crd, const float* charge, const int* type,
                       float4* pos, int* ptype, int* pgid, hipStream_t s);
 }

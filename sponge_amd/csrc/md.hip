@@ -709,6 +709,36 @@ int check_surrogate(const sponge_md* m, const sponge_rsur* rsur) {
 
 }  // namespace
 
+namespace sa {
+
+// hooks of the batched driver (batch.hip): a system's stream, whether its
+// schedule can be captured, the capture of k steps on its stream, and the
+// step counter advanced by replays
+hipStream_t md_stream(const sponge_md* m) { return m->stream; }
+
+bool md_graphable(const sponge_md* m) {
+    return m->nl->built && m->nl->opt.refresh_interval <= 0 && m->pme->opt.update_interval == 1 &&
+           !m->rsur;
+}
+
+int md_record_steps(sponge_md* m, int k) {
+    for (int c = 0; c < k; ++c) {
+        const int rc = md_step(m, m->step + c, 0);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+void md_advance(sponge_md* m, int k) { m->step += k; }
+
+int md_state_arrays(sponge_md* m, float** crd, float** charge) {
+    *crd = m->d_crd;
+    *charge = m->d_charge;
+    return 0;
+}
+
+}  // namespace sa
+
 extern "C" int sponge_md_set_surrogate(sponge_md* m, sponge_rsur* rsur, int period,
                                        int check_interval, float bound) {
     if (!m) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;

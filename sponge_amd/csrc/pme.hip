@@ -1267,6 +1267,81 @@ extern "C" int sponge_pme_grids(sponge_pme* p, float* Q, float* FBCFQ) {
     return 0;
 }
 
+// Autotuner of the FFT passes' tiling: lines per workgroup of the z, y and
+// x passes, each chosen in turn (the others held) as the fastest of a few
+// candidates by HIP-event timing of whole reciprocal-force calls on the
+// module's stream.  Every tiling computes the same per-line transforms, so
+// forces do not depend on the choice.
+extern "C" int sponge_pme_autotune(sponge_pme* p, const float* crd, const float* charge, int reps,
+                                   int* chosen) {
+    if (!p || !crd || !charge || reps < 1) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    hipStream_t s = p->stream;
+    float* frc = nullptr;
+    int rc = dmalloc(&frc, (size_t)3 * p->N);
+    if (rc) return rc;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        if (e0) (void)hipEventDestroy(e0);
+        dfree(frc);
+        return SPONGE_AMD_ERROR_MALLOC_FAILED;
+    }
+    const float* cell = &p->cell.a11;
+    const float* rcell = &p->rcell.a11;
+    float best_ms = 0.f;
+    auto measure = [&](float* ms) -> int {
+        int r = sponge_pme_reciprocal_force(p, crd, cell, rcell, charge, frc, 0, 0, nullptr, nullptr, 0);
+        if (r) return r;
+        SA_CHECK(hipEventRecord(e0, s));
+        for (int k = 0; k < reps && !r; ++k)
+            r = sponge_pme_reciprocal_force(p, crd, cell, rcell, charge, frc, 0, 0, nullptr, nullptr, 0);
+        if (r) return r;
+        SA_CHECK(hipEventRecord(e1, s));
+        SA_CHECK(hipEventSynchronize(e1));
+        SA_CHECK(hipEventElapsedTime(ms, e0, e1));
+        return 0;
+    };
+    int* axes[3] = {&p->tl_z, &p->tl_y, &p->tl_x};
+    const int cand[4] = {2, 4, 8, 16};
+    rc = measure(&best_ms);
+    for (int a = 0; a < 3 && !rc; ++a) {
+        const int keep = *axes[a];
+        int best = keep;
+        for (int c : cand) {
+            if (c == keep) continue;
+            *axes[a] = c;
+            float ms = 0.f;
+            rc = measure(&ms);
+            if (rc) break;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = c;
+            }
+        }
+        *axes[a] = best;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(s);
+    dfree(frc);
+    if (!rc && chosen) {
+        chosen[0] = p->tl_z;
+        chosen[1] = p->tl_y;
+        chosen[2] = p->tl_x;
+    }
+    return rc;
+}
+
+extern "C" int sponge_pme_set_fft_tiling(sponge_pme* p, const int* tiles) {
+    if (!p || !tiles) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    // even line counts up to 16 (the candidates of the autotuner)
+    for (int a = 0; a < 3; ++a)
+        if (tiles[a] < 2 || tiles[a] > 16 || (tiles[a] & 1)) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    p->tl_z = tiles[0];
+    p->tl_y = tiles[1];
+    p->tl_x = tiles[2];
+    return 0;
+}
+
 extern "C" int sponge_pme_clear(sponge_pme* p) {
     if (!p) return 0;
     (void)hipStreamSynchronize(p->stream);

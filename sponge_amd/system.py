@@ -290,3 +290,25 @@ def small_water_box(n_side=4, spacing=3.1, cutoff=None, seed=7, n_na=0, n_cl=0, 
     rc = cutoff if cutoff is not None else min(8.0, 0.5 * L / 1.5 * 2 - 2.0)
     return build_water_ion_box(f"water{n_side}", (n_side,) * 3, L, n_na=n_na, n_cl=n_cl,
                                cutoff=rc, fft=fft, seed=seed)
+
+
+WATER_NUMBER_DENSITY = 1.0 / 18.015 * 0.602214076  # molecules per A^3 at 1.0 g/cm^3
+
+
+def water_box(n_waters, cutoff=9.0, skin=2.0, seed=7, name=None):
+    """Cubic box of n_waters rigid waters at 1.0 g/cm^3 (lattice sites,
+    seeded orientations, 300 K velocities); PME grid from the automatic
+    ~1 A sizing."""
+    L = (n_waters / WATER_NUMBER_DENSITY) ** (1.0 / 3.0)
+    m = int(math.ceil(n_waters ** (1.0 / 3.0)))
+    return build_water_ion_box(name or f"water{n_waters}", (m, m, m), L, n_sites=n_waters,
+                               cutoff=cutoff, skin=skin, fft=None, seed=seed)
+
+
+def batched_boxes(n_boxes=48, lo=500, hi=6000, seed=20260304, cutoff=9.0):
+    """The batched workload: n_boxes water boxes whose sizes are drawn
+    log-uniformly (seeded) between lo and hi waters, each at 1.0 g/cm^3 with
+    its own PME grid at ~1 A and the given cutoff."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(math.log(lo), math.log(hi), n_boxes)).astype(int)
+    return [water_box(int(n), cutoff=cutoff, seed=seed + 1 + k) for k, n in enumerate(sizes)]
