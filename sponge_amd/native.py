@@ -16,6 +16,12 @@ Formats (as the reference's loaders read them):
                       with "table_" (SPONGE/control.cpp:50-94)
 Every reader raises ValueError on a malformed file, where the reference
 raises its BadFileFormat error.
+
+Writers of the outputs the reference's tools read:
+  frc.dat   float32 VECTOR[N] per frame, appended (MD_core/output.hpp:1691-1706)
+  mdout     a header of the printed keys, then one row per print step, every
+            field "%15s" followed by a space (a newline after the last)
+            (SPONGE/utils/control/print.hpp:143-205)
 """
 from __future__ import annotations
 
@@ -133,3 +139,48 @@ def load_system(directory, prefix="sys_flexible", cutoff=10.0, skin=2.0, name=No
                   crd=crd, vel=np.zeros_like(crd), charge=charge, mass=mass, lj_type=types,
                   lj_A=A, lj_B=B, excl_start=es, excl_num=en, excl_list=el, n_waters=0,
                   cutoff=float(cutoff), skin=float(skin), fft=None, tolerance=tolerance)
+
+
+def append_frc_frame(path, frc):
+    """Appends one frame of forces (N x 3) to a frc.dat trajectory."""
+    a = np.ascontiguousarray(frc, np.float32).reshape(-1)
+    with open(path, "ab") as f:
+        a.tofile(f)
+
+
+class MdoutWriter:
+    """mdout in the reference's layout: `keys` as the header, one row per
+    write; values are formatted with `fmt` (a string is written as is)."""
+
+    def __init__(self, path, keys, fmt="%.2f"):
+        self.path = path
+        self.keys = list(keys)
+        self.fmt = fmt
+        with open(path, "w") as f:
+            f.write(self._line(self.keys))
+
+    @staticmethod
+    def _line(fields):
+        return "".join("%15s%s" % (x, "\n" if i == len(fields) - 1 else " ")
+                       for i, x in enumerate(fields))
+
+    def write(self, values):
+        row = []
+        for k in self.keys:
+            v = values[k]
+            row.append(v if isinstance(v, str) else
+                       ("%d" % v if isinstance(v, (int, np.integer)) else self.fmt % v))
+        with open(self.path, "a") as f:
+            f.write(self._line(row))
+
+
+def read_mdout(path):
+    """{key: [values]} of an mdout written in the reference's layout."""
+    with open(path) as f:
+        lines = [l.split() for l in f.read().splitlines() if l.strip()]
+    keys = lines[0]
+    out = {k: [] for k in keys}
+    for row in lines[1:]:
+        for k, v in zip(keys, row):
+            out[k].append(float(v))
+    return out

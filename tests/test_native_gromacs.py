@@ -76,3 +76,23 @@ def test_oracle_against_gromacs(it):
           f"d {d_pm:+.3f})")
     assert abs(d_lj) <= tol["lj_short_vs_lj_sr"]
     assert abs(d_pm) <= tol["pm_vs_coulomb_sr_plus_recip"]
+
+
+def test_output_writers(tmp_path):
+    from sponge_amd import native
+    frc = np.arange(12, dtype=np.float32).reshape(4, 3)
+    p = tmp_path / "frc.dat"
+    native.append_frc_frame(p, frc)
+    native.append_frc_frame(p, 2 * frc)
+    raw = np.fromfile(p, dtype=np.float32)
+    # the reference's extractor takes the last frame of width 3N
+    assert raw.size == 24 and np.array_equal(raw[-12:].reshape(4, 3), 2 * frc)
+    m = tmp_path / "mdout.txt"
+    w = native.MdoutWriter(m, ["step", "potential", "LJ", "PM"])
+    w.write({"step": 0, "potential": -3976.7394, "LJ": 1237.96, "PM": -7437.5})
+    w.write({"step": 10, "potential": -3970.1, "LJ": 1230.0, "PM": -7430.25})
+    lines = open(m).read().splitlines()
+    assert lines[0] == "%15s %15s %15s %15s" % ("step", "potential", "LJ", "PM")
+    assert lines[1] == "%15s %15s %15s %15s" % ("0", "-3976.74", "1237.96", "-7437.50")
+    back = native.read_mdout(m)
+    assert back["step"] == [0.0, 10.0] and back["PM"] == [-7437.5, -7430.25]
