@@ -14,7 +14,9 @@
 // both directions of every pair, each entry a 16-bit position in the row
 // cell's candidate stream (the atoms of its stencil cells, in sorted order).
 // Positions and charges are gathered once per step into a float4 array in
-// sorted space.  One workgroup per cell stages that stream in LDS; one
+// sorted space.  One workgroup (16 wavefronts, so that two workgroups fill a
+// CU's LDS and give 8 wavefronts per SIMD to hide the list and LDS
+// latencies) per cell stages that stream in LDS; one
 // wavefront owns one row and reduces its force by shuffles into a plain store
 // (no atomics); a final pass scatters the rows into the caller's AoS arrays.
 // The kernel over a caller-provided reference-format half list (k_lj_pairs)
@@ -294,7 +296,7 @@ __device__ __forceinline__ void lj_row(int p, int lane, int n, const unsigned sh
 }
 
 template <bool ORTHO, bool NEED_E, bool NEED_V, bool FAST>
-__global__ void __launch_bounds__(512) k_lj_cells(
+__global__ void __launch_bounds__(1024) k_lj_cells(
     const int* __restrict__ start, const int* __restrict__ nranges, int max_ranges,
     const int* __restrict__ g_rfirst, const int* __restrict__ g_roff,
     const int* __restrict__ list_count, const unsigned short* __restrict__ list, int cap,
@@ -417,7 +419,7 @@ int launch_rows(sponge_lj* lj, sponge_nl* nl, LT6 c, LT6 r, float beta, int need
                       sizeof(float2) * lj->T * lj->T;
     const bool fast = beta * lj->cutoff <= kErfcxMax && !need_e && !need_v;
 #define SA_LJ_ROWS(E, V, F)                                                                         \
-    hipLaunchKernelGGL((k_lj_cells<ORTHO, E, V, F>), dim3(nl->G), dim3(512), sh, s, nl->d_cell_start, \
+    hipLaunchKernelGGL((k_lj_cells<ORTHO, E, V, F>), dim3(nl->G), dim3(1024), sh, s, nl->d_cell_start, \
                        nl->d_nranges, nl->max_ranges, nl->d_rfirst, nl->d_roff, nl->d_list_count,    \
                        nl->d_list, nl->cap, lj->d_pos, lj->d_ptype, lj->d_pgid, lj->d_AB_sq, lj->T,  \
                        c, r, lj->rc2_thr, beta, lj->stage, lj->d_facc, lj->d_eacc, lj->d_vacc)
