@@ -282,6 +282,64 @@ int sponge_pme_set_fft_tiling(sponge_pme* pme, const int* tiles);
 int sponge_pme_clear(sponge_pme* pme);
 
 /* ------------------------------------------------------------------------- */
+/* LJ soft core: LJ_SOFT_CORE (SPONGE/Lennard_Jones_force/LJ_soft_core.h)      */
+/* ------------------------------------------------------------------------- */
+
+typedef struct sponge_ljsc sponge_ljsc;
+
+/* The mdin keys read by LJ_SOFT_CORE::Initial (LJ_soft_core.cpp:490-560):
+ * lambda_lj, soft_core_alpha (0.5), soft_core_powfer (1.0), soft_core_sigma
+ * (3.0 A), soft_core_sigma_min (0.0). */
+typedef struct sponge_ljsc_options {
+    float lambda;
+    float alpha;
+    float p;
+    float sigma;
+    float sigma_min;
+} sponge_ljsc_options;
+
+void sponge_ljsc_default_options(sponge_ljsc_options* opt);
+
+/* LJ_SOFT_CORE::Initial: the A- and B-state tables (lower triangles, already
+ * scaled by 12 and 6 as Native_Load_LJ_Soft_Core leaves them) and per-atom
+ * types, host arrays; also the long-range factors of the lambda-mixed and the
+ * B - A C6 sums (LJ_soft_core.cpp:640-668).  At most 64 types per state. */
+int sponge_ljsc_initial(sponge_ljsc** out, int atom_numbers, int type_numbers_A,
+                        int type_numbers_B, const float* LJ_AA, const float* LJ_AB,
+                        const float* LJ_BA, const float* LJ_BB, const int* atom_type_A,
+                        const int* atom_type_B, float cutoff, const sponge_ljsc_options* opt,
+                        void* stream);
+
+int sponge_ljsc_long_range_factors(sponge_ljsc* h, float* factor, float* factor_TI);
+
+/* LJ_Soft_Core_PME_Direct_Force_With_Atom_Energy_And_Virial
+ * (LJ_soft_core.cpp:757-826): frc += soft-core LJ + direct Coulomb forces over
+ * the list; with need_atom_energy, atom_energy += LJ + Coulomb per atom and
+ * atom_direct_pme_energy (may be NULL) = the direct Coulomb part; with
+ * need_virial, atom_virial += the pair virial.  Device arrays. */
+int sponge_ljsc_direct_force(sponge_ljsc* h, sponge_nl* nl, const float* crd, const float* charge,
+                             float* frc, const float* cell, const float* rcell, float pme_beta,
+                             int need_atom_energy, float* atom_energy, int need_virial,
+                             float* atom_virial, float* atom_direct_pme_energy);
+
+/* Get_Partial_H_Partial_Lambda_With_Columb_Direct (LJ_soft_core.cpp:828-880):
+ * dU/dlambda of the LJ part (with the long-range TI term) and of the direct
+ * Coulomb part (charge_BA = charge_B - charge_A, device, may be NULL);
+ * host outputs; synchronises. */
+int sponge_ljsc_du_dlambda(sponge_ljsc* h, sponge_nl* nl, const float* crd, const float* charge,
+                           const float* charge_BA, const float* cell, const float* rcell,
+                           float pme_beta, float* du_lj, float* du_direct);
+
+int sponge_ljsc_long_range_correction(sponge_ljsc* h, int need_pressure, float* d_virial,
+                                      int need_potential, float* d_potential, float volume);
+
+/* LJ_soft_short of the last energy call (sum of the per-atom LJ energies);
+ * synchronises. */
+int sponge_ljsc_step_print(sponge_ljsc* h, float* LJ_soft_short);
+
+int sponge_ljsc_clear(sponge_ljsc* h);
+
+/* ------------------------------------------------------------------------- */
 /* Learned reciprocal-force surrogate                                         */
 /* ------------------------------------------------------------------------- */
 

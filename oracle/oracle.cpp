@@ -23,6 +23,7 @@
 // Pinning status is recorded next to each function ("pinned by: ...").
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <complex>
 #include <cstdint>
@@ -565,6 +566,141 @@ ORC_API void orc_lj_direct(int local_n, int solvent_n, const int* counts, const 
             v[0] += virial.a11; v[1] += virial.a21; v[2] += virial.a22;
             v[3] += virial.a31; v[4] += virial.a32; v[5] += virial.a33;
         }
+    }
+}
+
+// LJ soft core with direct Coulomb, restating
+// Lennard_Jones_And_Direct_Coulomb_Soft_Core_CUDA (SPONGE/Lennard_Jones_force/
+// LJ_soft_core.cpp:195-440) and its pair functions (LJ_soft_core.h:33-214):
+// state A (types typeA, tables AA/AB) weighted 1 - lambda, state B (typeB,
+// BA/BB) weighted lambda; a pair whose repulsion vanishes in exactly one
+// state takes soft-core distances r_A = (r^6 + alpha lambda^p sigma_A)^(1/6),
+// r_B = (r^6 + alpha (1 - lambda)^p sigma_B)^(1/6) with sigma = A / (2B)
+// (sigma6 when A or B is zero, at least sigma6_min).  Outputs are
+// accumulated: forces, per-atom energy (LJ + Coulomb), LJ and direct-Coulomb
+// energies, virial, and the dU/dlambda totals du[0] (LJ), du[1] (direct).
+// Pinned by: the plain kernel (identical states), the end points lambda = 0
+// / 1 with a dummy state in closed form, and dU/dlambda against central
+// differences of the FP64 energy (tests/test_oracle_softcore.py).
+ORC_API void orc_ljsc_direct(int local_n, const int* counts, const int* serial, int max_nb,
+                             const float* crd, const float* charge, const float* charge_BA,
+                             const int* typeA, const int* typeB, const float* AA, const float* AB,
+                             const float* BA, const float* BB, const float* cell6,
+                             const float* rcell6, float cutoff, float pme_beta, float lambda,
+                             float alpha, float p, float sigma6, float sigma6_min, int need_energy,
+                             int need_virial, int need_du, float* frc, float* atom_energy,
+                             float* atom_virial, float* atom_direct_cf_energy, float* atom_LJ_ene,
+                             double* du) {
+    LT cell = ld_lt(cell6), rcell = ld_lt(rcell6);
+    const float lambda_ = 1.0f - lambda;
+    auto lj_force = [](float r, float A, float B) { return (B - A * powf(r, -6.0f)) * powf(r, -8.0f); };
+    auto lj_energy = [](float r, float A, float B) {
+        float r6 = powf(r, -6.0f);
+        return (0.083333333f * A * r6 - 0.166666667f * B) * r6;
+    };
+    auto cf_force = [&](float qq, float r) {
+        float br = pme_beta * r;
+        return qq * powf(r, -3.0f) * (br * TWO_DIVIDED_BY_SQRT_PI * expf(-br * br) + erfcf(br));
+    };
+    auto cf_energy = [&](float qq, float r) { return qq * erfcf(pme_beta * r) / r; };
+    auto sc_sigma = [&](float A, float B) {
+        return (A == 0 || B == 0) ? sigma6 : fmaxf(0.5f * A / B, sigma6_min);
+    };
+    auto sc_dist = [&](float sigma, float r, float lam) {
+        float alp = alpha * powf(lam, p);
+        return powf(powf(r, 6.0f) + alp * sigma, 0.16666667f);
+    };
+    auto sc_lj_force = [](float r, float rs, float A, float B) {
+        return powf(r, 4.0f) * (B - A * powf(rs, -6.0f)) * powf(rs, -12.0f);
+    };
+    auto sc_cf_force = [&](float qq, float r, float rs) {
+        float b = rs * pme_beta;
+        return qq * powf(r, 4.0f) *
+               (expf(-b * b) * TWO_DIVIDED_BY_SQRT_PI * pme_beta + erfcf(b) / rs) * powf(rs, -6.0f);
+    };
+    auto sc_du = [&](float F, float sigma, float rs, float lam) {
+        return 0.16666667f * p * alpha * (1 - lam) * powf(lam + FLT_MIN, p - 1.0f) * F *
+               powf(rs, -4.0f) * sigma;
+    };
+    double du_lj = 0, du_cf = 0;
+    ORC_PRAGMA(omp parallel for schedule(dynamic) reduction(+ : du_lj, du_cf))
+    for (int atom_i = 0; atom_i < local_n; ++atom_i) {
+        V3 frc_record = {0, 0, 0};
+        LT virial = {0, 0, 0, 0, 0, 0};
+        float energy_lj = 0, energy_coulomb = 0, d_lj = 0, d_cf = 0;
+        V3 r1 = ld3(crd, atom_i);
+        const float q1 = charge[atom_i], qba1 = charge_BA ? charge_BA[atom_i] : 0.f;
+        const int* s = serial + (size_t)atom_i * max_nb;
+        for (int j = 0; j < counts[atom_i]; ++j) {
+            int atom_j = s[j];
+            V3 dr = disp_pbc(ld3(crd, atom_j), r1, cell, rcell);
+            float dr_abs = sqrtf(dr.x * dr.x + dr.y * dr.y + dr.z * dr.z);
+            if (!(dr_abs < cutoff)) continue;
+            const float q2 = charge[atom_j], qba2 = charge_BA ? charge_BA[atom_j] : 0.f;
+            const float qq = q1 * q2;
+            const int ta = lj_type_pair(typeA[atom_i], typeA[atom_j]);
+            const int tb = lj_type_pair(typeB[atom_i], typeB[atom_j]);
+            const float Aa = AA[ta], Ab = AB[ta], Ba = BA[tb], Bb = BB[tb];
+            float frc_abs, e_lj = 0, e_cf = 0;
+            if (Ba * Aa != 0 || Ba + Aa == 0) {
+                frc_abs = lambda_ * lj_force(dr_abs, Aa, Ab) + lambda * lj_force(dr_abs, Ba, Bb);
+                frc_abs = frc_abs - cf_force(qq, dr_abs);
+                if (need_energy) {
+                    e_cf = cf_energy(qq, dr_abs);
+                    e_lj = lambda_ * lj_energy(dr_abs, Aa, Ab) + lambda * lj_energy(dr_abs, Ba, Bb);
+                }
+                if (need_du) {
+                    d_lj += lj_energy(dr_abs, Ba, Bb) - lj_energy(dr_abs, Aa, Ab);
+                    d_cf += erfcf(pme_beta * dr_abs) * (qba2 * q1 + q2 * qba1) / dr_abs;
+                }
+            } else {
+                const float sA = sc_sigma(Aa, Ab), sB = sc_sigma(Ba, Bb);
+                const float rA = sc_dist(sA, dr_abs, lambda), rB = sc_dist(sB, dr_abs, lambda_);
+                frc_abs = lambda_ * sc_lj_force(dr_abs, rA, Aa, Ab) + lambda * sc_lj_force(dr_abs, rB, Ba, Bb);
+                frc_abs = frc_abs - (lambda_ * sc_cf_force(qq, dr_abs, rA) + lambda * sc_cf_force(qq, dr_abs, rB));
+                if (need_energy) {
+                    e_cf = lambda_ * cf_energy(qq, rA) + lambda * cf_energy(qq, rB);
+                    e_lj = lambda_ * lj_energy(rA, Aa, Ab) + lambda * lj_energy(rB, Ba, Bb);
+                }
+                if (need_du) {
+                    d_lj += lj_energy(rB, Ba, Bb) - lj_energy(rA, Aa, Ab);
+                    d_lj += sc_du(lj_force(rA, Aa, Ab), sA, rA, lambda) -
+                            sc_du(lj_force(rB, Ba, Bb), sB, rB, lambda_);
+                    d_cf += cf_energy(qq, rB) - cf_energy(qq, rA);
+                    d_cf += sc_du(cf_force(qq, rB), sB, rB, lambda_) - sc_du(cf_force(qq, rA), sA, rA, lambda);
+                    const float eB = erfcf(pme_beta * rB) * (qba2 * q1 + q2 * qba1) / rB;
+                    const float eA = erfcf(pme_beta * rA) * (qba2 * q1 + q2 * qba1) / rA;
+                    d_cf += lambda * eB + lambda_ * eA;
+                }
+            }
+            V3 frc_lin = vscale(frc_abs, dr);
+            frc_record = vadd(frc_record, frc_lin);
+            ORC_ATOMIC frc[3 * atom_j] -= frc_lin.x;
+            ORC_ATOMIC frc[3 * atom_j + 1] -= frc_lin.y;
+            ORC_ATOMIC frc[3 * atom_j + 2] -= frc_lin.z;
+            if (need_virial) virial = lt_sub(virial, virial_fd(frc_lin, dr));
+            energy_lj += e_lj;
+            energy_coulomb += e_cf;
+        }
+        ORC_ATOMIC frc[3 * atom_i] += frc_record.x;
+        ORC_ATOMIC frc[3 * atom_i + 1] += frc_record.y;
+        ORC_ATOMIC frc[3 * atom_i + 2] += frc_record.z;
+        if (need_energy) {
+            atom_energy[atom_i] += energy_lj + energy_coulomb;
+            if (atom_LJ_ene) atom_LJ_ene[atom_i] += energy_lj;
+            if (atom_direct_cf_energy) atom_direct_cf_energy[atom_i] += energy_coulomb;
+        }
+        if (need_virial) {
+            float* v = atom_virial + 6 * atom_i;
+            v[0] += virial.a11; v[1] += virial.a21; v[2] += virial.a22;
+            v[3] += virial.a31; v[4] += virial.a32; v[5] += virial.a33;
+        }
+        du_lj += d_lj;
+        du_cf += d_cf;
+    }
+    if (need_du) {
+        du[0] += du_lj;
+        du[1] += du_cf;
     }
 }
 

@@ -47,6 +47,11 @@ def _load(name):
     lib.orc_lj_direct.argtypes = [C.c_int, C.c_int, _i32p, _i32p, C.c_int, _f32p, _f32p, _i32p,
                                   _f32p, _f32p, _f32p, _f32p, C.c_float, C.c_float, C.c_int,
                                   C.c_int, _f32p, _f32p, _f32p, _f32p, _f32p]
+    lib.orc_ljsc_direct.argtypes = [C.c_int, _i32p, _i32p, C.c_int, _f32p, _f32p, _f32p, _i32p,
+                                    _i32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, C.c_float,
+                                    C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
+                                    C.c_float, C.c_int, C.c_int, C.c_int, _f32p, _f32p, _f32p,
+                                    _f32p, _f32p, _f64p]
     lib.orc_pair_counts.argtypes = [C.c_int, _i32p, _i32p, C.c_int, _f32p, _f32p, _f32p, C.c_float,
                                     C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
     lib.orc_lj_long_range_factor.restype = C.c_float
@@ -174,6 +179,30 @@ def lj_direct(sys_, counts, serial, beta, crd=None, need_energy=True, need_viria
                             frc, e, v, edir, elj)
     return dict(frc=frc.reshape(N, 3), atom_energy=e, atom_virial=v.reshape(N, 6),
                 direct_energy=edir, lj_energy=elj)
+
+
+def ljsc_direct(sys_, counts, serial, beta, sc, crd=None, need_energy=True, need_virial=True,
+                need_du=True):
+    """Soft-core LJ + direct Coulomb on a half list.  `sc` holds typeA, typeB,
+    AA, AB, BA, BB, charge_BA, lambda, alpha, p, sigma6, sigma6_min."""
+    crd = sys_.crd if crd is None else crd
+    N = len(crd)
+    frc = np.zeros(3 * N, np.float32)
+    e = np.zeros(N, np.float32)
+    v = np.zeros(6 * N, np.float32)
+    edir = np.zeros(N, np.float32)
+    elj = np.zeros(N, np.float32)
+    du = np.zeros(2, np.float64)
+    qba = sc.get("charge_BA")
+    lib().orc_ljsc_direct(N, i32(counts), i32(serial.reshape(-1)), serial.shape[1], f32(crd),
+                          f32(sys_.charge), f32(qba if qba is not None else np.zeros(N)),
+                          i32(sc["typeA"]), i32(sc["typeB"]), f32(sc["AA"]), f32(sc["AB"]),
+                          f32(sc["BA"]), f32(sc["BB"]), sys_.cell(), sys_.rcell(), sys_.cutoff,
+                          beta, sc["lambda"], sc["alpha"], sc["p"], sc["sigma6"],
+                          sc["sigma6_min"], int(need_energy), int(need_virial), int(need_du), frc,
+                          e, v, edir, elj, du)
+    return dict(frc=frc.reshape(N, 3), atom_energy=e, atom_virial=v.reshape(N, 6),
+                direct_energy=edir, lj_energy=elj, du_lj=du[0], du_direct=du[1])
 
 
 def pair_counts(sys_, counts, serial, crd=None):

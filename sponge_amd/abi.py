@@ -107,6 +107,15 @@ SIGNATURES = {
     "sponge_batch_run": (_I, [_P, _I, _I]),
     "sponge_batch_synchronize": (_I, [_P]),
     "sponge_batch_clear": (_I, [_P]),
+    "sponge_ljsc_default_options": (None, [C.c_void_p]),
+    "sponge_ljsc_initial": (_I, [C.POINTER(_P), _I, _I, _I, _FP, _FP, _FP, _FP, C.POINTER(_I),
+                                 C.POINTER(_I), _F, C.c_void_p, _P]),
+    "sponge_ljsc_long_range_factors": (_I, [_P, _FP, _FP]),
+    "sponge_ljsc_direct_force": (_I, [_P, _P, _P, _P, _P, _FP, _FP, _F, _I, _P, _I, _P, _P]),
+    "sponge_ljsc_du_dlambda": (_I, [_P, _P, _P, _P, _P, _FP, _FP, _F, _FP, _FP]),
+    "sponge_ljsc_long_range_correction": (_I, [_P, _I, _P, _I, _P, _F]),
+    "sponge_ljsc_step_print": (_I, [_P, _FP]),
+    "sponge_ljsc_clear": (_I, [_P]),
     "sponge_rsur_default_options": (None, [C.c_void_p]),
     "sponge_rsur_model_size": (_I, []),
     "sponge_rsur_initial": (_I, [C.POINTER(_P), _I, _F, C.c_void_p, _P]),
@@ -366,6 +375,68 @@ class ParticleMesh:
     def clear(self):
         if self.h:
             self.lib.sponge_pme_clear(self.h)
+            self.h = None
+
+    __del__ = clear
+
+
+class LJSoftCoreOptions(C.Structure):
+    _fields_ = [("lambda_", C.c_float), ("alpha", C.c_float), ("p", C.c_float),
+                ("sigma", C.c_float), ("sigma_min", C.c_float)]
+
+
+class LJSoftCore:
+    """LJ_SOFT_CORE (SPONGE/Lennard_Jones_force/LJ_soft_core.h)."""
+
+    def __init__(self, atom_numbers, AA, AB, BA, BB, typeA, typeB, cutoff, stream=None,
+                 **opts):
+        self.lib = load()
+        o = LJSoftCoreOptions()
+        self.lib.sponge_ljsc_default_options(C.byref(o))
+        for k, v in opts.items():
+            setattr(o, "lambda_" if k == "lambda" else k, v)
+        tA = np.ascontiguousarray(typeA, np.int32)
+        tB = np.ascontiguousarray(typeB, np.int32)
+        TA = int(round((np.sqrt(8 * len(AA) + 1) - 1) / 2))
+        TB = int(round((np.sqrt(8 * len(BA) + 1) - 1) / 2))
+        arrs = [_fp(x) for x in (AA, AB, BA, BB)]
+        h = C.c_void_p()
+        ip = C.POINTER(C.c_int)
+        _check(self.lib.sponge_ljsc_initial(C.byref(h), int(atom_numbers), TA, TB,
+                                            *[p for _, p in arrs], tA.ctypes.data_as(ip),
+                                            tB.ctypes.data_as(ip), float(cutoff), C.byref(o),
+                                            _stream_ptr(stream)), "sponge_ljsc_initial")
+        self.h = h
+        self.N = int(atom_numbers)
+
+    def direct_force(self, nl, crd, charge, frc, box, beta, atom_energy=None, atom_virial=None,
+                     direct_energy=None):
+        _check(self.lib.sponge_ljsc_direct_force(
+            self.h, nl.h, _ptr(crd), _ptr(charge), _ptr(frc), *box.args(), float(beta),
+            int(atom_energy is not None), _ptr(atom_energy), int(atom_virial is not None),
+            _ptr(atom_virial), _ptr(direct_energy)), "sponge_ljsc_direct_force")
+
+    def du_dlambda(self, nl, crd, charge, box, beta, charge_BA=None):
+        a, b = C.c_float(), C.c_float()
+        _check(self.lib.sponge_ljsc_du_dlambda(self.h, nl.h, _ptr(crd), _ptr(charge),
+                                               _ptr(charge_BA), *box.args(), float(beta),
+                                               C.byref(a), C.byref(b)), "sponge_ljsc_du_dlambda")
+        return a.value, b.value
+
+    def long_range_factors(self):
+        a, b = C.c_float(), C.c_float()
+        _check(self.lib.sponge_ljsc_long_range_factors(self.h, C.byref(a), C.byref(b)),
+               "sponge_ljsc_long_range_factors")
+        return a.value, b.value
+
+    def step_print(self):
+        a = C.c_float()
+        _check(self.lib.sponge_ljsc_step_print(self.h, C.byref(a)), "sponge_ljsc_step_print")
+        return a.value
+
+    def clear(self):
+        if self.h:
+            self.lib.sponge_ljsc_clear(self.h)
             self.h = None
 
     __del__ = clear
