@@ -268,6 +268,13 @@ int sponge_pme_step_print(sponge_pme* pme, float* PM, float* direct, float* reci
  * arrays of fftx*ffty*fftz floats; either may be NULL. */
 int sponge_pme_grids(sponge_pme* pme, float* Q, float* FBCFQ);
 
+/* Particle_Mesh::Update_Box (PM_force.cpp:1240-1256, up_box_bc :1171-1238):
+ * the BC / virial_BC tables and the neutralizing factor for a new
+ * (orthorhombic) box on the same grid, as a barostat step needs (host
+ * LTMatrix3 pair).  Synchronises.  The neighbor list is re-initialized on a
+ * box change, as the reference does (SPONGE/main.cpp:2104-2110). */
+int sponge_pme_update_box(sponge_pme* pme, const float* cell, const float* rcell);
+
 /* Autotuner of the FFT passes (no counterpart in the reference, which calls
  * cuFFT / FFTW plans): lines per workgroup of the z, y and x passes, each the
  * fastest of {2, 4, 8, 16} by event timing of `reps` reciprocal-force calls on

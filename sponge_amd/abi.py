@@ -100,6 +100,7 @@ SIGNATURES = {
     "sponge_md_set_surrogate": (_I, [_P, _P, _I, _I, _F]),
     "sponge_md_surrogate_stats": (_I, [_P, C.POINTER(C.c_longlong), C.POINTER(C.c_double)]),
     "sponge_md_collect_frames": (_I, [_P, _P, _I, _I]),
+    "sponge_pme_update_box": (_I, [_P, _FP, _FP]),
     "sponge_pme_autotune": (_I, [_P, _P, _P, _I, C.POINTER(_I)]),
     "sponge_pme_set_fft_tiling": (_I, [_P, C.POINTER(_I)]),
     "sponge_batch_initial": (_I, [C.POINTER(_P), _I, C.POINTER(_P)]),
@@ -368,6 +369,11 @@ class ParticleMesh:
         _check(self.lib.sponge_pme_step_print(self.h, *[C.byref(x) for x in v]), "pme step_print")
         return dict(PM=v[0].value, direct=v[1].value, reciprocal=v[2].value,
                     self_energy=v[3].value, correction=v[4].value)
+
+    def update_box(self, box):
+        """Particle_Mesh::Update_Box for a new Box (same grid)."""
+        _check(self.lib.sponge_pme_update_box(self.h, *box.args()), "sponge_pme_update_box")
+        self.box = box
 
     def grids(self, Q=None, FBCFQ=None):
         _check(self.lib.sponge_pme_grids(self.h, _ptr(Q), _ptr(FBCFQ)), "sponge_pme_grids")

@@ -840,6 +840,53 @@ int lines_per_block(int n) {
     return t & ~1;
 }
 
+// BC and virial_BC (PM_force.cpp:545-611; recomputed for a new box as
+// up_box_bc does, :1171-1238) in the [kz][ky][kx] order of the convolution
+// pass, from the module's cell, rcell, volume and beta.
+void bc_tables(const sponge_pme* p, std::vector<float>& bct, std::vector<float>& vbct) {
+    const int fx = p->fx, fy = p->fy, fz = p->fz;
+    std::vector<float> B1(fx), B2(fy), B3(fz);
+    for (int k = 0; k < fx; ++k) B1[k] = getb(k, fx, 4);
+    for (int k = 0; k < fy; ++k) B2[k] = getb(k, fy, 4);
+    for (int k = 0; k < fz; ++k) B3[k] = getb(k, fz, 4);
+    bct.assign(p->nfft, 0.f);
+    vbct.assign((size_t)6 * p->nfft, 0.f);
+    const float mprefactor = kPI * kPI / p->beta / p->beta;
+    const LT6& rc = p->rcell;
+    for (int kx = 0; kx < fx; ++kx) {
+        const float kxrp = kx > fx / 2 ? kx - fx : kx;
+        for (int ky = 0; ky < fy; ++ky) {
+            const float kyrp = ky > fy / 2 ? ky - fy : ky;
+            for (int kz = 0; kz <= fz / 2; ++kz) {
+                const float kzrp = kz;
+                // m = MultiplyTranspose({kx, ky, kz}, rcell)
+                const float mx = kxrp * rc.a11;
+                const float my = kxrp * rc.a21 + kyrp * rc.a22;
+                const float mz = kxrp * rc.a31 + kyrp * rc.a32 + kzrp * rc.a33;
+                const float msq = mx * mx + my * my + mz * mz;
+                float bc, v[6];
+                if (kx + ky + kz == 0) {
+                    bc = 0;
+                    for (auto& t : v) t = 0;
+                } else {
+                    bc = (float)1.0 / kPI / msq * expf(-mprefactor * msq) / p->volume;
+                    const float t = 2 / msq * (1 + mprefactor * msq);
+                    v[0] = 1 - t * mx * mx;
+                    v[1] = 0 - t * my * mx;
+                    v[2] = 1 - t * my * my;
+                    v[3] = 0 - t * mz * mx;
+                    v[4] = 0 - t * mz * my;
+                    v[5] = 1 - t * mz * mz;
+                }
+                bc *= B1[kx] * B2[ky] * B3[kz];
+                const size_t ti = ((size_t)kz * fy + ky) * fx + kx;
+                bct[ti] = bc;
+                for (int c = 0; c < 6; ++c) vbct[(size_t)c * p->nfft + ti] = 0.5f * bc * v[c];
+            }
+        }
+    }
+}
+
 void pme_free(sponge_pme* p) {
     for (auto& t : p->d_tw) dfree(t);
     dfree(p->d_Q);
@@ -968,47 +1015,8 @@ extern "C" int sponge_pme_initial(sponge_pme** out, int atom_numbers, const floa
     p->tl_y = lines_per_block(fy);
     p->tl_x = lines_per_block(fx);
 
-    // BC and virial_BC (PM_force.cpp:545-611) in the [kz][ky][kx] order of
-    // the convolution pass
-    std::vector<float> B1(fx), B2(fy), B3(fz);
-    for (int k = 0; k < fx; ++k) B1[k] = getb(k, fx, 4);
-    for (int k = 0; k < fy; ++k) B2[k] = getb(k, fy, 4);
-    for (int k = 0; k < fz; ++k) B3[k] = getb(k, fz, 4);
-    std::vector<float> bct(p->nfft), vbct((size_t)6 * p->nfft);
-    const float mprefactor = kPI * kPI / p->beta / p->beta;
-    const LT6& rc = p->rcell;
-    for (int kx = 0; kx < fx; ++kx) {
-        const float kxrp = kx > fx / 2 ? kx - fx : kx;
-        for (int ky = 0; ky < fy; ++ky) {
-            const float kyrp = ky > fy / 2 ? ky - fy : ky;
-            for (int kz = 0; kz <= fz / 2; ++kz) {
-                const float kzrp = kz;
-                // m = MultiplyTranspose({kx, ky, kz}, rcell)
-                const float mx = kxrp * rc.a11;
-                const float my = kxrp * rc.a21 + kyrp * rc.a22;
-                const float mz = kxrp * rc.a31 + kyrp * rc.a32 + kzrp * rc.a33;
-                const float msq = mx * mx + my * my + mz * mz;
-                float bc, v[6];
-                if (kx + ky + kz == 0) {
-                    bc = 0;
-                    for (auto& t : v) t = 0;
-                } else {
-                    bc = (float)1.0 / kPI / msq * expf(-mprefactor * msq) / p->volume;
-                    const float t = 2 / msq * (1 + mprefactor * msq);
-                    v[0] = 1 - t * mx * mx;
-                    v[1] = 0 - t * my * mx;
-                    v[2] = 1 - t * my * my;
-                    v[3] = 0 - t * mz * mx;
-                    v[4] = 0 - t * mz * my;
-                    v[5] = 1 - t * mz * mz;
-                }
-                bc *= B1[kx] * B2[ky] * B3[kz];
-                const size_t ti = ((size_t)kz * fy + ky) * fx + kx;
-                bct[ti] = bc;
-                for (int c = 0; c < 6; ++c) vbct[(size_t)c * p->nfft + ti] = 0.5f * bc * v[c];
-            }
-        }
-    }
+    std::vector<float> bct, vbct;
+    bc_tables(p, bct, vbct);
     int e = 0;
     for (int d = 0; d < 3; ++d) e |= dmalloc(&p->d_tw[d], dims[d]);
     e |= dmalloc(&p->d_Q, p->nall);
@@ -1264,6 +1272,26 @@ extern "C" int sponge_pme_grids(sponge_pme* p, float* Q, float* FBCFQ) {
     if (FBCFQ)
         SA_CHECK(hipMemcpyAsync(FBCFQ, p->d_FBCFQ, p->nall * sizeof(float), hipMemcpyDeviceToDevice,
                                 p->stream));
+    return 0;
+}
+
+// Particle_Mesh::Update_Box (PM_force.cpp:1240-1256): the BC and virial_BC
+// tables and the neutralizing factor of a new box (same grid); synchronises.
+extern "C" int sponge_pme_update_box(sponge_pme* p, const float* cell, const float* rcell) {
+    if (!p || !cell || !rcell) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    const LT6 c = lt_from(cell);
+    if (!lt_is_orthorhombic(c)) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    p->cell = c;
+    p->rcell = lt_from(rcell);
+    p->volume = p->cell.a11 * p->cell.a22 * p->cell.a33;
+    p->neutralizing_factor = -0.5 * 3.1415926535897932f / (p->beta * p->beta * p->volume);
+    std::vector<float> bct, vbct;
+    bc_tables(p, bct, vbct);
+    SA_CHECK(hipMemcpyAsync(p->d_BCt, bct.data(), bct.size() * sizeof(float), hipMemcpyHostToDevice,
+                            p->stream));
+    SA_CHECK(hipMemcpyAsync(p->d_vBCt, vbct.data(), vbct.size() * sizeof(float),
+                            hipMemcpyHostToDevice, p->stream));
+    SA_CHECK(hipStreamSynchronize(p->stream));
     return 0;
 }
 

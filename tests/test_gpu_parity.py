@@ -269,3 +269,32 @@ def test_pme_excluded(dev, cfg1):
     st = pm.step_print()
     check_total("PM_correction", st["correction"], ref["correction_energy"].sum(dtype=np.float64),
                 ref64["energy"])
+
+
+def test_pme_update_box(dev, cfg1):
+    """Update_Box: a module moved to a new box (affinely scaled coordinates,
+    same grid) gives the forces, energy and virial of a module initialised
+    at that box, and of the oracle there."""
+    import dataclasses
+    import torch
+    from oracle import oracle as O
+    from sponge_amd import abi
+    scale = np.float32(1.03)
+    s2 = dataclasses.replace(cfg1, box=(cfg1.box * scale).astype(np.float32),
+                             crd=(cfg1.crd * scale).astype(np.float32))
+    d = DevSystem(s2, dev)
+    moved = abi.ParticleMesh(d.N, abi.Box(cfg1.box), s2.cutoff, fftx=27, ffty=27, fftz=27)
+    moved.update_box(abi.Box(s2.box))
+    fresh = abi.ParticleMesh(d.N, abi.Box(s2.box), s2.cutoff, fftx=27, ffty=27, fftz=27)
+    out = []
+    for pm in (moved, fresh):
+        frc, vir, pot = d.zeros(d.N, 3), d.zeros(6), d.zeros(1)
+        pm.reciprocal_force(d.crd, d.charge, frc, virial=vir, potential=pot)
+        torch.cuda.synchronize()
+        out.append((frc.cpu().numpy(), vir.cpu().numpy(), float(pot.cpu()[0]), pm.step_print()))
+    (f1, v1, p1, e1), (f2, v2, p2, e2) = out
+    assert rel_rms(f1, f2) < 1e-6 and np.allclose(v1, v2, rtol=1e-6, atol=1e-3)
+    assert rel(p1, p2) < 1e-6 and rel(e1["reciprocal"], e2["reciprocal"]) < 1e-6
+    ref = O.PME(s2, (27, 27, 27), O.get_beta(s2.cutoff, s2.tolerance)).reciprocal()
+    assert rel_rms(f1, ref["frc"]) < FORCE_TOL
+    assert rel(e1["reciprocal"], ref["reciprocal"]) < 1e-5
