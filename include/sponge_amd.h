@@ -471,6 +471,10 @@ int sponge_md_set_state(sponge_md* md, const float* crd, const float* vel);
 /* The module handles the driver owns (for timing and list export). */
 int sponge_md_handles(sponge_md* md, sponge_nl** nl, sponge_lj** lj, sponge_pme** pme);
 
+/* sponge_pme_autotune on the driver's PME module and current coordinates;
+ * tiles[3] (host, may be NULL) gets the choice.  Synchronises. */
+int sponge_md_autotune(sponge_md* md, int reps, int* tiles);
+
 /* Learned multiple-time-step schedule (no counterpart in the reference; the
  * reference's own way to skip reciprocal steps is PME update_interval,
  * PM_force.cpp:1082, 1135-1140): with rsur set, the steps with step % period

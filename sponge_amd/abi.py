@@ -97,6 +97,7 @@ SIGNATURES = {
                                C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), _I, C.POINTER(_I),
                                _FP, _F, _F, _F, C.POINTER(NLOptions), C.POINTER(PMEOptions), _P]),
     "sponge_md_run": (_I, [_P, _I, _I, _I]),
+    "sponge_md_autotune": (_I, [_P, _I, C.POINTER(_I)]),
     "sponge_md_set_surrogate": (_I, [_P, _P, _I, _I, _F]),
     "sponge_md_surrogate_stats": (_I, [_P, C.POINTER(C.c_longlong), C.POINTER(C.c_double)]),
     "sponge_md_collect_frames": (_I, [_P, _P, _I, _I]),
@@ -647,6 +648,12 @@ class MDSystem:
             t = 2 * ek / (dof * KB)
             if t > 0:
                 self.set_state(vel=vel * np.float32(np.sqrt(temperature / t)))
+
+    def autotune(self, reps=5):
+        """Tunes the PME FFT tiling on the current coordinates; returns it."""
+        t = (C.c_int * 3)()
+        _check(self.lib.sponge_md_autotune(self.h, int(reps), t), "sponge_md_autotune")
+        return tuple(t)
 
     def set_surrogate(self, rsur, period=4, check_interval=8, bound=0.05):
         _check(self.lib.sponge_md_set_surrogate(self.h, rsur.h if rsur is not None else None,

@@ -739,6 +739,15 @@ int md_state_arrays(sponge_md* m, float** crd, float** charge) {
 
 }  // namespace sa
 
+// FFT tiling of the driver's PME module tuned on its current coordinates
+// (sponge_pme_autotune); the captured step is rebuilt with the new tiling.
+extern "C" int sponge_md_autotune(sponge_md* m, int reps, int* tiles) {
+    if (!m || reps < 1) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    capture_destroy(m->capture);
+    m->capture_len = 0;
+    return sponge_pme_autotune(m->pme, m->d_crd, m->d_charge, reps, tiles);
+}
+
 extern "C" int sponge_md_set_surrogate(sponge_md* m, sponge_rsur* rsur, int period,
                                        int check_interval, float bound) {
     if (!m) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
