@@ -35,7 +35,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per force-only pair-kernel launch on this workload: rocprofv3
 # FETCH_SIZE and WRITE_SIZE runs scaled by a 256 MiB copy (2.0 x FETCH_SIZE,
 # 1.0 x WRITE_SIZE); commands and table in profiles/traffic_water_ions_95k.md
-PAIR_KERNEL_TRAFFIC_BYTES = 159.9e6
+PAIR_KERNEL_TRAFFIC_BYTES = 159.5e6
 TIMERS = ("lj_pairs", "nl_search", "pme_spread", "pme_gather", "pme_fft_z_r2c", "pme_fft_y",
           "pme_fft_x_conv", "pme_ifft_y", "pme_fft_z_c2r")
 
