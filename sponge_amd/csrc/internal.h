@@ -105,14 +105,17 @@ struct sponge_pme {
     float* d_correction_atom_energy = nullptr;
     double* d_sum = nullptr;
     int tl_z = 2, tl_y = 2, tl_x = 2;  // lines per block of the FFT passes
-    // spread bricks: the grid tiled by Bx x By x Bz bricks; atoms binned by the
-    // brick of their base point, cap per brick, the rest in an overflow list
+    // spread bricks: the grid cut into Bx x By x Bz bricks (the last brick of an
+    // axis takes the remainder); atoms binned by the brick of their base
+    // point, cap per brick, the rest in an overflow list
     int bx = 0, by = 0, bz = 0, nbx = 0, nby = 0, nbz = 0, nb = 0, bcap = 0;
     int* d_bcount = nullptr;      // [nb + 1]: atoms per brick, [nb] overflow count
     float4* d_bw = nullptr;       // [nb][bcap] fractions x, y, z and charge
     int* d_bu = nullptr;          // [nb][bcap] packed base point ux | uy << 10 | uz << 20
     float4* d_ow = nullptr;       // [N] overflow entries
     int* d_ou = nullptr;
+    float* d_tiles = nullptr;     // [nb][tile points] halo tiles of the spread
+    int spread_waves = 1;         // wavefronts (tile copies) per spread workgroup
 };
 
 namespace sa {

@@ -18,6 +18,7 @@
 // The stencil indices of PME_Atom_Near are recomputed on the fly in spread
 // and gather (they are a pure function of the coordinates), so no 64-int
 // per-atom table is stored.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -396,18 +397,12 @@ __device__ __forceinline__ Near atom_near(const float* __restrict__ crd, int i, 
 // lane for its whole kernel (the lane's k never changes), so the hot loops
 // do not reload the coefficient tables.
 struct BspCoef {
-    float a, b, c, d;     // M: a f^3 + b f^2 + c f + d
-    float da, db, dc;     // dM: da f^2 + db f + dc
+    float a, b, c, d;     // M: ((a f + b) f + c) f + d
+    float da, db, dc;     // dM: (da f + db) f + dc
     __device__ __forceinline__ explicit BspCoef(int k)
         : a(cMa[k]), b(cMb[k]), c(cMc[k]), d(cMd[k]), da(cdMa[k]), db(cdMb[k]), dc(cdMc[k]) {}
-    __device__ __forceinline__ float w(float f) const {
-        const float f2 = f * f;
-        return a * f * f2 + b * f2 + c * f + d;
-    }
-    __device__ __forceinline__ float dw(float f) const {
-        const float f2 = f * f;
-        return da * f2 + db * f + dc;
-    }
+    __device__ __forceinline__ float w(float f) const { return fmaf(fmaf(fmaf(a, f, b), f, c), f, d); }
+    __device__ __forceinline__ float dw(float f) const { return fmaf(fmaf(da, f, db), f, dc); }
 };
 
 // grid index of stencil point (kx, ky, kz) below u (PM_force.cpp:737-759)
@@ -422,14 +417,23 @@ __device__ __forceinline__ int stencil_index(const Near& nr, int kx, int ky, int
     return (x * fy + y) * fz + z;
 }
 
-// Brick spread.  The charge grid is tiled by bricks of Bx x By x Bz points;
-// one workgroup owns one brick, accumulates it in LDS and stores it with plain
-// writes, so the grid needs neither atomics nor a clearing pass.  An atom
-// spreads onto the points u - k (k = 0..3 per axis) below its base point u,
-// which lie in the brick of u and the brick below it on each axis; so a brick
-// receives from the atoms binned in it and in its upper neighbours (8 bins),
-// plus the overflow list.  Same weights and order as PME_Q_Spread
+// Tile spread.  The charge grid is cut into bricks of Bx x By x Bz points (the
+// last brick of an axis takes the remainder, Bx..2Bx-1 points).  An atom
+// spreads onto the points u - k (k = 0..3 per axis) below its base point u;
+// so every atom whose base point lies in a brick spreads inside that brick's
+// halo tile, the brick extended by 3 points downwards on each axis.  One
+// workgroup per brick spreads the atoms binned in it into the tile in LDS and
+// stores the tile; a second pass sums, for every grid point, the entries of
+// the (at most 8) tiles covering it in a fixed order.  Neither pass uses an
+// atomic or needs the grid cleared, and the grid is the same on every run
+// for the same binning.  Same weights and order as PME_Q_Spread
 // (PM_force.cpp:787-812): q Mx(kx) My(ky) Mz(kz).
+
+// brick of grid coordinate x on one axis (the last brick takes the remainder)
+__device__ __forceinline__ int brick_of(int x, int B, int nbk) {
+    const int h = x / B;
+    return h < nbk ? h : nbk - 1;
+}
 
 // Bin the atoms by the brick of their base point (and clear the FFT energy /
 // virial accumulators of this evaluation).  Slots: an LDS histogram of the
@@ -441,11 +445,11 @@ constexpr int kBinLds = 4096;  // bricks tracked in LDS by the bin kernel
 
 __global__ void __launch_bounds__(256) k_pme_bin(int N, const float* __restrict__ crd,
                                                  const float* __restrict__ charge, LT6 rc, int fx,
-                                                 int fy, int fz, int bx, int by, int bz, int nby,
-                                                 int nbz, int nb, int bcap, int* __restrict__ bcount,
-                                                 float4* __restrict__ bw, int* __restrict__ bu,
-                                                 float4* __restrict__ ow, int* __restrict__ ou,
-                                                 double* __restrict__ acc) {
+                                                 int fy, int fz, int bx, int by, int bz, int nbx,
+                                                 int nby, int nbz, int nb, int bcap,
+                                                 int* __restrict__ bcount, float4* __restrict__ bw,
+                                                 int* __restrict__ bu, float4* __restrict__ ow,
+                                                 int* __restrict__ ou, double* __restrict__ acc) {
     __shared__ int lcount[kBinLds];
     __shared__ int lbase[kBinLds];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -455,7 +459,8 @@ __global__ void __launch_bounds__(256) k_pme_bin(int N, const float* __restrict_
     float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
     if (live) {
         const Near nr = atom_near(crd, i, rc, fx, fy, fz);
-        h = ((nr.ux / bx) * nby + nr.uy / by) * nbz + nr.uz / bz;
+        h = (brick_of(nr.ux, bx, nbx) * nby + brick_of(nr.uy, by, nby)) * nbz +
+            brick_of(nr.uz, bz, nbz);
         w = make_float4(nr.fx, nr.fy, nr.fz, charge[i]);
         u = nr.ux | (nr.uy << 10) | (nr.uz << 20);
     }
@@ -493,126 +498,178 @@ __global__ void __launch_bounds__(256) k_pme_bin(int N, const float* __restrict_
     }
 }
 
-// Does the stencil below base point u (one axis) reach brick b of size B?
-// (general form, for overflow entries whose bin is not known)
-__device__ __forceinline__ bool reaches(int u, int b, int B, int nbk) {
-    const int h = u / B;
-    if (h == b) return true;
-    const int up = b + 1 == nbk ? 0 : b + 1;
-    return h == up && u - h * B < 3;
-}
-
 __device__ __forceinline__ float lane_f(float v, int lane) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
-__global__ void __launch_bounds__(256) k_pme_spread_bricks(
-    int fx, int fy, int fz, int bx, int by, int bz, int nbx, int nby, int nbz, int nb, int bcap,
+// k / d for 0 <= k < 2^16 and 1 <= d <= 64 from inv = rcp(d) (relative error
+// below 2^-22): (k + 0.5) / d lies at least 0.5 / d from an integer and the
+// product errs by less than 2^16 / d * 2^-21, so truncation is exact
+__device__ __forceinline__ int div_small(int k, float inv) {
+    return (int)(((float)k + 0.5f) * inv);
+}
+
+// Tile geometry (host and device).  A tile is sized for the largest brick,
+// m = B or the remainder brick: (mx + 3) x (my + 3) x (mz + 3) points, dense
+// in global memory.  In LDS the z stride is 1, the y stride mz + 3 rounded up
+// to 4 and the x stride is rounded up to 16 mod 32 floats, which puts the 64
+// stencil offsets kx*sx + ky*sy + kz of one atom into distinct banks per half
+// wavefront for mz = 8.
+struct TileGeom {
+    int tx, ty, tz;  // tile extent (points)
+    int sy, sx;      // LDS strides (floats)
+    __host__ __device__ int lds_floats() const { return tx * sx; }
+    __host__ __device__ int points() const { return tx * ty * tz; }
+};
+
+inline TileGeom tile_geom(int mx, int my, int mz) {
+    TileGeom g;
+    g.tx = mx + 3;
+    g.ty = my + 3;
+    g.tz = mz + 3;
+    g.sy = (g.tz + 3) / 4 * 4;
+    const int p = g.ty * g.sy;
+    g.sx = p + ((16 - p % 32) + 32) % 32;
+    return g;
+}
+
+// One workgroup per brick.  Each lane of a wavefront owns one stencil offset
+// (kx, ky, kz); the brick's atoms are taken one at a time with their base
+// point in scalar registers, so an atom costs the three weight polynomials and
+// one LDS read-add-write per lane, every lane in range.  Every wavefront
+// accumulates into its own copy of the tile: the LDS operations of one
+// wavefront complete in order, so the plain read-add-write needs no atomic (an
+// LDS float atomic costs about three cycles per lane on CDNA4); the copies are
+// summed when the tile is stored.
+__global__ void __launch_bounds__(256) k_pme_spread_tiles(
+    int bx, int by, int bz, int nbx, int nby, int nbz, int nb, int bcap, TileGeom tg,
     const int* __restrict__ bcount, const float4* __restrict__ bw, const int* __restrict__ bu,
-    const float4* __restrict__ ow, const int* __restrict__ ou, float* __restrict__ Q) {
+    const float4* __restrict__ ow, const int* __restrict__ ou, float* __restrict__ tiles) {
     extern __shared__ float acc[];
-    __shared__ int s_first[9], s_off[10], s_dx[9], s_sx[9];
     const int b = blockIdx.x;
     const int cz = b % nbz, cy = (b / nbz) % nby, cx = b / (nbz * nby);
     const int X0 = cx * bx, Y0 = cy * by, Z0 = cz * bz;
-    const int npts = bx * by * bz;
-    for (int k = threadIdx.x; k < npts; k += blockDim.x) acc[k] = 0.f;
-    // sources: bins (cx + dx, cy + dy, cz + dz), d in {0, 1}, each bin once,
-    // then the overflow list; flattened so that all wavefronts share the work
-    if (threadIdx.x == 0) {
-        int ns = 0, off = 0;
-        for (int src = 0; src < 9; ++src) {
-            int first, n, d = 0, sxyz = 0;
-            if (src < 8) {
-                const int dx = src >> 2, dy = (src >> 1) & 1, dz = src & 1;
-                if ((dx && nbx == 1) || (dy && nby == 1) || (dz && nbz == 1)) continue;
-                const int sx = cx + dx == nbx ? 0 : cx + dx;
-                const int sy = cy + dy == nby ? 0 : cy + dy;
-                const int sz = cz + dz == nbz ? 0 : cz + dz;
-                const int sb = (sx * nby + sy) * nbz + sz;
-                first = sb * bcap;
-                n = min(bcount[sb], bcap);
-                d = src;
-                sxyz = sx | (sy << 10) | (sz << 20);
-            } else {
-                first = -1;  // overflow list
-                n = bcount[nb];
-                d = 8;
-            }
-            if (n <= 0) continue;
-            s_first[ns] = first;
-            s_off[ns] = off;
-            s_dx[ns] = d;
-            s_sx[ns] = sxyz;
-            off += n;
-            ++ns;
-        }
-        s_off[ns] = off;
-        for (int k = ns + 1; k < 10; ++k) s_off[k] = off;
-    }
-    __syncthreads();
-    const int total = s_off[9];
+    const int nwave = blockDim.x / kWave;
+    const int tsize = tg.lds_floats();
     const int lane = threadIdx.x & (kWave - 1);
-    const int wave = threadIdx.x / kWave, nwave = blockDim.x / kWave;
+    const int wave = threadIdx.x / kWave;
+    // the first chunk of the brick's slots is loaded alongside its count
+    // (slots below bcap always exist; those past the count are ignored)
+    const int g0 = lane * nwave + wave;
+    float4 w0 = make_float4(0.f, 0.f, 0.f, 0.f);
+    int u0 = 0;
+    if (g0 < bcap) {
+        w0 = bw[(size_t)b * bcap + g0];
+        u0 = bu[(size_t)b * bcap + g0];
+    }
+    const int n_own = min(bcount[b], bcap);
+    const int n_over = bcount[nb];
+    for (int k = threadIdx.x; k < nwave * tsize; k += blockDim.x) acc[k] = 0.f;
+    __syncthreads();
+    float* __restrict__ mine = acc + wave * tsize;
     const int kx = lane >> 4, ky = (lane >> 2) & 3, kz = lane & 3;
+    int loff = kx * tg.sx + ky * tg.sy + kz;
+    // keep the tile-relative base point scalar: without this the compiler
+    // folds the lane offset back into per-lane multiplies
+    asm volatile("" : "+v"(loff));
     const BspCoef Cx(kx), Cy(ky), Cz(kz);
-    for (int base = wave * kWave; base < total; base += nwave * kWave) {
-        const int g = base + lane;
-        float4 wj = make_float4(0.f, 0.f, 0.f, 0.f);
-        int uj = 0;
+    // tile corner: (X0 - 3, Y0 - 3, Z0 - 3)
+    const int sbase = (3 - X0) * tg.sx + (3 - Y0) * tg.sy + (3 - Z0);
+    const int total = n_own + n_over;
+    // atoms dealt to the wavefronts in turn (a brick holds about one
+    // wavefront's worth)
+    for (int base = 0; base < total; base += nwave * kWave) {
+        const int g = base + lane * nwave + wave;
+        float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
+        int u = 0;
         bool hit = false;
-        if (g < total) {
-            int k = 0;
-            while (s_off[k + 1] <= g) ++k;
-            const int j = g - s_off[k];
-            const int d = s_dx[k];
-            if (d < 8) {
-                const int e = s_first[k] + j;
-                wj = bw[e];
-                uj = bu[e];
-                // the bin's own brick coordinates fix the home brick of the atom
-                const int sxyz = s_sx[k];
-                const int ux = uj & 1023, uy = (uj >> 10) & 1023, uz = uj >> 20;
-                hit = (!(d & 4) || ux - (sxyz & 1023) * bx < 3) &&
-                      (!(d & 2) || uy - ((sxyz >> 10) & 1023) * by < 3) &&
-                      (!(d & 1) || uz - (sxyz >> 20) * bz < 3);
+        if (g < n_own) {
+            if (base == 0) {
+                w = w0;
+                u = u0;
             } else {
-                wj = ow[j];
-                uj = ou[j];
-                hit = reaches(uj & 1023, cx, bx, nbx) && reaches((uj >> 10) & 1023, cy, by, nby) &&
-                      reaches(uj >> 20, cz, bz, nbz);
+                w = bw[(size_t)b * bcap + g];
+                u = bu[(size_t)b * bcap + g];
             }
+            hit = w.w != 0.f;  // an uncharged atom adds nothing
+        } else if (g < total) {
+            w = ow[g - n_own];
+            u = ou[g - n_own];
+            hit = w.w != 0.f && brick_of(u & 1023, bx, nbx) == cx &&
+                  brick_of((u >> 10) & 1023, by, nby) == cy && brick_of(u >> 20, bz, nbz) == cz;
         }
         unsigned long long m = __ballot(hit);
         while (m) {
-            const int src_lane = __builtin_ctzll(m);
+            const int src = __builtin_ctzll(m);
             m &= m - 1;
-            const int u = __builtin_amdgcn_readlane(uj, src_lane);
-            const float wfx = lane_f(wj.x, src_lane);
-            const float wfy = lane_f(wj.y, src_lane);
-            const float wfz = lane_f(wj.z, src_lane);
-            const float q = lane_f(wj.w, src_lane);
-            int x = (u & 1023) - kx;
-            if (x < 0) x += fx;
-            int y = ((u >> 10) & 1023) - ky;
-            if (y < 0) y += fy;
-            int z = (u >> 20) - kz;
-            if (z < 0) z += fz;
-            x -= X0;
-            y -= Y0;
-            z -= Z0;
-            if (x >= 0 && x < bx && y >= 0 && y < by && z >= 0 && z < bz) {
-                float tq = q * Cx.w(wfx);
-                tq = tq * Cy.w(wfy);
-                tq = tq * Cz.w(wfz);
-                atomicAdd(&acc[(x * by + y) * bz + z], tq);
-            }
+            const int us = __builtin_amdgcn_readlane(u, src);
+            const int a0 = sbase + (us & 1023) * tg.sx + ((us >> 10) & 1023) * tg.sy + (us >> 20);
+            const float wfx = lane_f(w.x, src);
+            const float wfy = lane_f(w.y, src);
+            const float wfz = lane_f(w.z, src);
+            const float q = lane_f(w.w, src);
+            float tq = q * Cx.w(wfx);
+            tq = tq * Cy.w(wfy);
+            tq = tq * Cz.w(wfz);
+            mine[a0 - loff] += tq;
         }
     }
     __syncthreads();
+    float* __restrict__ out = tiles + (size_t)b * tg.points();
+    const int npts = tg.points();
+    const float inv_tz = __builtin_amdgcn_rcpf((float)tg.tz);
+    const float inv_ty = __builtin_amdgcn_rcpf((float)tg.ty);
     for (int k = threadIdx.x; k < npts; k += blockDim.x) {
-        const int z = k % bz, y = (k / bz) % by, x = k / (bz * by);
-        Q[((size_t)(X0 + x) * fy + (Y0 + y)) * fz + (Z0 + z)] = acc[k];
+        const int r = div_small(k, inv_tz), z = k - r * tg.tz;
+        const int x = div_small(r, inv_ty), y = r - x * tg.ty;
+        const int a = x * tg.sx + y * tg.sy + z;
+        float v = acc[a];
+        for (int w = 1; w < nwave; ++w) v += acc[w * tsize + a];
+        out[k] = v;
+    }
+}
+
+// Grid point (x, y, z) of brick c at local (lx, ly, lz) sums, per axis, its
+// own tile at l + 3 and, for the top 3 points of the brick, the tile of the
+// brick above at l - extent + 3 (the same tile when the axis has one brick).
+__global__ void __launch_bounds__(256) k_pme_tiles_reduce(int fx, int fy, int fz, int bx, int by,
+                                                          int bz, int nbx, int nby, int nbz,
+                                                          TileGeom tg,
+                                                          const float* __restrict__ tiles,
+                                                          float* __restrict__ Q) {
+    const int b = blockIdx.x;
+    const int cz = b % nbz, cy = (b / nbz) % nby, cx = b / (nbz * nby);
+    const int X0 = cx * bx, Y0 = cy * by, Z0 = cz * bz;
+    const int ex = cx == nbx - 1 ? fx - X0 : bx;
+    const int ey = cy == nby - 1 ? fy - Y0 : by;
+    const int ez = cz == nbz - 1 ? fz - Z0 : bz;
+    const int ax = cx + 1 == nbx ? 0 : cx + 1;
+    const int ay = cy + 1 == nby ? 0 : cy + 1;
+    const int az = cz + 1 == nbz ? 0 : cz + 1;
+    const size_t tn = (size_t)tg.points();
+    const int npts = ex * ey * ez;
+    const float inv_ez = __builtin_amdgcn_rcpf((float)ez);
+    const float inv_ey = __builtin_amdgcn_rcpf((float)ey);
+#pragma unroll 2
+    for (int k = threadIdx.x; k < npts; k += blockDim.x) {
+        const int r = div_small(k, inv_ez), lz = k - r * ez;
+        const int lx = div_small(r, inv_ey), ly = r - lx * ey;
+        // the up to 8 covering tiles, loaded independently, summed in a fixed order
+        const bool hx = lx >= ex - 3, hy = ly >= ey - 3, hz = lz >= ez - 3;
+        float t[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const int dx = c >> 2, dy = (c >> 1) & 1, dz = c & 1;
+            t[c] = 0.f;
+            if ((!dx || hx) && (!dy || hy) && (!dz || hz)) {
+                const int bxi = dx ? ax : cx, tx = dx ? lx - ex + 3 : lx + 3;
+                const int byi = dy ? ay : cy, ty = dy ? ly - ey + 3 : ly + 3;
+                const int bzi = dz ? az : cz, tz = dz ? lz - ez + 3 : lz + 3;
+                t[c] = tiles[(size_t)((bxi * nby + byi) * nbz + bzi) * tn + (tx * tg.ty + ty) * tg.tz + tz];
+            }
+        }
+        const float v = ((t[0] + t[1]) + (t[2] + t[3])) + ((t[4] + t[5]) + (t[6] + t[7]));
+        Q[((size_t)(X0 + lx) * fy + (Y0 + ly)) * fz + (Z0 + lz)] = v;
     }
 }
 
@@ -904,13 +961,14 @@ void pme_free(sponge_pme* p) {
     dfree(p->d_bu);
     dfree(p->d_ow);
     dfree(p->d_ou);
+    dfree(p->d_tiles);
 }
 
 // Brick edge on one axis: the divisor of the grid size closest to 8 among
-// 3..16 (the stencil then spans at most two bricks per axis), else the whole
-// axis.
+// 3..16; for a larger size without one, 8 (the last brick then takes the
+// remainder, 9..15 points).
 int brick_edge(int f) {
-    int best = f, score = 1 << 30;
+    int best = f > 16 ? 8 : f, score = 1 << 30;
     for (int d = 3; d <= 16 && d <= f; ++d) {
         if (f % d) continue;
         const int sc = d > 8 ? d - 8 : 8 - d;
@@ -920,6 +978,13 @@ int brick_edge(int f) {
         }
     }
     return best;
+}
+
+// tile geometry of the spread: sized for the largest brick of each axis
+TileGeom pme_tile_geom(const sponge_pme* p) {
+    return tile_geom(std::max(p->bx, p->fx - (p->nbx - 1) * p->bx),
+                     std::max(p->by, p->fy - (p->nby - 1) * p->by),
+                     std::max(p->bz, p->fz - (p->nbz - 1) * p->bz));
 }
 
 }  // namespace
@@ -1047,6 +1112,16 @@ extern "C" int sponge_pme_initial(sponge_pme** out, int atom_numbers, const floa
     e |= dmalloc(&p->d_bu, (size_t)p->nb * p->bcap);
     e |= dmalloc(&p->d_ow, (size_t)p->N);
     e |= dmalloc(&p->d_ou, (size_t)p->N);
+    {
+        // tile copies per spread workgroup: as many as let every brick's
+        // workgroup be resident at once (160 KB of LDS per CU, 256 CUs), 1..4
+        const TileGeom tg = pme_tile_geom(p);
+        const long long per_cu = (p->nb + 255) / 256;
+        const long long bytes = (long long)tg.lds_floats() * (long long)sizeof(float);
+        const long long fit = (160LL * 1024) / (per_cu * bytes);
+        p->spread_waves = (int)std::max(1LL, std::min(4LL, fit));
+        e |= dmalloc(&p->d_tiles, (size_t)p->nb * tg.points());
+    }
     if (e) {
         pme_free(p);
         delete p;
@@ -1169,13 +1244,19 @@ extern "C" int sponge_pme_reciprocal_force(sponge_pme* p, const float* crd, cons
         {
             ScopedTimer timer("pme_spread", s);
             hipLaunchKernelGGL(k_pme_bin, dim3(grid_for(N, 256)), dim3(256), 0, s, N, crd, charge,
-                               r, p->fx, p->fy, p->fz, p->bx, p->by, p->bz, p->nby, p->nbz, p->nb,
-                               p->bcap, p->d_bcount, p->d_bw, p->d_bu, p->d_ow, p->d_ou, p->d_acc);
+                               r, p->fx, p->fy, p->fz, p->bx, p->by, p->bz, p->nbx, p->nby, p->nbz,
+                               p->nb, p->bcap, p->d_bcount, p->d_bw, p->d_bu, p->d_ow, p->d_ou,
+                               p->d_acc);
             SA_LAUNCH_CHECK();
-            const size_t sh = sizeof(float) * p->bx * p->by * p->bz;
-            hipLaunchKernelGGL(k_pme_spread_bricks, dim3(p->nb), dim3(256), sh, s, p->fx, p->fy,
-                               p->fz, p->bx, p->by, p->bz, p->nbx, p->nby, p->nbz, p->nb, p->bcap,
-                               p->d_bcount, p->d_bw, p->d_bu, p->d_ow, p->d_ou, p->d_Q);
+            const TileGeom tg = pme_tile_geom(p);
+            const size_t sh = sizeof(float) * (size_t)p->spread_waves * tg.lds_floats();
+            hipLaunchKernelGGL(k_pme_spread_tiles, dim3(p->nb), dim3(kWave * p->spread_waves), sh,
+                               s, p->bx, p->by, p->bz, p->nbx, p->nby, p->nbz, p->nb, p->bcap, tg,
+                               p->d_bcount, p->d_bw, p->d_bu, p->d_ow, p->d_ou, p->d_tiles);
+            SA_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_pme_tiles_reduce, dim3(p->nb), dim3(256), 0, s, p->fx, p->fy,
+                               p->fz, p->bx, p->by, p->bz, p->nbx, p->nby, p->nbz, tg, p->d_tiles,
+                               p->d_Q);
             SA_LAUNCH_CHECK();
         }
         rc = pme_fft_convolve(p, need_virial);

@@ -195,7 +195,7 @@ def test_pme_reciprocal(dev, cfg1, pme_ref):
     F = d.zeros(nall)
     pm.grids(Q, F)
     torch.cuda.synchronize()
-    # intermediate grids (float atomics / FP32 FFT vs the oracle's serial
+    # intermediate grids (FP32 tile sums / FP32 FFT vs the oracle's serial
     # spread and FP64 FFT)
     assert rel_rms(Q.cpu().numpy(), grids["Q"]) < 1e-5
     assert rel_rms(F.cpu().numpy(), grids["FBCFQ"]) < 1e-5
@@ -239,9 +239,37 @@ def test_pme_spread_dense_bins_and_repeat(dev, cfg1):
     for Q, frc in out:
         assert rel_rms(Q, grids["Q"]) < 1e-5
         assert rel_rms(frc, ref["frc"]) < FORCE_TOL
-    # float atomics: same grid up to summation order (bins left uncleared
-    # would double it)
+    # same grid up to the order of the atoms within a bin, which the binning
+    # atomics decide (bins left uncleared would double it)
     assert rel_rms(out[1][0], out[0][0]) < 1e-5
+
+
+@pytest.mark.parametrize("fft", [(10, 12, 14), (5, 27, 16), (9, 20, 7)])
+def test_pme_tile_shapes(dev, cfg1, fft):
+    """Grids whose bricks span a whole axis (5, 7, 9, 10: the halo of the
+    single tile wraps onto the same brick) or split it into 2 or 3 bricks of
+    6, 7, 8, 9 or 10 points: grid and forces against the oracle."""
+    import torch
+    from oracle import oracle as O
+    from sponge_amd import abi
+    s = cfg1
+    beta = O.get_beta(s.cutoff, s.tolerance)
+    p = O.PME(s, fft, beta)
+    ref = p.reciprocal()
+    grids = p.grids()
+    d = DevSystem(s, dev)
+    pm = abi.ParticleMesh(d.N, abi.Box(s.box), s.cutoff, fftx=fft[0], ffty=fft[1], fftz=fft[2])
+    assert pm.fft == fft
+    frc = d.zeros(d.N, 3)
+    pm.reciprocal_force(d.crd, d.charge, frc, step=0)
+    nall = int(np.prod(fft))
+    Q = d.zeros(nall)
+    F = d.zeros(nall)
+    pm.grids(Q, F)
+    torch.cuda.synchronize()
+    print(f"case: fft {fft} grid rel rms {rel_rms(Q.cpu().numpy(), grids['Q']):.2e}")
+    assert rel_rms(Q.cpu().numpy(), grids["Q"]) < 1e-5
+    assert rel_rms(frc.cpu().numpy(), ref["frc"]) < FORCE_TOL
 
 
 def test_pme_excluded(dev, cfg1):
