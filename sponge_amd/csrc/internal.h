@@ -109,11 +109,14 @@ struct sponge_pme {
     // axis takes the remainder); atoms binned by the brick of their base
     // point, cap per brick, the rest in an overflow list
     int bx = 0, by = 0, bz = 0, nbx = 0, nby = 0, nbz = 0, nb = 0, bcap = 0;
-    int* d_bcount = nullptr;      // [nb + 1]: atoms per brick, [nb] overflow count
+    int* d_bcount = nullptr;      // [nb + 1]: atoms per brick, [nb] overflow count (zero between evaluations)
     float4* d_bw = nullptr;       // [nb][bcap] fractions x, y, z and charge
     int* d_bu = nullptr;          // [nb][bcap] packed base point ux | uy << 10 | uz << 20
     float4* d_ow = nullptr;       // [N] overflow entries
     int* d_ou = nullptr;
+    int* d_bi = nullptr;          // [nb][bcap] atom index of each binned atom
+    int* d_oi = nullptr;          // [N] atom index of each overflow entry
+    int* d_gcount = nullptr;      // [nb + 1] the counts as the gather reads them
     float* d_tiles = nullptr;     // [nb][tile points] halo tiles of the spread
     int spread_waves = 1;         // wavefronts (tile copies) per spread workgroup
 };

@@ -272,12 +272,9 @@ __global__ void __launch_bounds__(512) k_nl_search_cells(
             wi[u] = sexw[p];
             cnt[u] = 0;
         }
-        for (int base = 0; base < total; base += kWave) {
-            const int t = base + lane;
-            const bool in = t < total;
-            float4 pj4 = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
-            if (t < staged) pj4 = cand[t];
-            else if (in) pj4 = spos[stream_to_sorted(t, nr, r_first, r_off)];
+        // one pass of 64 candidates (stream positions t, this lane's at pj4)
+        // against the wavefront's rows
+        auto pass = [&](int t, bool in, float4 pj4) {
             const float3 pj = make_float3(pj4.x, pj4.y, pj4.z);
             const int gj = __float_as_int(pj4.w);
 #pragma unroll
@@ -295,6 +292,23 @@ __global__ void __launch_bounds__(512) k_nl_search_cells(
                 }
                 cnt[u] += __popcll(m);
             }
+        };
+        // staged candidates: LDS only, so no pass waits on the list stores
+        // in flight (the vector memory counter also counts stores)
+        for (int base = 0; base < staged; base += kWave) {
+            const int t = base + lane;
+            const bool in = t < staged;
+            float4 pj4 = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
+            if (in) pj4 = cand[t];
+            pass(t, in, pj4);
+        }
+        // the rest of a stream longer than the stage, from global memory
+        for (int base = staged / kWave * kWave; base < total; base += kWave) {
+            const int t = base + lane;
+            const bool in = t >= staged && t < total;
+            float4 pj4 = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
+            if (in) pj4 = spos[stream_to_sorted(t, nr, r_first, r_off)];
+            pass(t, in, pj4);
         }
         if (lane == 0) {
 #pragma unroll

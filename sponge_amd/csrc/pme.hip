@@ -36,9 +36,6 @@ __constant__ float cMa[4] = {(float)(1.0 / 6.0), -0.5f, 0.5f, (float)(-1.0 / 6.0
 __constant__ float cMb[4] = {0.f, 0.5f, -1.f, 0.5f};
 __constant__ float cMc[4] = {0.f, 0.5f, 0.f, -0.5f};
 __constant__ float cMd[4] = {0.f, (float)(1.0 / 6.0), (float)(4.0 / 6.0), (float)(1.0 / 6.0)};
-__constant__ float cdMa[4] = {0.5f, -1.5f, 1.5f, -0.5f};
-__constant__ float cdMb[4] = {0.f, 1.f, -2.f, 1.f};
-__constant__ float cdMc[4] = {0.f, 0.5f, 0.f, -0.5f};
 
 // ---------------------------------------------------------------------------
 // complex helpers
@@ -398,24 +395,10 @@ __device__ __forceinline__ Near atom_near(const float* __restrict__ crd, int i, 
 // do not reload the coefficient tables.
 struct BspCoef {
     float a, b, c, d;     // M: ((a f + b) f + c) f + d
-    float da, db, dc;     // dM: (da f + db) f + dc
-    __device__ __forceinline__ explicit BspCoef(int k)
-        : a(cMa[k]), b(cMb[k]), c(cMc[k]), d(cMd[k]), da(cdMa[k]), db(cdMb[k]), dc(cdMc[k]) {}
+    __device__ __forceinline__ explicit BspCoef(int k) : a(cMa[k]), b(cMb[k]), c(cMc[k]), d(cMd[k]) {}
     __device__ __forceinline__ float w(float f) const { return fmaf(fmaf(fmaf(a, f, b), f, c), f, d); }
-    __device__ __forceinline__ float dw(float f) const { return fmaf(fmaf(da, f, db), f, dc); }
 };
 
-// grid index of stencil point (kx, ky, kz) below u (PM_force.cpp:737-759)
-__device__ __forceinline__ int stencil_index(const Near& nr, int kx, int ky, int kz, int fx,
-                                             int fy, int fz) {
-    int x = nr.ux - kx;
-    if (x < 0) x += fx;
-    int y = nr.uy - ky;
-    if (y < 0) y += fy;
-    int z = nr.uz - kz;
-    if (z < 0) z += fz;
-    return (x * fy + y) * fz + z;
-}
 
 // Tile spread.  The charge grid is cut into bricks of Bx x By x Bz points (the
 // last brick of an axis takes the remainder, Bx..2Bx-1 points).  An atom
@@ -448,8 +431,9 @@ __global__ void __launch_bounds__(256) k_pme_bin(int N, const float* __restrict_
                                                  int fy, int fz, int bx, int by, int bz, int nbx,
                                                  int nby, int nbz, int nb, int bcap,
                                                  int* __restrict__ bcount, float4* __restrict__ bw,
-                                                 int* __restrict__ bu, float4* __restrict__ ow,
-                                                 int* __restrict__ ou, double* __restrict__ acc) {
+                                                 int* __restrict__ bu, int* __restrict__ bi,
+                                                 float4* __restrict__ ow, int* __restrict__ ou,
+                                                 int* __restrict__ oi, double* __restrict__ acc) {
     __shared__ int lcount[kBinLds];
     __shared__ int lbase[kBinLds];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -491,10 +475,12 @@ __global__ void __launch_bounds__(256) k_pme_bin(int N, const float* __restrict_
     if (slot < bcap) {
         bw[(size_t)h * bcap + slot] = w;
         bu[(size_t)h * bcap + slot] = u;
+        bi[(size_t)h * bcap + slot] = i;
     } else {
         const int o = atomicAdd(&bcount[nb], 1);
         ow[o] = w;
         ou[o] = u;
+        oi[o] = i;
     }
 }
 
@@ -632,12 +618,24 @@ __global__ void __launch_bounds__(256) k_pme_spread_tiles(
 // Grid point (x, y, z) of brick c at local (lx, ly, lz) sums, per axis, its
 // own tile at l + 3 and, for the top 3 points of the brick, the tile of the
 // brick above at l - extent + 3 (the same tile when the axis has one brick).
+// It also hands the bin counts to the gather (gcount) and clears them for the
+// next evaluation's binning (the spread, their other reader, has finished).
 __global__ void __launch_bounds__(256) k_pme_tiles_reduce(int fx, int fy, int fz, int bx, int by,
                                                           int bz, int nbx, int nby, int nbz,
-                                                          TileGeom tg,
+                                                          int nb, TileGeom tg,
                                                           const float* __restrict__ tiles,
-                                                          float* __restrict__ Q) {
+                                                          float* __restrict__ Q,
+                                                          int* __restrict__ bcount,
+                                                          int* __restrict__ gcount) {
     const int b = blockIdx.x;
+    if (threadIdx.x == 0) {
+        gcount[b] = bcount[b];
+        bcount[b] = 0;
+        if (b == 0) {
+            gcount[nb] = bcount[nb];
+            bcount[nb] = 0;
+        }
+    }
     const int cz = b % nbz, cy = (b / nbz) % nby, cx = b / (nbz * nby);
     const int X0 = cx * bx, Y0 = cy * by, Z0 = cz * bz;
     const int ex = cx == nbx - 1 ? fx - X0 : bx;
@@ -656,58 +654,156 @@ __global__ void __launch_bounds__(256) k_pme_tiles_reduce(int fx, int fy, int fz
         const int lx = div_small(r, inv_ey), ly = r - lx * ey;
         // the up to 8 covering tiles, loaded independently, summed in a fixed order
         const bool hx = lx >= ex - 3, hy = ly >= ey - 3, hz = lz >= ez - 3;
+        // (an absent tile reads the own tile's entry and adds zero, so the
+        // loads stay unconditional and all in flight together)
         float t[8];
+        size_t at[8];
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
             const int dx = c >> 2, dy = (c >> 1) & 1, dz = c & 1;
-            t[c] = 0.f;
-            if ((!dx || hx) && (!dy || hy) && (!dz || hz)) {
-                const int bxi = dx ? ax : cx, tx = dx ? lx - ex + 3 : lx + 3;
-                const int byi = dy ? ay : cy, ty = dy ? ly - ey + 3 : ly + 3;
-                const int bzi = dz ? az : cz, tz = dz ? lz - ez + 3 : lz + 3;
-                t[c] = tiles[(size_t)((bxi * nby + byi) * nbz + bzi) * tn + (tx * tg.ty + ty) * tg.tz + tz];
-            }
+            const int bxi = dx ? ax : cx, tx = dx ? lx - ex + 3 : lx + 3;
+            const int byi = dy ? ay : cy, ty = dy ? ly - ey + 3 : ly + 3;
+            const int bzi = dz ? az : cz, tz = dz ? lz - ez + 3 : lz + 3;
+            at[c] = (size_t)((bxi * nby + byi) * nbz + bzi) * tn + (tx * tg.ty + ty) * tg.tz + tz;
+        }
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const int dx = c >> 2, dy = (c >> 1) & 1, dz = c & 1;
+            const bool ok = (!dx || hx) && (!dy || hy) && (!dz || hz);
+            const float v = tiles[ok ? at[c] : at[0]];
+            t[c] = ok ? v : 0.f;
         }
         const float v = ((t[0] + t[1]) + (t[2] + t[3])) + ((t[4] + t[5]) + (t[6] + t[7]));
         Q[((size_t)(X0 + lx) * fy + (Y0 + ly)) * fz + (Z0 + lz)] = v;
     }
 }
 
+// B-spline weight polynomials M_k and their derivatives dM_k as compile-time
+// constants (M_k the same as cMa..cMd) for the thread-per-atom gather
+struct BspK {
+    static constexpr float a[4] = {(float)(1.0 / 6.0), -0.5f, 0.5f, (float)(-1.0 / 6.0)};
+    static constexpr float b[4] = {0.f, 0.5f, -1.f, 0.5f};
+    static constexpr float c[4] = {0.f, 0.5f, 0.f, -0.5f};
+    static constexpr float d[4] = {0.f, (float)(1.0 / 6.0), (float)(4.0 / 6.0), (float)(1.0 / 6.0)};
+    static constexpr float da[4] = {0.5f, -1.5f, 1.5f, -0.5f};
+    static constexpr float db[4] = {0.f, 1.f, -2.f, 1.f};
+    static constexpr float dc[4] = {0.f, 0.5f, 0.f, -0.5f};
+    __device__ static __forceinline__ float w(int k, float f) {
+        return fmaf(fmaf(fmaf(a[k], f, b[k]), f, c[k]), f, d[k]);
+    }
+    __device__ static __forceinline__ float dw(int k, float f) {
+        return fmaf(fmaf(da[k], f, db[k]), f, dc[k]);
+    }
+};
+
 // PME_Final + device_add_force: frc += update_interval * F (PM_force.cpp:
-// 852-892, 1014-1022), one wavefront per atom.
-__global__ void __launch_bounds__(256) k_pme_gather(int N, const float* __restrict__ crd,
-                                                    const float* __restrict__ charge, LT6 rc,
-                                                    int fx, int fy, int fz,
-                                                    const float* __restrict__ phi,
-                                                    float update_interval,
-                                                    float* __restrict__ frc,
-                                                    int* __restrict__ bcount, int nb) {
-    // the spread of this evaluation is complete: clear its bins for the next
-    if (blockIdx.x == 0)
-        for (int k = threadIdx.x; k <= nb; k += blockDim.x) bcount[k] = 0;
-    const int lane = threadIdx.x & (kWave - 1);
-    const int kx = lane >> 4, ky = (lane >> 2) & 3, kz = lane & 3;
-    const BspCoef Cx(kx), Cy(ky), Cz(kz);
-    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-    const int nwaves = gridDim.x * blockDim.x / kWave;
-    for (int i = wave; i < N; i += nwaves) {
-        const Near nr = atom_near(crd, i, rc, fx, fy, fz);
-        const float dqf = -phi[stencil_index(nr, kx, ky, kz, fx, fy, fz)] * charge[i];
-        const float wx = Cx.w(nr.fx), wy = Cy.w(nr.fy), wz = Cz.w(nr.fz);
-        const float dx = Cx.dw(nr.fx), dy = Cy.dw(nr.fy), dz = Cz.dw(nr.fz);
-        const float gx = dx * wy * wz * fx, gy = dy * wx * wz * fy, gz = dz * wx * wy * fz;
-        // MultiplyTranspose(g, rcell) = {gx a11, gx a21 + gy a22, gx a31 + gy a32 + gz a33}
-        float ox = dqf * (gx * rc.a11);
-        float oy = dqf * (gx * rc.a21 + gy * rc.a22);
-        float oz = dqf * (gx * rc.a31 + gy * rc.a32 + gz * rc.a33);
-        ox = wave_sum(ox);
-        oy = wave_sum(oy);
-        oz = wave_sum(oz);
-        if (lane == 0) {
-            frc[3 * i] = frc[3 * i] + update_interval * ox;
-            frc[3 * i + 1] = frc[3 * i + 1] + update_interval * oy;
-            frc[3 * i + 2] = frc[3 * i + 2] + update_interval * oz;
+// 852-892, 1014-1022).  One workgroup per brick: the brick's halo tile of the
+// convolved grid (the points the stencils of its atoms reach) is read into
+// LDS with coalesced loads, then each thread takes one of the brick's atoms, as
+// binned by the spread: its 4 x 4 x 4 stencil is read from the tile as 16 rows
+// of 4 points and contracted with the weights in registers (per row the z sums
+// with M_z and dM_z, then the x / y factors).
+constexpr int kGatherThreads = 256;
+constexpr int kGatherLoads = 8;
+
+__global__ void __launch_bounds__(kGatherThreads) k_pme_gather_tiles(
+    int fx, int fy, int fz, int bx, int by, int bz, int nbx, int nby, int nbz, int nb, int bcap,
+    TileGeom tg, const int* __restrict__ gcount, const float4* __restrict__ bw,
+    const int* __restrict__ bu, const int* __restrict__ bi, const float4* __restrict__ ow,
+    const int* __restrict__ ou, const int* __restrict__ oi, const float* __restrict__ phi, LT6 rc,
+    float update_interval, float* __restrict__ frc) {
+    extern __shared__ float tile[];
+    const int b = blockIdx.x;
+    const int cz = b % nbz, cy = (b / nbz) % nby, cx = b / (nbz * nby);
+    const int X0 = cx * bx, Y0 = cy * by, Z0 = cz * bz;
+    const int TX = (cx == nbx - 1 ? fx - X0 : bx) + 3;
+    const int TY = (cy == nby - 1 ? fy - Y0 : by) + 3;
+    const int TZ = (cz == nbz - 1 ? fz - Z0 : bz) + 3;
+    const int npts = TX * TY * TZ;
+    const float inv_tz = __builtin_amdgcn_rcpf((float)TZ);
+    const float inv_ty = __builtin_amdgcn_rcpf((float)TY);
+    // batches of kGatherLoads loads per thread in flight before their LDS writes
+    for (int k0 = threadIdx.x; k0 < npts; k0 += blockDim.x * kGatherLoads) {
+        float v[kGatherLoads];
+        int at[kGatherLoads];
+#pragma unroll
+        for (int j = 0; j < kGatherLoads; ++j) {
+            // points past the tile repeat its last one (loaded, not stored),
+            // so the loads stay unconditional and all in flight together
+            const int kk = k0 + j * (int)blockDim.x;
+            const int k = kk < npts ? kk : npts - 1;
+            const int r = div_small(k, inv_tz), tz = k - r * TZ;
+            const int tx = div_small(r, inv_ty), ty = r - tx * TY;
+            int gx = X0 - 3 + tx, gy = Y0 - 3 + ty, gz = Z0 - 3 + tz;
+            gx += gx < 0 ? fx : 0;
+            gy += gy < 0 ? fy : 0;
+            gz += gz < 0 ? fz : 0;
+            v[j] = phi[((size_t)gx * fy + gy) * fz + gz];
+            at[j] = kk < npts ? tx * tg.sx + ty * tg.sy + tz : -1;
         }
+#pragma unroll
+        for (int j = 0; j < kGatherLoads; ++j)
+            if (at[j] >= 0) tile[at[j]] = v[j];
+    }
+    __syncthreads();
+    const int n_own = min(gcount[b], bcap);
+    const int total = n_own + gcount[nb];
+    for (int g = threadIdx.x; g < total; g += blockDim.x) {
+        float4 w;
+        int u, i;
+        if (g < n_own) {
+            w = bw[(size_t)b * bcap + g];
+            u = bu[(size_t)b * bcap + g];
+            i = bi[(size_t)b * bcap + g];
+        } else {
+            w = ow[g - n_own];
+            u = ou[g - n_own];
+            i = oi[g - n_own];
+            if (brick_of(u & 1023, bx, nbx) != cx || brick_of((u >> 10) & 1023, by, nby) != cy ||
+                brick_of(u >> 20, bz, nbz) != cz)
+                continue;
+        }
+        // base point in the tile: stencil point k lies at t - k
+        const int rx = (u & 1023) - X0 + 3, ry = ((u >> 10) & 1023) - Y0 + 3, rz = (u >> 20) - Z0 + 3;
+        float wx[4], wy[4], wz[4], dx[4], dy[4], dz[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            wx[k] = BspK::w(k, w.x);
+            wy[k] = BspK::w(k, w.y);
+            wz[k] = BspK::w(k, w.z);
+            dx[k] = BspK::dw(k, w.x);
+            dy[k] = BspK::dw(k, w.y);
+            dz[k] = BspK::dw(k, w.z);
+        }
+        float gx = 0.f, gy = 0.f, gz = 0.f;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const float* row = tile + (rx - a) * tg.sx + (ry - c) * tg.sy + rz;
+                float sw = 0.f, sd = 0.f;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const float v = row[-k];
+                    sw = fmaf(v, wz[k], sw);
+                    sd = fmaf(v, dz[k], sd);
+                }
+                gx = fmaf(dx[a] * wy[c], sw, gx);
+                gy = fmaf(wx[a] * dy[c], sw, gy);
+                gz = fmaf(wx[a] * wy[c], sd, gz);
+            }
+        }
+        gx *= (float)fx;
+        gy *= (float)fy;
+        gz *= (float)fz;
+        const float dq = -w.w;
+        // MultiplyTranspose(g, rcell) = {gx a11, gx a21 + gy a22, gx a31 + gy a32 + gz a33}
+        const float ox = dq * (gx * rc.a11);
+        const float oy = dq * (gx * rc.a21 + gy * rc.a22);
+        const float oz = dq * (gx * rc.a31 + gy * rc.a32 + gz * rc.a33);
+        frc[3 * i] = frc[3 * i] + update_interval * ox;
+        frc[3 * i + 1] = frc[3 * i + 1] + update_interval * oy;
+        frc[3 * i + 2] = frc[3 * i + 2] + update_interval * oz;
     }
 }
 
@@ -962,6 +1058,9 @@ void pme_free(sponge_pme* p) {
     dfree(p->d_ow);
     dfree(p->d_ou);
     dfree(p->d_tiles);
+    dfree(p->d_bi);
+    dfree(p->d_oi);
+    dfree(p->d_gcount);
 }
 
 // Brick edge on one axis: the divisor of the grid size closest to 8 among
@@ -1112,6 +1211,9 @@ extern "C" int sponge_pme_initial(sponge_pme** out, int atom_numbers, const floa
     e |= dmalloc(&p->d_bu, (size_t)p->nb * p->bcap);
     e |= dmalloc(&p->d_ow, (size_t)p->N);
     e |= dmalloc(&p->d_ou, (size_t)p->N);
+    e |= dmalloc(&p->d_bi, (size_t)p->nb * p->bcap);
+    e |= dmalloc(&p->d_oi, (size_t)p->N);
+    e |= dmalloc(&p->d_gcount, (size_t)p->nb + 1);
     {
         // tile copies per spread workgroup: as many as let every brick's
         // workgroup be resident at once (160 KB of LDS per CU, 256 CUs), 1..4
@@ -1239,14 +1341,12 @@ extern "C" int sponge_pme_reciprocal_force(sponge_pme* p, const float* crd, cons
     const int N = p->N;
     int rc;
     if (step % p->opt.update_interval == 0) {
-        int blocks = grid_for(N, 4);
-        if (blocks > 16384) blocks = 16384;
         {
             ScopedTimer timer("pme_spread", s);
             hipLaunchKernelGGL(k_pme_bin, dim3(grid_for(N, 256)), dim3(256), 0, s, N, crd, charge,
                                r, p->fx, p->fy, p->fz, p->bx, p->by, p->bz, p->nbx, p->nby, p->nbz,
-                               p->nb, p->bcap, p->d_bcount, p->d_bw, p->d_bu, p->d_ow, p->d_ou,
-                               p->d_acc);
+                               p->nb, p->bcap, p->d_bcount, p->d_bw, p->d_bu, p->d_bi, p->d_ow,
+                               p->d_ou, p->d_oi, p->d_acc);
             SA_LAUNCH_CHECK();
             const TileGeom tg = pme_tile_geom(p);
             const size_t sh = sizeof(float) * (size_t)p->spread_waves * tg.lds_floats();
@@ -1255,8 +1355,8 @@ extern "C" int sponge_pme_reciprocal_force(sponge_pme* p, const float* crd, cons
                                p->d_bcount, p->d_bw, p->d_bu, p->d_ow, p->d_ou, p->d_tiles);
             SA_LAUNCH_CHECK();
             hipLaunchKernelGGL(k_pme_tiles_reduce, dim3(p->nb), dim3(256), 0, s, p->fx, p->fy,
-                               p->fz, p->bx, p->by, p->bz, p->nbx, p->nby, p->nbz, tg, p->d_tiles,
-                               p->d_Q);
+                               p->fz, p->bx, p->by, p->bz, p->nbx, p->nby, p->nbz, p->nb, tg,
+                               p->d_tiles, p->d_Q, p->d_bcount, p->d_gcount);
             SA_LAUNCH_CHECK();
         }
         rc = pme_fft_convolve(p, need_virial);
@@ -1267,9 +1367,12 @@ extern "C" int sponge_pme_reciprocal_force(sponge_pme* p, const float* crd, cons
         }
         {
             ScopedTimer timer("pme_gather", s);
-            hipLaunchKernelGGL(k_pme_gather, dim3(blocks), dim3(256), 0, s, N, crd, charge, r,
-                               p->fx, p->fy, p->fz, p->d_FBCFQ, (float)p->opt.update_interval,
-                               frc, p->d_bcount, p->nb);
+            const TileGeom tg = pme_tile_geom(p);
+            hipLaunchKernelGGL(k_pme_gather_tiles, dim3(p->nb), dim3(kGatherThreads),
+                               sizeof(float) * tg.lds_floats(), s, p->fx, p->fy, p->fz, p->bx,
+                               p->by, p->bz, p->nbx, p->nby, p->nbz, p->nb, p->bcap, tg,
+                               p->d_gcount, p->d_bw, p->d_bu, p->d_bi, p->d_ow, p->d_ou, p->d_oi,
+                               p->d_FBCFQ, r, (float)p->opt.update_interval, frc);
             SA_LAUNCH_CHECK();
         }
     }

@@ -54,6 +54,59 @@ def test_neighbor_list_bit_exact(dev, cfg1, variant):
     assert n == len(want) and dims == tuple(O.nl_grid_dims(s.cell(), s.cutoff, s.skin))
 
 
+@pytest.mark.parametrize("variant", ["lattice", "perturbed"])
+def test_neighbor_list_bit_exact_large_box(dev, variant):
+    """24,000 atoms in a 62.1 A box (10 cells per axis, some atoms given as
+    images a box length away): the set equals the brute-force reference set."""
+    from oracle import oracle as O
+    from sponge_amd import abi
+    from sponge_amd import system as S
+    s = S.config(2)
+    crd = s.crd if variant == "lattice" else _perturbed(s, 5, 0.3)
+    d = DevSystem(s, dev, crd)
+    nl = abi.NeighborList(d.N, s.cutoff, s.skin, abi.Box(s.box))
+    nl.update(d.crd, step=0, update=abi.NL_FORCED, excl=d.excl)
+    got, _, _ = export_pairs(nl, d.N, dev)
+    cb, sb, total = O.nl_bruteforce(s, crd=crd)
+    want = O.pair_array(cb, sb)
+    print(f"case: {variant} {len(want)} pairs")
+    assert total == len(want)
+    assert np.array_equal(got, want), (len(got), len(want))
+
+
+def test_neighbor_list_threshold_edges_far_images(dev):
+    """The threshold cases in a 60 A box with atoms given as images several
+    box lengths away: r^2 == (rc+skin)^2 is out, one ulp inside is in, across
+    a face, exclusions honoured."""
+    from oracle import oracle as O
+    from sponge_amd import abi
+    from sponge_amd import system as S
+    L = 60.0
+    inside = float(np.nextafter(np.float32(13.0), np.float32(0)))
+    crd = np.array([[1.0 + 2 * L, 1.0, 1.0], [13.0, 1.0, 1.0],          # r = 12 exactly: out
+                    [1.0, 1.0, 31.0], [inside, 1.0 - L, 31.0],          # one ulp inside: in
+                    [59.5, 31.0, 1.0 + 3 * L], [0.25, 31.0, 1.0],       # across the face
+                    [5.0, 31.0, 31.0], [6.0, 31.0, 31.0 - 2 * L]],      # excluded pair
+                   np.float32)
+    N = len(crd)
+    s = S.System(name="edge60", box=np.array([L] * 3, np.float32), crd=crd,
+                 vel=np.zeros_like(crd), charge=np.zeros(N, np.float32),
+                 mass=np.ones(N, np.float32), lj_type=np.zeros(N, np.int32),
+                 lj_A=np.zeros(1, np.float32), lj_B=np.zeros(1, np.float32),
+                 excl_start=np.zeros(N, np.int32), excl_num=np.array([0] * 6 + [1, 0], np.int32),
+                 excl_list=np.array([7], np.int32), n_waters=0, cutoff=10.0, skin=2.0)
+    s.excl_start[7] = 1
+    d = DevSystem(s, dev)
+    nl = abi.NeighborList(N, s.cutoff, s.skin, abi.Box(s.box))
+    nl.update(d.crd, update=abi.NL_FORCED, excl=d.excl)
+    got, c, _ = export_pairs(nl, N, dev)
+    cb, sb, _ = O.nl_bruteforce(s)
+    want = O.pair_array(cb, sb)
+    print(f"case: pairs {sorted((int(k // N), int(k % N)) for k in got)}")
+    assert np.array_equal(got, want)
+    assert {(int(k // N), int(k % N)) for k in got} == {(2, 3), (4, 5)}
+
+
 def test_neighbor_list_threshold_edges(dev):
     """Pairs exactly at r^2 == (rc+skin)^2 are excluded (strict <), one ulp
     inside included, across a periodic face, with exclusions honoured."""
