@@ -475,6 +475,17 @@ int sponge_md_handles(sponge_md* md, sponge_nl** nl, sponge_lj** lj, sponge_pme*
  * tiles[3] (host, may be NULL) gets the choice.  Synchronises. */
 int sponge_md_autotune(sponge_md* md, int reps, int* tiles);
 
+/* Whether a plain PME step runs its reciprocal half (bin, spread, FFT
+ * convolution) on a second stream beside the list and pair kernels, joined
+ * before the gather (no counterpart in the reference, whose Main_Calculate_
+ * Force, SPONGE/main.cpp, evaluates the terms one after another): mode 1 on,
+ * 0 off, -1 automatic (on from SPONGE_AMD_MD_OVERLAP_MIN_ATOMS atoms, where
+ * the fork and join cost less than the overlap saves).  The captured step is
+ * rebuilt.  The forces are the same either way up to the order of the float
+ * additions into the force array. */
+#define SPONGE_AMD_MD_OVERLAP_MIN_ATOMS 12000
+int sponge_md_set_reciprocal_overlap(sponge_md* md, int mode);
+
 /* Learned multiple-time-step schedule (no counterpart in the reference; the
  * reference's own way to skip reciprocal steps is PME update_interval,
  * PM_force.cpp:1082, 1135-1140): with rsur set, the steps with step % period

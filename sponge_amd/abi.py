@@ -98,6 +98,7 @@ SIGNATURES = {
                                _FP, _F, _F, _F, C.POINTER(NLOptions), C.POINTER(PMEOptions), _P]),
     "sponge_md_run": (_I, [_P, _I, _I, _I]),
     "sponge_md_autotune": (_I, [_P, _I, C.POINTER(_I)]),
+    "sponge_md_set_reciprocal_overlap": (_I, [_P, _I]),
     "sponge_md_set_surrogate": (_I, [_P, _P, _I, _I, _F]),
     "sponge_md_surrogate_stats": (_I, [_P, C.POINTER(C.c_longlong), C.POINTER(C.c_double)]),
     "sponge_md_collect_frames": (_I, [_P, _P, _I, _I]),
@@ -654,6 +655,12 @@ class MDSystem:
         t = (C.c_int * 3)()
         _check(self.lib.sponge_md_autotune(self.h, int(reps), t), "sponge_md_autotune")
         return tuple(t)
+
+    def set_reciprocal_overlap(self, mode):
+        """1: the reciprocal half of a plain PME step on a second stream
+        beside the pair work; 0: one stream; -1: from the atom count."""
+        _check(self.lib.sponge_md_set_reciprocal_overlap(self.h, int(mode)),
+               "sponge_md_set_reciprocal_overlap")
 
     def set_surrogate(self, rsur, period=4, check_interval=8, bound=0.05):
         _check(self.lib.sponge_md_set_surrogate(self.h, rsur.h if rsur is not None else None,

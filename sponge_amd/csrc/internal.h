@@ -174,6 +174,15 @@ int rsur_features_sorted(sponge_rsur* h, sponge_nl* nl, const float4* pos, const
                          const int* ex_start, const int* ex_list, const int* ex_num);
 int rsur_predict_sorted(sponge_rsur* h, float* frc, int assign, const float* anchor);
 void rsur_set_stream(sponge_rsur* h, hipStream_t s);
+// the reciprocal force in two halves (spread + FFT convolution; gather +
+// energy) on the module's stream, and that stream
+int pme_recip_begin(sponge_pme* p, const float* crd, const float* cell, const float* rcell,
+                    const float* charge, int need_virial, int need_energy, int step);
+int pme_recip_end(sponge_pme* p, const float* rcell, const float* charge, float* frc,
+                  int need_virial, int need_energy, float* d_virial, float* d_potential, int step);
+hipStream_t pme_stream(const sponge_pme* p);
+bool pme_calculates_reciprocal(const sponge_pme* p);
+void pme_set_stream(sponge_pme* p, hipStream_t s);
 int rsur_atoms(const sponge_rsur* h);
 float rsur_cutoff(const sponge_rsur* h);
 

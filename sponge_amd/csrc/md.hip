@@ -73,6 +73,11 @@ struct sponge_md {
     unsigned long long* d_scount = nullptr;  // [0] checks, [1] fallbacks
     int use_graph = 0;
     int own_stream = 0;  // the driver created its stream (graph capture needs one)
+    // the reciprocal half (spread + FFT convolution) runs on a stream of its
+    // own beside the list and pair work of the step, joined before the gather
+    hipStream_t recip_stream = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    int overlap_mode = -1;  // 1 on, 0 off, -1 from the atom count
 };
 
 namespace {
@@ -337,6 +342,11 @@ void md_free(sponge_md* m) {
     dfree(m->d_scount);
     if (m->own_stream && m->stream) (void)hipStreamDestroy(m->stream);
     m->stream = nullptr;
+    if (m->recip_stream) (void)hipStreamDestroy(m->recip_stream);
+    if (m->ev_fork) (void)hipEventDestroy(m->ev_fork);
+    if (m->ev_join) (void)hipEventDestroy(m->ev_join);
+    m->recip_stream = nullptr;
+    m->ev_fork = m->ev_join = nullptr;
 }
 
 // FP64 sums of a VECTOR array's components into out[0..2]
@@ -384,8 +394,10 @@ int schedule_cycle(const sponge_md* m) {
     return m->sur_check > 0 ? m->sur_period * m->sur_check : m->sur_period;
 }
 
-// One MD step (force evaluation + integration), all on m->stream.
-int md_step(sponge_md* m, long long step, int need_energy, StepKind* kind_out = nullptr) {
+// One MD step (force evaluation + integration) on m->stream (with the
+// reciprocal half of a plain PME step forked onto m->recip_stream).
+int md_step(sponge_md* m, long long step, int need_energy, StepKind* kind_out = nullptr,
+            bool allow_fork = true) {
     hipStream_t s = m->stream;
     const int N = m->N;
     int rc = 0;
@@ -396,6 +408,28 @@ int md_step(sponge_md* m, long long step, int need_energy, StepKind* kind_out = 
     if (rc) return rc;
     const float* cell = &m->cell.a11;
     const float* rcell = &m->rcell.a11;
+    const StepKind kind = step_kind(m, step, need_energy);
+    if (kind_out) *kind_out = kind;
+    // plain PME step: the reciprocal half that reads only the coordinates is
+    // forked onto its own stream at the start of the step (its kernels are
+    // latency-bound and fill the GPU beside the list and pair kernels).
+    // Timers cut captures on the main stream, so with timers on the step
+    // stays on one stream; so does a step recorded into the batched driver's
+    // capture, where m->stream is itself a stream forked from the capture's
+    // origin (the fork is made only from an origin or an eager stream).
+    const bool wanted = m->overlap_mode > 0 ||
+                        (m->overlap_mode < 0 && m->N >= SPONGE_AMD_MD_OVERLAP_MIN_ATOMS);
+    const bool overlap = wanted && allow_fork && kind == StepKind::Pme && !m->rsur &&
+                         !m->collect && !g_timing && pme_calculates_reciprocal(m->pme);
+    if (overlap) {
+        SA_CHECK(hipEventRecord(m->ev_fork, s));
+        SA_CHECK(hipStreamWaitEvent(m->recip_stream, m->ev_fork, 0));
+        pme_set_stream(m->pme, m->recip_stream);
+        rc = pme_recip_begin(m->pme, m->d_crd, cell, rcell, m->d_charge, 0, need_energy, (int)step);
+        pme_set_stream(m->pme, s);
+        if (rc) return rc;
+        SA_CHECK(hipEventRecord(m->ev_join, m->recip_stream));
+    }
     rc = nl_update_impl(m->nl, nullptr, N, 0, m->d_crd, cell, rcell, (int)step,
                         SPONGE_AMD_NL_CONDITIONAL, m->d_excl_start, m->d_excl_list, m->d_excl_num,
                         m->need_cleared);
@@ -417,10 +451,13 @@ int md_step(sponge_md* m, long long step, int need_energy, StepKind* kind_out = 
         rc = sponge_lj_long_range_correction(m->lj, 0, nullptr, 1, m->d_potential, m->volume);
         if (rc) return rc;
     }
-    const StepKind kind = step_kind(m, step, need_energy);
-    if (kind_out) *kind_out = kind;
     const int N3 = 3 * N;
-    if (kind == StepKind::Pme && m->rsur && !m->collect) {
+    if (overlap) {
+        SA_CHECK(hipStreamWaitEvent(s, m->ev_join, 0));
+        rc = pme_recip_end(m->pme, rcell, m->d_charge, m->d_frc, 0, need_energy, nullptr,
+                           need_energy ? m->d_potential : nullptr, (int)step);
+        if (rc) return rc;
+    } else if (kind == StepKind::Pme && m->rsur && !m->collect) {
         // exact step of the learned schedule: the anchor of the next steps
         rc = fill_async(m->d_rec, 0.f, N3, s);
         if (!rc)
@@ -528,6 +565,14 @@ extern "C" int sponge_md_initial(sponge_md** out, int atom_numbers, const float*
         m->own_stream = 1;
     }
     stream = m->stream;
+    // default priority, like every stream of this library
+    if (hipStreamCreateWithFlags(&m->recip_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&m->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&m->ev_join, hipEventDisableTiming) != hipSuccess) {
+        md_free(m);
+        delete m;
+        return SPONGE_AMD_ERROR_MALLOC_FAILED;
+    }
     const float one = 1.0f;
     m->cell = {box_length[0], 0.f, box_length[1], 0.f, 0.f, box_length[2]};
     m->rcell = {one / box_length[0], 0.f, one / box_length[1], 0.f, 0.f, one / box_length[2]};
@@ -723,7 +768,7 @@ bool md_graphable(const sponge_md* m) {
 
 int md_record_steps(sponge_md* m, int k) {
     for (int c = 0; c < k; ++c) {
-        const int rc = md_step(m, m->step + c, 0);
+        const int rc = md_step(m, m->step + c, 0, nullptr, false);
         if (rc) return rc;
     }
     return 0;
@@ -738,6 +783,14 @@ int md_state_arrays(sponge_md* m, float** crd, float** charge) {
 }
 
 }  // namespace sa
+
+extern "C" int sponge_md_set_reciprocal_overlap(sponge_md* m, int mode) {
+    if (!m || mode < -1 || mode > 1) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    m->overlap_mode = mode;
+    capture_destroy(m->capture);
+    m->capture_len = 0;
+    return 0;
+}
 
 // FFT tiling of the driver's PME module tuned on its current coordinates
 // (sponge_pme_autotune); the captured step is rebuilt with the new tiling.

@@ -163,8 +163,9 @@ __device__ __forceinline__ void stockham_stage(const float2* a, float2* b, int n
 }
 
 // Transform nl lines in place (result returned; ping-pong between a and b).
+// tw: the line length's twiddles, staged in LDS by the caller.
 template <int SIGN>
-__device__ float2* fft_lines(float2* a, float2* b, int nl, int n, int ld, const sa_radix_plan& pl,
+__device__ __forceinline__ float2* fft_lines(float2* a, float2* b, int nl, int n, int ld, const sa_radix_plan& pl,
                              const float2* __restrict__ tw) {
     int Ns = 1;
     for (int s = 0; s < pl.nst; ++s) {
@@ -185,17 +186,21 @@ __device__ float2* fft_lines(float2* a, float2* b, int nl, int n, int ld, const 
     return a;
 }
 
+constexpr int kMaxFftLen = 512;  // largest grid dimension (checked at init)
+
 // z pass, real -> half spectrum: block = (y block, x)
 __global__ void __launch_bounds__(256) k_fft_z_r2c(const float* __restrict__ Q,
                                                    float2* __restrict__ FQ, int fx, int fy, int fz,
                                                    int fzc, int TL, sa_radix_plan pl,
                                                    const float2* __restrict__ tw) {
     extern __shared__ float2 sm[];
+    __shared__ float2 stw[kMaxFftLen];
     const int x = blockIdx.y, y0 = blockIdx.x * TL;
     const int nl = min(TL, fy - y0), npair = (nl + 1) >> 1;
     float2* a = sm;
     float2* b = sm + npair * fz;
     const float* src = Q + ((size_t)x * fy + y0) * fz;
+    for (int k = threadIdx.x; k < fz; k += blockDim.x) stw[k] = tw[k];
     for (int e = threadIdx.x; e < npair * fz; e += blockDim.x) {
         const int m = e / fz, z = e - m * fz;
         const float v0 = src[(size_t)(2 * m) * fz + z];
@@ -203,7 +208,7 @@ __global__ void __launch_bounds__(256) k_fft_z_r2c(const float* __restrict__ Q,
         a[e] = make_float2(v0, v1);
     }
     __syncthreads();
-    const float2* res = fft_lines<-1>(a, b, npair, fz, fz, pl, tw);
+    const float2* res = fft_lines<-1>(a, b, npair, fz, fz, pl, stw);
     for (int e = threadIdx.x; e < fzc * nl; e += blockDim.x) {
         const int kz = e / nl, l = e - kz * nl;
         const int m = l >> 1;
@@ -221,14 +226,16 @@ template <int SIGN>
 __global__ void __launch_bounds__(256) k_fft_y(float2* __restrict__ FQ, int nlines, int fy, int TL,
                                                sa_radix_plan pl, const float2* __restrict__ tw) {
     extern __shared__ float2 sm[];
+    __shared__ float2 stw[kMaxFftLen];
     const int l0 = blockIdx.x * TL;
     const int nl = min(TL, nlines - l0);
     float2* a = sm;
     float2* b = sm + TL * fy;
     float2* base = FQ + (size_t)l0 * fy;
+    for (int k = threadIdx.x; k < fy; k += blockDim.x) stw[k] = tw[k];
     for (int e = threadIdx.x; e < nl * fy; e += blockDim.x) a[e] = base[e];
     __syncthreads();
-    const float2* res = fft_lines<SIGN>(a, b, nl, fy, fy, pl, tw);
+    const float2* res = fft_lines<SIGN>(a, b, nl, fy, fy, pl, stw);
     for (int e = threadIdx.x; e < nl * fy; e += blockDim.x) base[e] = res[e];
 }
 
@@ -238,22 +245,27 @@ __global__ void __launch_bounds__(256) k_fft_y(float2* __restrict__ FQ, int nlin
 // 0.5 * sum Q * FBCFQ, PM_force.cpp:896-919); acc[1..6] -= factor |F|^2
 // virial_BC with factor 0.5 on kz = 0 and kz = fzc-1 (PME_Sum_Virial,
 // :1024-1068, including its odd-fz weighting).
+// ACC: accumulate the energy (and, with need_virial, the virial) sums; off on
+// force-only evaluations.
+template <bool ACC>
 __global__ void __launch_bounds__(256) k_fft_x_conv(
     float2* __restrict__ FQ, const float* __restrict__ BCt, const float* __restrict__ vBCt,
     long long nfft, int fx, int fy, int fz, int fzc, int TL, sa_radix_plan pl,
     const float2* __restrict__ tw, int need_virial, double* __restrict__ acc) {
     extern __shared__ float2 sm[];
     __shared__ double red[7][4];
+    __shared__ float2 stw[kMaxFftLen];
     const int kz = blockIdx.y, y0 = blockIdx.x * TL;
     const int nl = min(TL, fy - y0);
     float2* a = sm;
     float2* b = sm + TL * fx;
+    for (int k = threadIdx.x; k < fx; k += blockDim.x) stw[k] = tw[k];
     for (int e = threadIdx.x; e < fx * nl; e += blockDim.x) {
         const int xi = e / nl, l = e - xi * nl;
         a[l * fx + xi] = FQ[((size_t)kz * fx + xi) * fy + y0 + l];
     }
     __syncthreads();
-    float2* res = fft_lines<-1>(a, b, nl, fx, fx, pl, tw);
+    float2* res = fft_lines<-1>(a, b, nl, fx, fx, pl, stw);
     float2* other = (res == a) ? b : a;
     const double wE = (kz == 0 || 2 * kz == fz) ? 1.0 : 2.0;
     const float vfac = (kz == 0 || kz == fzc - 1) ? 0.5f : 1.0f;
@@ -264,21 +276,24 @@ __global__ void __launch_bounds__(256) k_fft_x_conv(
         const size_t ti = ((size_t)kz * fy + y0 + l) * fx + kx;
         const float2 F = res[e];
         const float bc = BCt[ti];
-        const float mag = F.x * F.x + F.y * F.y;
-        e_acc += wE * (double)bc * (double)mag;
-        if (need_virial) {
-            const float s = vfac * mag;
+        if (ACC) {
+            const float mag = F.x * F.x + F.y * F.y;
+            e_acc += wE * (double)bc * (double)mag;
+            if (need_virial) {
+                const float s = vfac * mag;
 #pragma unroll
-            for (int c = 0; c < 6; ++c) v[c] -= (double)(s * vBCt[(size_t)c * nfft + ti]);
+                for (int c = 0; c < 6; ++c) v[c] -= (double)(s * vBCt[(size_t)c * nfft + ti]);
+            }
         }
         res[e] = make_float2(F.x * bc, F.y * bc);
     }
     __syncthreads();
-    const float2* out = fft_lines<+1>(res, other, nl, fx, fx, pl, tw);
+    const float2* out = fft_lines<+1>(res, other, nl, fx, fx, pl, stw);
     for (int e = threadIdx.x; e < fx * nl; e += blockDim.x) {
         const int xi = e / nl, l = e - xi * nl;
         FQ[((size_t)kz * fx + xi) * fy + y0 + l] = out[l * fx + xi];
     }
+    if (!ACC) return;
     // block reduction of the energy and virial sums
     e_acc = wave_sum(e_acc);
     if (need_virial)
@@ -309,8 +324,10 @@ __global__ void __launch_bounds__(256) k_fft_z_c2r(const float2* __restrict__ FQ
                                                    int fzc, int TL, sa_radix_plan pl,
                                                    const float2* __restrict__ tw) {
     extern __shared__ float2 sm[];
+    __shared__ float2 stw[kMaxFftLen];
     const int x = blockIdx.y, y0 = blockIdx.x * TL;
     const int nl = min(TL, fy - y0), npair = (nl + 1) >> 1;
+    for (int k = threadIdx.x; k < fz; k += blockDim.x) stw[k] = tw[k];
     float2* h = sm;                       // [nl][fzc]
     float2* a = sm + TL * fzc;            // [npair][fz]
     float2* b = a + ((TL + 1) >> 1) * fz;
@@ -341,7 +358,7 @@ __global__ void __launch_bounds__(256) k_fft_z_c2r(const float2* __restrict__ FQ
         a[e] = make_float2(A.x - B.y, A.y + B.x);
     }
     __syncthreads();
-    const float2* res = fft_lines<+1>(a, b, npair, fz, fz, pl, tw);
+    const float2* res = fft_lines<+1>(a, b, npair, fz, fz, pl, stw);
     float* dst = out + ((size_t)x * fy + y0) * fz;
     for (int e = threadIdx.x; e < nl * fz; e += blockDim.x) {
         const int l = e / fz, z = e - l * fz;
@@ -1163,7 +1180,7 @@ extern "C" int sponge_pme_initial(sponge_pme** out, int atom_numbers, const floa
     p->fz = p->opt.fftz > 0 ? p->opt.fftz : sponge_amd_get_fft_parameter(p->cell.a33 / p->opt.grid_spacing);
     const int dims[3] = {p->fx, p->fy, p->fz};
     for (int d = 0; d < 3; ++d) {
-        if (dims[d] < 4 || dims[d] > 512 || !make_plan(dims[d], &p->plan[d])) {
+        if (dims[d] < 4 || dims[d] > kMaxFftLen || !make_plan(dims[d], &p->plan[d])) {
             delete p;
             return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
         }
@@ -1280,7 +1297,7 @@ extern "C" float* sponge_pme_direct_atom_energy(sponge_pme* p) {
 }
 
 // (the energy / virial accumulators acc[0..6] are cleared by k_pme_bin)
-static int pme_fft_convolve(sponge_pme* p, int need_virial) {
+static int pme_fft_convolve(sponge_pme* p, int need_virial, int need_energy) {
     hipStream_t s = p->stream;
     const int fx = p->fx, fy = p->fy, fz = p->fz, fzc = p->fzc;
     {
@@ -1303,9 +1320,10 @@ static int pme_fft_convolve(sponge_pme* p, int need_virial) {
         const int TL = p->tl_x;
         const size_t sh = (size_t)2 * TL * fx * sizeof(float2);
         ScopedTimer timer("pme_fft_x_conv", s);
-        hipLaunchKernelGGL(k_fft_x_conv, dim3(grid_for(fy, TL), fzc), dim3(256), sh, s, p->d_FQ,
-                           p->d_BCt, p->d_vBCt, p->nfft, fx, fy, fz, fzc, TL, p->plan[0],
-                           p->d_tw[0], need_virial, p->d_acc);
+        hipLaunchKernelGGL((need_energy || need_virial) ? k_fft_x_conv<true> : k_fft_x_conv<false>,
+                           dim3(grid_for(fy, TL), fzc), dim3(256), sh, s, p->d_FQ, p->d_BCt,
+                           p->d_vBCt, p->nfft, fx, fy, fz, fzc, TL, p->plan[0], p->d_tw[0],
+                           need_virial, p->d_acc);
         SA_LAUNCH_CHECK();
     }
     {
@@ -1327,57 +1345,62 @@ static int pme_fft_convolve(sponge_pme* p, int need_virial) {
     return 0;
 }
 
-extern "C" int sponge_pme_reciprocal_force(sponge_pme* p, const float* crd, const float* cell,
-                                           const float* rcell, const float* charge, float* frc,
-                                           int need_virial, int need_energy, float* d_virial,
-                                           float* d_potential, int step) {
-    if (!p || !crd || !cell || !rcell || !charge || !frc) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
-    if (!p->opt.calculate_reciprocal_part) return 0;
-    if ((need_virial && !d_virial) || (need_energy && !d_potential))
-        return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+// The reciprocal force in two halves, so that a caller can run the first (bin,
+// spread, FFT convolution: it reads only the coordinates and charges) on a
+// stream of its own beside other work and join before the second (gather,
+// energy), which adds to the force array.  Both use the module's stream.
+int sa::pme_recip_begin(sponge_pme* p, const float* crd, const float* cell, const float* rcell,
+                        const float* charge, int need_virial, int need_energy, int step) {
     const LT6 c = lt_from(cell), r = lt_from(rcell);
     if (memcmp(&c, &p->cell, sizeof(LT6)) != 0) return SPONGE_AMD_ERROR_CONFLICTING_COMMAND;
+    if (step % p->opt.update_interval != 0) return 0;
     hipStream_t s = p->stream;
     const int N = p->N;
-    int rc;
+    {
+        ScopedTimer timer("pme_spread", s);
+        hipLaunchKernelGGL(k_pme_bin, dim3(grid_for(N, 256)), dim3(256), 0, s, N, crd, charge, r,
+                           p->fx, p->fy, p->fz, p->bx, p->by, p->bz, p->nbx, p->nby, p->nbz, p->nb,
+                           p->bcap, p->d_bcount, p->d_bw, p->d_bu, p->d_bi, p->d_ow, p->d_ou,
+                           p->d_oi, p->d_acc);
+        SA_LAUNCH_CHECK();
+        const TileGeom tg = pme_tile_geom(p);
+        const size_t sh = sizeof(float) * (size_t)p->spread_waves * tg.lds_floats();
+        hipLaunchKernelGGL(k_pme_spread_tiles, dim3(p->nb), dim3(kWave * p->spread_waves), sh, s,
+                           p->bx, p->by, p->bz, p->nbx, p->nby, p->nbz, p->nb, p->bcap, tg,
+                           p->d_bcount, p->d_bw, p->d_bu, p->d_ow, p->d_ou, p->d_tiles);
+        SA_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_pme_tiles_reduce, dim3(p->nb), dim3(256), 0, s, p->fx, p->fy, p->fz,
+                           p->bx, p->by, p->bz, p->nbx, p->nby, p->nbz, p->nb, tg, p->d_tiles,
+                           p->d_Q, p->d_bcount, p->d_gcount);
+        SA_LAUNCH_CHECK();
+    }
+    // the energy sum is kept for the energy of later steps between
+    // reciprocal updates, so it is accumulated on every update then
+    return pme_fft_convolve(p, need_virial, need_energy || p->opt.update_interval > 1);
+}
+
+int sa::pme_recip_end(sponge_pme* p, const float* rcell, const float* charge, float* frc,
+                      int need_virial, int need_energy, float* d_virial, float* d_potential,
+                      int step) {
+    const LT6 r = lt_from(rcell);
+    hipStream_t s = p->stream;
+    const int N = p->N;
     if (step % p->opt.update_interval == 0) {
-        {
-            ScopedTimer timer("pme_spread", s);
-            hipLaunchKernelGGL(k_pme_bin, dim3(grid_for(N, 256)), dim3(256), 0, s, N, crd, charge,
-                               r, p->fx, p->fy, p->fz, p->bx, p->by, p->bz, p->nbx, p->nby, p->nbz,
-                               p->nb, p->bcap, p->d_bcount, p->d_bw, p->d_bu, p->d_bi, p->d_ow,
-                               p->d_ou, p->d_oi, p->d_acc);
-            SA_LAUNCH_CHECK();
-            const TileGeom tg = pme_tile_geom(p);
-            const size_t sh = sizeof(float) * (size_t)p->spread_waves * tg.lds_floats();
-            hipLaunchKernelGGL(k_pme_spread_tiles, dim3(p->nb), dim3(kWave * p->spread_waves), sh,
-                               s, p->bx, p->by, p->bz, p->nbx, p->nby, p->nbz, p->nb, p->bcap, tg,
-                               p->d_bcount, p->d_bw, p->d_bu, p->d_ow, p->d_ou, p->d_tiles);
-            SA_LAUNCH_CHECK();
-            hipLaunchKernelGGL(k_pme_tiles_reduce, dim3(p->nb), dim3(256), 0, s, p->fx, p->fy,
-                               p->fz, p->bx, p->by, p->bz, p->nbx, p->nby, p->nbz, p->nb, tg,
-                               p->d_tiles, p->d_Q, p->d_bcount, p->d_gcount);
-            SA_LAUNCH_CHECK();
-        }
-        rc = pme_fft_convolve(p, need_virial);
-        if (rc) return rc;
         if (need_virial) {
             hipLaunchKernelGGL(k_pme_virial_add, dim3(1), dim3(64), 0, s, p->d_acc, d_virial);
             SA_LAUNCH_CHECK();
         }
-        {
-            ScopedTimer timer("pme_gather", s);
-            const TileGeom tg = pme_tile_geom(p);
-            hipLaunchKernelGGL(k_pme_gather_tiles, dim3(p->nb), dim3(kGatherThreads),
-                               sizeof(float) * tg.lds_floats(), s, p->fx, p->fy, p->fz, p->bx,
-                               p->by, p->bz, p->nbx, p->nby, p->nbz, p->nb, p->bcap, tg,
-                               p->d_gcount, p->d_bw, p->d_bu, p->d_bi, p->d_ow, p->d_ou, p->d_oi,
-                               p->d_FBCFQ, r, (float)p->opt.update_interval, frc);
-            SA_LAUNCH_CHECK();
-        }
+        ScopedTimer timer("pme_gather", s);
+        const TileGeom tg = pme_tile_geom(p);
+        hipLaunchKernelGGL(k_pme_gather_tiles, dim3(p->nb), dim3(kGatherThreads),
+                           sizeof(float) * tg.lds_floats(), s, p->fx, p->fy, p->fz, p->bx, p->by,
+                           p->bz, p->nbx, p->nby, p->nbz, p->nb, p->bcap, tg, p->d_gcount, p->d_bw,
+                           p->d_bu, p->d_bi, p->d_ow, p->d_ou, p->d_oi, p->d_FBCFQ, r,
+                           (float)p->opt.update_interval, frc);
+        SA_LAUNCH_CHECK();
     }
     if (need_energy) {
-        rc = fill_async(p->d_acc + 7, 0.0, 2, s);
+        int rc = fill_async(p->d_acc + 7, 0.0, 2, s);
         if (rc) return rc;
         hipLaunchKernelGGL(k_pme_charge_sums, dim3(64), dim3(256), 0, s, N, charge, p->d_acc);
         SA_LAUNCH_CHECK();
@@ -1387,6 +1410,24 @@ extern "C" int sponge_pme_reciprocal_force(sponge_pme* p, const float* crd, cons
         SA_LAUNCH_CHECK();
     }
     return 0;
+}
+
+hipStream_t sa::pme_stream(const sponge_pme* p) { return p->stream; }
+bool sa::pme_calculates_reciprocal(const sponge_pme* p) { return p->opt.calculate_reciprocal_part != 0; }
+void sa::pme_set_stream(sponge_pme* p, hipStream_t s) { p->stream = s; }
+
+extern "C" int sponge_pme_reciprocal_force(sponge_pme* p, const float* crd, const float* cell,
+                                           const float* rcell, const float* charge, float* frc,
+                                           int need_virial, int need_energy, float* d_virial,
+                                           float* d_potential, int step) {
+    if (!p || !crd || !cell || !rcell || !charge || !frc) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    if (!p->opt.calculate_reciprocal_part) return 0;
+    if ((need_virial && !d_virial) || (need_energy && !d_potential))
+        return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    int rc = sa::pme_recip_begin(p, crd, cell, rcell, charge, need_virial, need_energy, step);
+    if (rc) return rc;
+    return sa::pme_recip_end(p, rcell, charge, frc, need_virial, need_energy, d_virial,
+                             d_potential, step);
 }
 
 extern "C" int sponge_pme_excluded_force(sponge_pme* p, int local_atom_numbers,

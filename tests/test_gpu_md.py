@@ -77,6 +77,38 @@ def test_graph_replay_matches_eager(dev):
     assert np.abs(va - vb).max() < 1e-2 * np.abs(va).max()
 
 
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_reciprocal_overlap_matches_one_stream(dev, use_graph):
+    """The reciprocal half on its own stream (forced on for this small box)
+    gives the one-stream trajectory up to the order of the float additions;
+    switching the mode between runs rebuilds the captured step."""
+    from sponge_amd import abi
+    from sponge_amd import system as S
+    s = S.config(1)
+    a = abi.MDSystem(s)
+    b = abi.MDSystem(s)
+    a.set_reciprocal_overlap(0)
+    b.set_reciprocal_overlap(1)
+    a.run(12, energy_interval=0, use_graph=use_graph)
+    b.run(12, energy_interval=0, use_graph=use_graph)
+    ca, va, fa = a.state()
+    cb, vb, fb = b.state()
+    print(f"case: graph={use_graph} max |dx| {np.abs(ca - cb).max():.2e}")
+    assert np.abs(ca - cb).max() < 1e-3
+    assert np.abs(va - vb).max() < 1e-2 * np.abs(va).max()
+    # energies on an energy step (the reciprocal energy sum is accumulated
+    # on the side stream) and a switch back to one stream
+    a.run(1, energy_interval=1)
+    b.run(1, energy_interval=1)
+    ea, eb = a.energies(), b.energies()
+    assert abs(ea["potential"] - eb["potential"]) < 1e-3 * abs(ea["potential"])
+    b.set_reciprocal_overlap(0)
+    b.run(8, energy_interval=0, use_graph=use_graph)
+    assert np.isfinite(b.state()[0]).all()
+    with pytest.raises(abi.SpongeError):
+        b.set_reciprocal_overlap(2)
+
+
 def test_kernel_timers_inside_graph(dev):
     """Timers recorded during capture become graph event nodes that are
     re-armed on every replay: one timed pair-kernel launch per step."""
