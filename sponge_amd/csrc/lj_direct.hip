@@ -135,6 +135,13 @@ __global__ void __launch_bounds__(256) k_lj_pairs(
 constexpr int kPairStage = 3584;
 constexpr int kMaxLdsTypes = 64;  // the square (A, B) table lives in LDS: up to 64 LJ types
 
+// The periodic image of x nearest the point C (orthorhombic box).
+__device__ __forceinline__ float3 near_image(float3 x, float3 C, const LT6& c, const LT6& r) {
+    return make_float3(x.x - c.a11 * rintf((x.x - C.x) * r.a11),
+                       x.y - c.a22 * rintf((x.y - C.y) * r.a22),
+                       x.z - c.a33 * rintf((x.z - C.z) * r.a33));
+}
+
 // Partner data of a row entry: from the block's LDS stage (every position of
 // the stream staged) or, for the rare stream longer than the stage, from the
 // sorted arrays in HBM.  The two sources are separate code paths chosen once
@@ -205,15 +212,19 @@ struct PairTerm {
 // (one 32-bit load: rows start 4-byte aligned since the row capacity is
 // even), with the next pass's pair loaded before the current one is used so
 // that the list's memory latency overlaps the arithmetic.
-template <bool ORTHO, bool NEED_E, bool NEED_V, bool FAST, bool STAGED>
+// SHIFT: the staged partner positions are the images nearest the cell's
+// centre C (see k_lj_cells), so the displacement is a plain difference with
+// the row atom's own image nearest C.
+template <bool ORTHO, bool NEED_E, bool NEED_V, bool FAST, bool STAGED, bool SHIFT = false>
 __device__ __forceinline__ void lj_row(int p, int lane, int n, const unsigned short* __restrict__ row,
                                        PairSrc src, int nr, const int* r_first, const int* r_off,
                                        const float4* __restrict__ pos, const float2* trow, int gi,
                                        const LT6& c, const LT6& r, float rc2_thr, float beta,
                                        float4* __restrict__ facc, float2* __restrict__ eacc,
-                                       float* __restrict__ vacc) {
+                                       float* __restrict__ vacc, float3 C = float3{0.f, 0.f, 0.f}) {
     const float4 pi4 = pos[p];
-    const float3 pi = make_float3(pi4.x, pi4.y, pi4.z);
+    const float3 pi = SHIFT ? near_image(make_float3(pi4.x, pi4.y, pi4.z), C, c, r)
+                            : make_float3(pi4.x, pi4.y, pi4.z);
     float fx = 0.f, fy = 0.f, fz = 0.f;
     float elj = 0.f, ec = 0.f;
     float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f, v5 = 0.f;
@@ -234,11 +245,22 @@ __device__ __forceinline__ void lj_row(int p, int lane, int n, const unsigned sh
         const float4 pb = src.pos[qb];
         const float2 aba = trow[src.typ[qa]];
         const float2 abb = trow[src.typ[qb]];
-        const float3 da = disp_pbc<ORTHO>(make_float3(pa.x, pa.y, pa.z), pi, c, r);
-        const float3 db = disp_pbc<ORTHO>(make_float3(pb.x, pb.y, pb.z), pi, c, r);
+        float3 da, db;
+        float r2a, r2b;
+        if (SHIFT) {
+            da = make_float3(pa.x - pi.x, pa.y - pi.y, pa.z - pi.z);
+            db = make_float3(pb.x - pi.x, pb.y - pi.y, pb.z - pi.z);
+            r2a = fmaf(da.z, da.z, fmaf(da.y, da.y, da.x * da.x));
+            r2b = fmaf(db.z, db.z, fmaf(db.y, db.y, db.x * db.x));
+        } else {
+            da = disp_pbc<ORTHO>(make_float3(pa.x, pa.y, pa.z), pi, c, r);
+            db = disp_pbc<ORTHO>(make_float3(pb.x, pb.y, pb.z), pi, c, r);
+            r2a = norm2_exact(da);
+            r2b = norm2_exact(db);
+        }
         // an absent entry is given r2 = the cutoff threshold: out of range
-        const float r2a = has_a ? norm2_exact(da) : rc2_thr;
-        const float r2b = has_b ? norm2_exact(db) : rc2_thr;
+        r2a = has_a ? r2a : rc2_thr;
+        r2b = has_b ? r2b : rc2_thr;
         const PairTerm<NEED_E, NEED_V, FAST> A(da, r2a, pi4.w, pa.w, aba, rc2_thr, beta);
         const PairTerm<NEED_E, NEED_V, FAST> B(db, r2b, pi4.w, pb.w, abb, rc2_thr, beta);
         fx += A.lx + B.lx;
@@ -302,6 +324,7 @@ __global__ void __launch_bounds__(1024) k_lj_cells(
     const int* __restrict__ list_count, const unsigned short* __restrict__ list, int cap,
     const float4* __restrict__ pos, const int* __restrict__ ptype, const int* __restrict__ pgid,
     const float2* __restrict__ AB_sq, int T, LT6 c, LT6 r, float rc2_thr, float beta, int stage,
+    float gl, int Nx, int Ny, int Nz, float cutoff,
     float4* __restrict__ facc, float2* __restrict__ eacc, float* __restrict__ vacc) {
     extern __shared__ float4 cpos[];
     int* ctyp = reinterpret_cast<int*>(cpos + stage);
@@ -312,6 +335,26 @@ __global__ void __launch_bounds__(1024) k_lj_cells(
     const int cell = blockIdx.x;
     const int a0 = start[cell], a1 = start[cell + 1];
     if (a0 == a1) return;
+    // Force-only steps in an orthorhombic box wider than twice (cutoff + half
+    // the cell + 1 A) on every axis: the partners are staged as their images
+    // nearest the cell's centre C.  A pair within the cutoff then has its
+    // minimum image there (every other image is more than L/2 - (rc + h) > 0
+    // further), so the displacement is a plain difference; it differs from
+    // the reference arithmetic only by the rounding of the shifted
+    // coordinates (a few ulp of a coordinate), and pairs beyond the cutoff
+    // stay beyond it.
+    float3 C = make_float3(0.f, 0.f, 0.f);
+    bool shift = false;
+    if (FAST && ORTHO) {
+        const int ix = cell / (Ny * Nz), iy = (cell / Nz) % Ny, iz = cell % Nz;
+        const float lx = ix * gl, hx = ix == Nx - 1 ? c.a11 : (ix + 1) * gl;
+        const float ly = iy * gl, hy = iy == Ny - 1 ? c.a22 : (iy + 1) * gl;
+        const float lz = iz * gl, hz = iz == Nz - 1 ? c.a33 : (iz + 1) * gl;
+        C = make_float3(0.5f * (lx + hx), 0.5f * (ly + hy), 0.5f * (lz + hz));
+        shift = 0.5f * c.a11 > cutoff + 0.5f * (hx - lx) + 1.f &&
+                0.5f * c.a22 > cutoff + 0.5f * (hy - ly) + 1.f &&
+                0.5f * c.a33 > cutoff + 0.5f * (hz - lz) + 1.f;
+    }
     const int nr = nranges[cell];
     for (int k = threadIdx.x; k < nr; k += blockDim.x) r_first[k] = g_rfirst[(size_t)cell * max_ranges + k];
     for (int k = threadIdx.x; k <= nr; k += blockDim.x) r_off[k] = g_roff[(size_t)cell * (max_ranges + 1) + k];
@@ -321,7 +364,12 @@ __global__ void __launch_bounds__(1024) k_lj_cells(
     const bool staged = total <= stage;
     if (staged)
         stage_stream(nr, r_first, r_off, total, [&](int t, int q) {
-            cpos[t] = pos[q];
+            float4 v = pos[q];
+            if (shift) {
+                const float3 w = near_image(make_float3(v.x, v.y, v.z), C, c, r);
+                v = make_float4(w.x, w.y, w.z, v.w);
+            }
+            cpos[t] = v;
             ctyp[t] = ptype[q];
             if (NEED_E || NEED_V) cgid[t] = pgid[q];
         });
@@ -333,7 +381,12 @@ __global__ void __launch_bounds__(1024) k_lj_cells(
         const int gi = (NEED_E || NEED_V) ? pgid[p] : 0;
         const int n = list_count[p];
         const unsigned short* row = list + (size_t)p * cap;
-        if (staged)
+        if (staged && shift)
+            lj_row<ORTHO, NEED_E, NEED_V, FAST, true, true>(p, lane, n, row,
+                                                            PairSrc{cpos, ctyp, cgid}, nr, r_first,
+                                                            r_off, pos, trow, gi, c, r, rc2_thr,
+                                                            beta, facc, eacc, vacc, C);
+        else if (staged)
             lj_row<ORTHO, NEED_E, NEED_V, FAST, true>(p, lane, n, row, PairSrc{cpos, ctyp, cgid}, nr,
                                                       r_first, r_off, pos, trow, gi, c, r, rc2_thr,
                                                       beta, facc, eacc, vacc);
@@ -422,7 +475,8 @@ int launch_rows(sponge_lj* lj, sponge_nl* nl, LT6 c, LT6 r, float beta, int need
     hipLaunchKernelGGL((k_lj_cells<ORTHO, E, V, F>), dim3(nl->G), dim3(1024), sh, s, nl->d_cell_start, \
                        nl->d_nranges, nl->max_ranges, nl->d_rfirst, nl->d_roff, nl->d_list_count,    \
                        nl->d_list, nl->cap, lj->d_pos, lj->d_ptype, lj->d_pgid, lj->d_AB_sq, lj->T,  \
-                       c, r, lj->rc2_thr, beta, lj->stage, lj->d_facc, lj->d_eacc, lj->d_vacc)
+                       c, r, lj->rc2_thr, beta, lj->stage, nl->grid_length, nl->Nx, nl->Ny,       \
+                       nl->Nz, lj->cutoff, lj->d_facc, lj->d_eacc, lj->d_vacc)
     if (fast) SA_LJ_ROWS(false, false, true);
     else if (need_e && need_v) SA_LJ_ROWS(true, true, false);
     else if (need_e) SA_LJ_ROWS(true, false, false);

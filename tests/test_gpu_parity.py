@@ -74,6 +74,37 @@ def test_neighbor_list_bit_exact_large_box(dev, variant):
     assert np.array_equal(got, want), (len(got), len(want))
 
 
+@pytest.mark.parametrize("variant", ["lattice", "perturbed"])
+def test_lj_direct_force_only_large_box(dev, variant):
+    """Force-only pair forces in a box of more than twice (cutoff + cell)
+    (24,000 atoms, 62.1 A), where the pair kernel stages partners as their
+    images nearest the cell centre; some atoms given as images a box length
+    away: per-atom forces against the oracle's reference arithmetic."""
+    import torch
+    from oracle import oracle as O
+    from sponge_amd import abi
+    from sponge_amd import system as S
+    s = S.config(2)
+    crd = s.crd if variant == "lattice" else _perturbed(s, 7, 0.2)
+    beta = O.get_beta(s.cutoff, s.tolerance)
+    c, ser, flags = O.nl_build(s, crd=crd)
+    assert flags == 0
+    import dataclasses
+    ref = O.lj_direct(dataclasses.replace(s, crd=crd), c, ser, beta, need_energy=False,
+                      need_virial=False)
+    d = DevSystem(s, dev, crd)
+    box = abi.Box(s.box)
+    nl = abi.NeighborList(d.N, s.cutoff, s.skin, box)
+    nl.update(d.crd, update=abi.NL_FORCED, excl=d.excl)
+    lj = abi.LennardJones(d.N, s.n_types, s.lj_A, s.lj_B, s.lj_type, s.cutoff)
+    f = d.zeros(d.N, 3)
+    lj.direct_force(nl, d.crd, d.charge, f, box, beta)
+    torch.cuda.synchronize()
+    e = rel_rms(f.cpu().numpy(), ref["frc"])
+    print(f"case: {variant} force rel rms {e:.2e}")
+    assert e < FORCE_TOL
+
+
 def test_neighbor_list_threshold_edges_far_images(dev):
     """The threshold cases in a 60 A box with atoms given as images several
     box lengths away: r^2 == (rc+skin)^2 is out, one ulp inside is in, across
