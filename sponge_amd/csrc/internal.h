@@ -178,10 +178,13 @@ void rsur_set_stream(sponge_rsur* h, hipStream_t s);
 // energy) on the module's stream, and that stream
 int pme_recip_begin(sponge_pme* p, const float* crd, const float* cell, const float* rcell,
                     const float* charge, int need_virial, int need_energy, int step);
+// extra (nullable, [N][3]): forces the gather adds with the reciprocal ones
 int pme_recip_end(sponge_pme* p, const float* rcell, const float* charge, float* frc,
-                  int need_virial, int need_energy, float* d_virial, float* d_potential, int step);
+                  int need_virial, int need_energy, float* d_virial, float* d_potential, int step,
+                  const float* extra = nullptr);
 hipStream_t pme_stream(const sponge_pme* p);
 bool pme_calculates_reciprocal(const sponge_pme* p);
+bool pme_updates_on(const sponge_pme* p, int step);  // the reciprocal part is evaluated this step
 void pme_set_stream(sponge_pme* p, hipStream_t s);
 int rsur_atoms(const sponge_rsur* h);
 float rsur_cutoff(const sponge_rsur* h);

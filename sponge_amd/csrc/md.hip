@@ -78,6 +78,7 @@ struct sponge_md {
     hipStream_t recip_stream = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int overlap_mode = -1;  // 1 on, 0 off, -1 from the atom count
+    float* d_exfrc = nullptr;  // [N][3] excluded-pair correction computed beside the pair work
 };
 
 namespace {
@@ -342,6 +343,7 @@ void md_free(sponge_md* m) {
     dfree(m->d_scount);
     if (m->own_stream && m->stream) (void)hipStreamDestroy(m->stream);
     m->stream = nullptr;
+    dfree(m->d_exfrc);
     if (m->recip_stream) (void)hipStreamDestroy(m->recip_stream);
     if (m->ev_fork) (void)hipEventDestroy(m->ev_fork);
     if (m->ev_join) (void)hipEventDestroy(m->ev_join);
@@ -421,11 +423,21 @@ int md_step(sponge_md* m, long long step, int need_energy, StepKind* kind_out = 
                         (m->overlap_mode < 0 && m->N >= SPONGE_AMD_MD_OVERLAP_MIN_ATOMS);
     const bool overlap = wanted && allow_fork && kind == StepKind::Pme && !m->rsur &&
                          !m->collect && !g_timing && pme_calculates_reciprocal(m->pme);
+    // on a force-only step the excluded-pair correction (which needs only the
+    // coordinates and the exclusion lists) joins the side stream too, into its
+    // own array that the gather adds; energy steps keep it on the MD stream,
+    // where the per-atom energies are summed
+    const bool ex_side = overlap && !need_energy && m->d_exfrc && pme_updates_on(m->pme, (int)step);
     if (overlap) {
         SA_CHECK(hipEventRecord(m->ev_fork, s));
         SA_CHECK(hipStreamWaitEvent(m->recip_stream, m->ev_fork, 0));
         pme_set_stream(m->pme, m->recip_stream);
         rc = pme_recip_begin(m->pme, m->d_crd, cell, rcell, m->d_charge, 0, need_energy, (int)step);
+        if (!rc && ex_side) rc = fill_async(m->d_exfrc, 0.f, 3 * N, m->recip_stream);
+        if (!rc && ex_side)
+            rc = sponge_pme_excluded_force(m->pme, N, nullptr, nullptr, m->d_crd, cell, rcell,
+                                           m->d_charge, m->d_excl_start, m->d_excl_list,
+                                           m->d_excl_num, m->d_exfrc, 0, nullptr, 0, nullptr);
         pme_set_stream(m->pme, s);
         if (rc) return rc;
         SA_CHECK(hipEventRecord(m->ev_join, m->recip_stream));
@@ -443,10 +455,13 @@ int md_step(sponge_md* m, long long step, int need_energy, StepKind* kind_out = 
                             m->pme->d_direct_atom_energy, m->nl->d_need);
     if (rc) return rc;
     m->need_cleared = true;
-    rc = sponge_pme_excluded_force(m->pme, N, nullptr, nullptr, m->d_crd, cell, rcell, m->d_charge,
-                                   m->d_excl_start, m->d_excl_list, m->d_excl_num, m->d_frc,
-                                   need_energy, m->d_atom_energy, 0, nullptr);
-    if (rc) return rc;
+    if (!ex_side) {
+        rc = sponge_pme_excluded_force(m->pme, N, nullptr, nullptr, m->d_crd, cell, rcell,
+                                       m->d_charge, m->d_excl_start, m->d_excl_list,
+                                       m->d_excl_num, m->d_frc, need_energy, m->d_atom_energy, 0,
+                                       nullptr);
+        if (rc) return rc;
+    }
     if (need_energy) {
         rc = sponge_lj_long_range_correction(m->lj, 0, nullptr, 1, m->d_potential, m->volume);
         if (rc) return rc;
@@ -455,7 +470,8 @@ int md_step(sponge_md* m, long long step, int need_energy, StepKind* kind_out = 
     if (overlap) {
         SA_CHECK(hipStreamWaitEvent(s, m->ev_join, 0));
         rc = pme_recip_end(m->pme, rcell, m->d_charge, m->d_frc, 0, need_energy, nullptr,
-                           need_energy ? m->d_potential : nullptr, (int)step);
+                           need_energy ? m->d_potential : nullptr, (int)step,
+                           ex_side ? m->d_exfrc : nullptr);
         if (rc) return rc;
     } else if (kind == StepKind::Pme && m->rsur && !m->collect) {
         // exact step of the learned schedule: the anchor of the next steps
@@ -568,7 +584,8 @@ extern "C" int sponge_md_initial(sponge_md** out, int atom_numbers, const float*
     // default priority, like every stream of this library
     if (hipStreamCreateWithFlags(&m->recip_stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&m->ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&m->ev_join, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&m->ev_join, hipEventDisableTiming) != hipSuccess ||
+        dmalloc(&m->d_exfrc, (size_t)3 * N)) {
         md_free(m);
         delete m;
         return SPONGE_AMD_ERROR_MALLOC_FAILED;

@@ -728,7 +728,7 @@ __global__ void __launch_bounds__(kGatherThreads) k_pme_gather_tiles(
     TileGeom tg, const int* __restrict__ gcount, const float4* __restrict__ bw,
     const int* __restrict__ bu, const int* __restrict__ bi, const float4* __restrict__ ow,
     const int* __restrict__ ou, const int* __restrict__ oi, const float* __restrict__ phi, LT6 rc,
-    float update_interval, float* __restrict__ frc) {
+    float update_interval, float* __restrict__ frc, const float* __restrict__ extra) {
     extern __shared__ float tile[];
     const int b = blockIdx.x;
     const int cz = b % nbz, cy = (b / nbz) % nby, cx = b / (nbz * nby);
@@ -818,9 +818,15 @@ __global__ void __launch_bounds__(kGatherThreads) k_pme_gather_tiles(
         const float ox = dq * (gx * rc.a11);
         const float oy = dq * (gx * rc.a21 + gy * rc.a22);
         const float oz = dq * (gx * rc.a31 + gy * rc.a32 + gz * rc.a33);
-        frc[3 * i] = frc[3 * i] + update_interval * ox;
-        frc[3 * i + 1] = frc[3 * i + 1] + update_interval * oy;
-        frc[3 * i + 2] = frc[3 * i + 2] + update_interval * oz;
+        float ex = 0.f, ey = 0.f, ez = 0.f;
+        if (extra) {
+            ex = extra[3 * i];
+            ey = extra[3 * i + 1];
+            ez = extra[3 * i + 2];
+        }
+        frc[3 * i] = frc[3 * i] + update_interval * ox + ex;
+        frc[3 * i + 1] = frc[3 * i + 1] + update_interval * oy + ey;
+        frc[3 * i + 2] = frc[3 * i + 2] + update_interval * oz + ez;
     }
 }
 
@@ -1381,7 +1387,7 @@ int sa::pme_recip_begin(sponge_pme* p, const float* crd, const float* cell, cons
 
 int sa::pme_recip_end(sponge_pme* p, const float* rcell, const float* charge, float* frc,
                       int need_virial, int need_energy, float* d_virial, float* d_potential,
-                      int step) {
+                      int step, const float* extra) {
     const LT6 r = lt_from(rcell);
     hipStream_t s = p->stream;
     const int N = p->N;
@@ -1396,7 +1402,7 @@ int sa::pme_recip_end(sponge_pme* p, const float* rcell, const float* charge, fl
                            sizeof(float) * tg.lds_floats(), s, p->fx, p->fy, p->fz, p->bx, p->by,
                            p->bz, p->nbx, p->nby, p->nbz, p->nb, p->bcap, tg, p->d_gcount, p->d_bw,
                            p->d_bu, p->d_bi, p->d_ow, p->d_ou, p->d_oi, p->d_FBCFQ, r,
-                           (float)p->opt.update_interval, frc);
+                           (float)p->opt.update_interval, frc, extra);
         SA_LAUNCH_CHECK();
     }
     if (need_energy) {
@@ -1414,6 +1420,7 @@ int sa::pme_recip_end(sponge_pme* p, const float* rcell, const float* charge, fl
 
 hipStream_t sa::pme_stream(const sponge_pme* p) { return p->stream; }
 bool sa::pme_calculates_reciprocal(const sponge_pme* p) { return p->opt.calculate_reciprocal_part != 0; }
+bool sa::pme_updates_on(const sponge_pme* p, int step) { return step % p->opt.update_interval == 0; }
 void sa::pme_set_stream(sponge_pme* p, hipStream_t s) { p->stream = s; }
 
 extern "C" int sponge_pme_reciprocal_force(sponge_pme* p, const float* crd, const float* cell,
