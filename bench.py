@@ -51,6 +51,29 @@ def pair_kernel_flops(n_in, n_list):
     return 45.0 * n_in + 17.0 * (n_list - n_in)
 
 
+def pme_hbm_rates(kernel, n_atoms, fft):
+    """Effective HBM rates of the reciprocal pieces from the eager-pass timers
+    (each timer spans its kernels' launches): algorithmic bytes per
+    evaluation -- spread 16 N + 4 G (coordinates and charges read, grid
+    written), gather 40 N + 4 G (coordinates and charges, grid, force
+    read-modify-write), convolution 8 G + 28 Nfft over the five FFT passes --
+    divided by the time."""
+    G = int(np.prod(fft))
+    nfft = fft[0] * fft[1] * (fft[2] // 2 + 1)
+    out = {}
+    pieces = {"spread": (16 * n_atoms + 4 * G, ["pme_spread"]),
+              "gather": (40 * n_atoms + 4 * G, ["pme_gather"]),
+              "fft_convolution": (8 * G + 28 * nfft, ["pme_fft_z_r2c", "pme_fft_y", "pme_fft_x_conv",
+                                                      "pme_ifft_y", "pme_fft_z_c2r"])}
+    for name, (nbytes, timers) in pieces.items():
+        if not all(kernel.get(t, (0, 0))[1] for t in timers):
+            continue
+        us = sum(kernel[t][0] * 1e3 / kernel[t][1] for t in timers)
+        gbs = nbytes / (us * 1e-6) / 1e9
+        out[name] = dict(bytes=nbytes, us=us, GB_s=gbs, frac=gbs / HBM_PEAK_GBS)
+    return out
+
+
 def pair_counts(crd, counts, serial, box, cutoff, chunk=1 << 22):
     """(pairs within the cutoff, entries) of a reference-format half list:
     FP32 minimum-image distances of every entry."""
@@ -375,7 +398,8 @@ def main():
                        else "eager, reciprocal half on a second stream"),
                pme_fft_tiling=list(fft_tiling),
                kernel_avg_us_eager_pass={k: (v[0] * 1e3 / v[1] if v[1] else None)
-                                         for k, v in kernel.items()})
+                                         for k, v in kernel.items()},
+               pme_hbm=pme_hbm_rates(kernel, N, s.fft))
     if not args.no_learned:
         out["learned"] = learned_form(md, s, torch, abi)
     if not args.no_batched:

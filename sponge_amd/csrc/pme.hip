@@ -501,6 +501,16 @@ __global__ void __launch_bounds__(256) k_pme_bin(int N, const float* __restrict_
     }
 }
 
+// Brick of a workgroup of the spread, tile-reduce and gather launches (one
+// workgroup per brick).  Workgroups go to the 8 XCDs in turn, so workgroup i
+// takes brick (i % 8) * (nb / 8) + i / 8: each XCD gets a contiguous range of
+// bricks (x-major order), whose neighbours' tiles mostly sit in the same L2.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int brick_for_block(int i, int nb) {
+    if (nb % kXcds) return i;
+    return (i % kXcds) * (nb / kXcds) + i / kXcds;
+}
+
 __device__ __forceinline__ float lane_f(float v, int lane) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
@@ -549,7 +559,7 @@ __global__ void __launch_bounds__(256) k_pme_spread_tiles(
     const int* __restrict__ bcount, const float4* __restrict__ bw, const int* __restrict__ bu,
     const float4* __restrict__ ow, const int* __restrict__ ou, float* __restrict__ tiles) {
     extern __shared__ float acc[];
-    const int b = blockIdx.x;
+    const int b = brick_for_block(blockIdx.x, gridDim.x);
     const int cz = b % nbz, cy = (b / nbz) % nby, cx = b / (nbz * nby);
     const int X0 = cx * bx, Y0 = cy * by, Z0 = cz * bz;
     const int nwave = blockDim.x / kWave;
@@ -644,7 +654,7 @@ __global__ void __launch_bounds__(256) k_pme_tiles_reduce(int fx, int fy, int fz
                                                           float* __restrict__ Q,
                                                           int* __restrict__ bcount,
                                                           int* __restrict__ gcount) {
-    const int b = blockIdx.x;
+    const int b = brick_for_block(blockIdx.x, gridDim.x);
     if (threadIdx.x == 0) {
         gcount[b] = bcount[b];
         bcount[b] = 0;
@@ -730,7 +740,7 @@ __global__ void __launch_bounds__(kGatherThreads) k_pme_gather_tiles(
     const int* __restrict__ ou, const int* __restrict__ oi, const float* __restrict__ phi, LT6 rc,
     float update_interval, float* __restrict__ frc, const float* __restrict__ extra) {
     extern __shared__ float tile[];
-    const int b = blockIdx.x;
+    const int b = brick_for_block(blockIdx.x, gridDim.x);
     const int cz = b % nbz, cy = (b / nbz) % nby, cx = b / (nbz * nby);
     const int X0 = cx * bx, Y0 = cy * by, Z0 = cz * bz;
     const int TX = (cx == nbx - 1 ? fx - X0 : bx) + 3;
