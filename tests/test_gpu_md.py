@@ -109,6 +109,26 @@ def test_reciprocal_overlap_matches_one_stream(dev, use_graph):
         b.set_reciprocal_overlap(2)
 
 
+def test_reciprocal_overlap_with_impulse_schedule(dev):
+    """PME update_interval = 2 (the reference's impulse schedule: the
+    reciprocal force every second step, doubled): the two-stream step skips
+    the reciprocal half on the other steps exactly as the one-stream step."""
+    from sponge_amd import abi
+    from sponge_amd import system as S
+    s = S.config(1)
+    a = abi.MDSystem(s, pme_opts={"update_interval": 2})
+    b = abi.MDSystem(s, pme_opts={"update_interval": 2})
+    a.set_reciprocal_overlap(0)
+    b.set_reciprocal_overlap(1)
+    a.run(11, energy_interval=0)
+    b.run(11, energy_interval=0)
+    ca, va, _ = a.state()
+    cb, vb, _ = b.state()
+    print(f"case: impulse schedule max |dx| {np.abs(ca - cb).max():.2e}")
+    assert np.abs(ca - cb).max() < 1e-3
+    assert np.abs(va - vb).max() < 1e-2 * np.abs(va).max()
+
+
 def test_kernel_timers_inside_graph(dev):
     """Timers recorded during capture become graph event nodes that are
     re-armed on every replay: one timed pair-kernel launch per step."""
