@@ -254,7 +254,9 @@ __global__ void __launch_bounds__(512) k_nl_search_cells(
     stage_stream(nr, r_first, r_off, staged, [&](int t, int q) { cand[t] = spos[q]; });
     __syncthreads();
     const int lane = threadIdx.x & (kWave - 1);
-    const int wave = threadIdx.x / kWave, nwave = blockDim.x / kWave;
+    // the wavefront index as a scalar: row bases and counters stay in SGPRs
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int nwave = blockDim.x / kWave;
     const int rows = a1 - a0;
     for (int k0 = 0; wave + k0 * nwave < rows; k0 += kSearchRows) {
         float3 pi[kSearchRows];
@@ -281,8 +283,9 @@ __global__ void __launch_bounds__(512) k_nl_search_cells(
             for (int u = 0; u < kSearchRows; ++u) {
                 if (!act[u]) continue;
                 // Get_Displacement(crd_i, crd_j) as in Find_Neighbors_Gridly
+                // evaluated for every lane (bitwise, no branch per test)
                 const float3 d = disp_pbc<ORTHO>(pi[u], pj, c, r);
-                bool ok = in && gj != gi[u] && norm2_exact(d) < thr;
+                bool ok = in & (gj != gi[u]) & (norm2_exact(d) < thr);
                 if (ok && gj >= wi[u].x && gj <= wi[u].y)
                     ok = !excluded_pair(gi[u], gj, ex_start, ex_list, ex_num);
                 const unsigned long long m = __ballot(ok);
