@@ -283,7 +283,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the hot path is "
-              "measured on one GPU's share; see DESIGN.md for how it would shard.",
+              "measured on one GPU's share, and value is that one GPU's throughput.",
               file=sys.stderr)
         sys.exit(2)
 
