@@ -220,6 +220,17 @@ class Box:
         self.rcell = np.array([one / L[0], 0, one / L[1], 0, 0, one / L[2]], np.float32)
         self.volume = float(np.float32(L[0] * L[1]) * L[2])
 
+    @classmethod
+    def from_cell(cls, cell, rcell):
+        """A general lower-triangular cell {a11, a21, a22, a31, a32, a33} and
+        its inverse, as given (the neighbor list and the pair kernels take
+        any such cell; the PME module takes orthorhombic ones)."""
+        b = cls.__new__(cls)
+        b.cell = np.ascontiguousarray(cell, np.float32).copy()
+        b.rcell = np.ascontiguousarray(rcell, np.float32).copy()
+        b.volume = float(np.float32(b.cell[0] * b.cell[2]) * b.cell[5])
+        return b
+
     def args(self):
         return self.cell.ctypes.data_as(_FP), self.rcell.ctypes.data_as(_FP)
 
