@@ -225,7 +225,8 @@ typedef struct sponge_pme_options {
 void sponge_pme_default_options(sponge_pme_options* opt);
 
 /* Particle_Mesh::Initial(controller, atom_numbers, boundary, box_length,
- * cutoff, ...) (PM_force.cpp:295-654), orthorhombic boxes. FFT sizes must be
+ * cutoff, ...) (PM_force.cpp:295-654), any lower-triangular cell (box_length
+ * for the grid sizing = the lengths of the cell vectors). FFT sizes must be
  * 2,3,5,7-smooth and at most 512. */
 int sponge_pme_initial(sponge_pme** out, int atom_numbers, const float* cell, const float* rcell,
                        float cutoff, const sponge_pme_options* opt, void* stream);
@@ -270,7 +271,7 @@ int sponge_pme_grids(sponge_pme* pme, float* Q, float* FBCFQ);
 
 /* Particle_Mesh::Update_Box (PM_force.cpp:1240-1256, up_box_bc :1171-1238):
  * the BC / virial_BC tables and the neutralizing factor for a new
- * (orthorhombic) box on the same grid, as a barostat step needs (host
+ * box on the same grid, as a barostat step needs (host
  * LTMatrix3 pair).  Synchronises.  The neighbor list is re-initialized on a
  * box change, as the reference does (SPONGE/main.cpp:2104-2110). */
 int sponge_pme_update_box(sponge_pme* pme, const float* cell, const float* rcell);

@@ -1186,14 +1186,22 @@ extern "C" int sponge_pme_initial(sponge_pme** out, int atom_numbers, const floa
     p->cell = lt_from(cell);
     p->rcell = lt_from(rcell);
     p->stream = (hipStream_t)stream;
-    if (!lt_is_orthorhombic(p->cell) || p->opt.update_interval <= 0 || !(p->opt.grid_spacing > 0)) {
+    if (p->opt.update_interval <= 0 || !(p->opt.grid_spacing > 0)) {
         delete p;
         return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
     }
-    // grid (PM_force.cpp:383-390): box_length / grid_spacing when not given
-    p->fx = p->opt.fftx > 0 ? p->opt.fftx : sponge_amd_get_fft_parameter(p->cell.a11 / p->opt.grid_spacing);
-    p->fy = p->opt.ffty > 0 ? p->opt.ffty : sponge_amd_get_fft_parameter(p->cell.a22 / p->opt.grid_spacing);
-    p->fz = p->opt.fftz > 0 ? p->opt.fftz : sponge_amd_get_fft_parameter(p->cell.a33 / p->opt.grid_spacing);
+    // grid (PM_force.cpp:383-394): box_length / grid_spacing when not given,
+    // box_length being the lengths of the cell vectors (MD_core.cpp:257-262);
+    // any lower-triangular cell: fractional coordinates, the BC tables and
+    // the gathered forces go through the full rcell
+    const LT6& cl = p->cell;
+    const bool ortho = lt_is_orthorhombic(cl);
+    const float lx = cl.a11;
+    const float ly = ortho ? cl.a22 : sqrtf(cl.a21 * cl.a21 + cl.a22 * cl.a22);
+    const float lz = ortho ? cl.a33 : sqrtf(cl.a31 * cl.a31 + cl.a32 * cl.a32 + cl.a33 * cl.a33);
+    p->fx = p->opt.fftx > 0 ? p->opt.fftx : sponge_amd_get_fft_parameter(lx / p->opt.grid_spacing);
+    p->fy = p->opt.ffty > 0 ? p->opt.ffty : sponge_amd_get_fft_parameter(ly / p->opt.grid_spacing);
+    p->fz = p->opt.fftz > 0 ? p->opt.fftz : sponge_amd_get_fft_parameter(lz / p->opt.grid_spacing);
     const int dims[3] = {p->fx, p->fy, p->fz};
     for (int d = 0; d < 3; ++d) {
         if (dims[d] < 4 || dims[d] > kMaxFftLen || !make_plan(dims[d], &p->plan[d])) {
@@ -1522,7 +1530,6 @@ extern "C" int sponge_pme_grids(sponge_pme* p, float* Q, float* FBCFQ) {
 extern "C" int sponge_pme_update_box(sponge_pme* p, const float* cell, const float* rcell) {
     if (!p || !cell || !rcell) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
     const LT6 c = lt_from(cell);
-    if (!lt_is_orthorhombic(c)) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
     p->cell = c;
     p->rcell = lt_from(rcell);
     p->volume = p->cell.a11 * p->cell.a22 * p->cell.a33;

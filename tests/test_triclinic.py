@@ -9,8 +9,11 @@ its FP32 pair-kernel restatement against an FP64 numpy restatement of the
 same pairs with the general minimum image (SPONGE/utils/boundary.h
 Get_Displacement).  On the GPU the list must equal the brute-force set bit for
 bit, and the pair kernel (energy path and force-only path) must meet the
-force, energy and virial bars against them.  The PME module takes
-orthorhombic boxes only and is not part of this test.
+force, energy and virial bars against them.  The PME reciprocal part takes
+the same cells: the oracle's general-cell form is pinned by the invariance of
+the lattice sum under a change of basis (an orthorhombic lattice written with
+sheared cell vectors, errors shrinking with the grid as PME's own), and the GPU
+must match the oracle on the sheared box.
 """
 import dataclasses
 
@@ -185,3 +188,77 @@ def test_gpu_list_and_pair_kernel_triclinic(dev, tri):
     lj.direct_force(nl, d.crd, d.charge, f2, box, beta)
     torch.cuda.synchronize()
     assert _rel_rms(f2.cpu().numpy(), ref["frc"]) < 1e-5
+
+
+def _with_cell(s, cell):
+    """System s (same coordinates) described with another cell basis."""
+    from sponge_amd import system as S
+    H = np.array([[cell[0], 0, 0], [cell[1], cell[2], 0], [cell[3], cell[4], cell[5]]], np.float64)
+    R = np.linalg.inv(H)
+    rcell = np.array([R[0, 0], R[1, 0], R[1, 1], R[2, 0], R[2, 1], R[2, 2]], np.float32)
+
+    class Cell(S.System):
+        def cell(self):
+            return cell
+
+        def rcell(self):
+            return rcell
+
+    return Cell(**{f.name: getattr(s, f.name) for f in dataclasses.fields(s)}), rcell
+
+
+def test_oracle_pme_basis_invariance():
+    """The same orthorhombic lattice with cell vectors (L,0,0), (L,L,0),
+    (0,0,L): reciprocal energy, forces and virial agree with the orthorhombic
+    description within PME's discretisation error, which shrinks from a 48^3
+    to a 64^3 grid as order-4 interpolation does (a transposed or truncated
+    rcell would differ at O(1))."""
+    from sponge_amd import system as S
+    s = S.config(1)
+    L = s.box.astype(np.float32)
+    beta = O.get_beta(s.cutoff, s.tolerance)
+    ortho, _ = _with_cell(s, np.array([L[0], 0, L[1], 0, 0, L[2]], np.float32))
+    shear, _ = _with_cell(s, np.array([L[0], L[0], L[1], 0, 0, L[2]], np.float32))
+    errs = []
+    for n in (48, 64):
+        a = O.PME(ortho, (n, n, n), beta).reciprocal()
+        b = O.PME(shear, (n, n, n), beta).reciprocal()
+        de = abs(a["reciprocal"] - b["reciprocal"]) / abs(a["reciprocal"])
+        df = _rel_rms(b["frc"], a["frc"])
+        dv = float(np.abs(a["virial"].astype(np.float64) - b["virial"]).max() / np.abs(a["virial"]).max())
+        print(f"grid {n}^3: energy {de:.2e}, forces {df:.2e}, virial {dv:.2e}")
+        errs.append((de, df, dv))
+    assert errs[1][0] < 1e-4 and errs[1][1] < 1e-3 and errs[1][2] < 1e-3
+    assert all(errs[1][k] < 0.6 * errs[0][k] for k in range(3))
+
+
+@pytest.mark.gpu
+def test_gpu_pme_reciprocal_triclinic(dev, tri):
+    import torch
+    from gpu_helpers import DevSystem
+    from sponge_amd import abi
+    s, cell, rcell, beta, c, ser = tri
+    d = DevSystem(s, dev)
+    pm = abi.ParticleMesh(d.N, abi.Box.from_cell(cell, rcell), s.cutoff)
+    print(f"case: {s.name} sheared by {TILT}, PME grid {pm.fft}")
+    # the grid follows the cell vector lengths (the reference's box_length,
+    # grid_spacing 1 A by default): |b| and |c| here, not a22 and a33
+    c32 = cell.astype(np.float32)
+    lb = np.sqrt(c32[1] * c32[1] + c32[2] * c32[2])
+    lc = np.sqrt(c32[3] * c32[3] + c32[4] * c32[4] + c32[5] * c32[5])
+    assert pm.fft == (O.get_fft_parameter(float(c32[0])), O.get_fft_parameter(float(lb)),
+                      O.get_fft_parameter(float(lc)))
+    ref = O.PME(s, pm.fft, beta).reciprocal()
+    frc = d.zeros(d.N, 3)
+    vir = d.zeros(6)
+    pot = d.zeros(1)
+    pm.reciprocal_force(d.crd, d.charge, frc, virial=vir, potential=pot)
+    torch.cuda.synchronize()
+    assert _rel_rms(frc.cpu().numpy(), ref["frc"]) < 1e-5
+    e = pm.step_print()
+    assert abs(e["reciprocal"] - ref["reciprocal"]) < 1e-6 * abs(ref["reciprocal"])
+    assert abs(float(pot.cpu()[0]) - ref["potential"]) < 1e-6 * abs(ref["potential"])
+    v, rv = vir.cpu().numpy().astype(np.float64), ref["virial"].astype(np.float64)
+    dv = float(np.abs(v - rv).max() / np.abs(rv).max())
+    print(f"E_reciprocal {e['reciprocal']:.4f} (oracle {ref['reciprocal']:.4f}), virial {dv:.2e}")
+    assert dv < 1e-6
