@@ -223,8 +223,8 @@ class Box:
     @classmethod
     def from_cell(cls, cell, rcell):
         """A general lower-triangular cell {a11, a21, a22, a31, a32, a33} and
-        its inverse, as given (the neighbor list and the pair kernels take
-        any such cell; the PME module takes orthorhombic ones)."""
+        its inverse, as given (the neighbor list, the pair kernels and the
+        PME module take any such cell)."""
         b = cls.__new__(cls)
         b.cell = np.ascontiguousarray(cell, np.float32).copy()
         b.rcell = np.ascontiguousarray(rcell, np.float32).copy()
