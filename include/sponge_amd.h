@@ -50,8 +50,12 @@ enum sponge_amd_nl_update { SPONGE_AMD_NL_CONDITIONAL = 0, SPONGE_AMD_NL_FORCED 
 
 /* Overflow flags reported by sponge_nl_check_overflow. */
 enum sponge_amd_nl_flag {
-    SPONGE_AMD_NL_GRID_OVERFLOW = 1, /* a cell held more than max_atom_in_grid_numbers */
-    SPONGE_AMD_NL_LIST_OVERFLOW = 2  /* an atom had more than max_neighbor_numbers */
+    SPONGE_AMD_NL_GRID_OVERFLOW = 1,  /* a cell held more than max_atom_in_grid_numbers */
+    SPONGE_AMD_NL_LIST_OVERFLOW = 2,  /* an atom had more than 2 * max_neighbor_numbers
+                                         partners in both directions: its row was cut */
+    SPONGE_AMD_NL_STREAM_OVERFLOW = 4 /* a cell's stencil held more than 65535 atoms: the
+                                         rows of that cell were left empty (no limit of the
+                                         options can lift this; an error in every mode) */
 };
 
 typedef struct sponge_nl sponge_nl;
@@ -129,8 +133,16 @@ int sponge_nl_update(sponge_nl* nl, const int* atom_local, int local_atom_number
 /* NEIGHBOR_LIST::Check_Overflow (neighbor_list.cpp:1027-1103), reporting
  * instead of re-initialising: synchronises the stream, returns the sticky
  * flags (sponge_amd_nl_flag bits) and clears them.  Returns OVERFLOW when a
- * flag was set and throw_error_when_overflow is on. */
+ * flag was set and throw_error_when_overflow is on, and always on
+ * SPONGE_AMD_NL_STREAM_OVERFLOW.  The reference's recovery (re-initialising
+ * with 1.1x the limit that overflowed) is the caller's: the NVE driver and the
+ * C++ facade do it. */
 int sponge_nl_check_overflow(sponge_nl* nl, int* flags);
+
+/* The options the list was initialised with (the public fields
+ * max_neighbor_numbers, max_atom_in_grid_numbers, ... of NEIGHBOR_LIST,
+ * neighbor_list.h:7-127). */
+int sponge_nl_get_options(sponge_nl* nl, sponge_nl_options* opt);
 
 /* The half list in the reference's representation: for local atom i,
  * counts[i] neighbours in serial[i*max_numbers ...] (local ids, j > i by
@@ -452,7 +464,14 @@ int sponge_md_initial(sponge_md** out, int atom_numbers, const float* crd, const
 /* Advances `steps` steps asynchronously.  Energies are evaluated on steps
  * with step % energy_interval == 0 (0 = never).  use_graph != 0 captures the
  * force-only step into one HIP graph on first use and replays it (default
- * schedule only: automatic list refresh, reciprocal part every step). */
+ * schedule only: automatic list refresh, reciprocal part every step).
+ * After every step with (step + 1) % check_overflow_interval == 0 (the
+ * neighbor-list option; <= 0 = never) the list's overflow flags are read
+ * (a host synchronisation), as NEIGHBOR_LIST::Check_Overflow does
+ * (neighbor_list.cpp:1027-1103): on an overflow the list is re-initialised
+ * with 1.1x the limit that overflowed and rebuilt on the next step, or, with
+ * throw_error_when_overflow (or a stream overflow, which no limit lifts), the
+ * call returns SPONGE_AMD_ERROR_OVERFLOW at that step. */
 int sponge_md_run(sponge_md* md, int steps, int energy_interval, int use_graph);
 
 /* Synchronises and reports out[10] = {potential, kinetic, LJ_short, LJ_long,
@@ -494,14 +513,21 @@ int sponge_md_set_reciprocal_overlap(sponge_md* md, int mode);
  * instead of PME's.  The first surrogate step after the PME step of every
  * check_interval-th period (0 = never) is also evaluated with PME; when the
  * prediction's relative RMS error exceeds bound, that step uses PME's force
- * (the fallback).  The surrogate runs on the driver's stream from then on;
- * its feature cutoff must not exceed the driver's cutoff.  rsur = NULL
- * restores plain PME.  Graph replay captures whole cycles of the schedule. */
+ * (the fallback) and the surrogate is switched off: every later step runs
+ * plain PME until this call installs a model again (the driver reads the
+ * fallback counter once per schedule cycle that holds a checked step).  The
+ * surrogate runs on the driver's stream from then on; its feature cutoff must
+ * not exceed the driver's cutoff.  rsur = NULL restores plain PME.  Graph
+ * replay captures whole cycles of the schedule.
+ * The surrogate's force is anchored on the last PME step and is not the
+ * gradient of an energy: this schedule does not conserve energy the way
+ * plain PME (or the reference's impulse schedule) does. */
 int sponge_md_set_surrogate(sponge_md* md, sponge_rsur* rsur, int period, int check_interval,
                             float bound);
 
-/* out[3] = {surrogate steps, checked steps, fallbacks}; *last_error = the
- * relative RMS error of the last checked step (either may be NULL). */
+/* out[4] = {surrogate steps, checked steps, fallbacks, switched off (0/1)};
+ * *last_error = the relative RMS error of the last checked step (either may
+ * be NULL). */
 int sponge_md_surrogate_stats(sponge_md* md, long long* out, double* last_error);
 
 /* Runs n_frames * stride plain steps (eager); the last step of every stride

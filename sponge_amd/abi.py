@@ -68,6 +68,7 @@ SIGNATURES = {
     "sponge_nl_initial": (_I, [C.POINTER(_P), _I, _F, _F, _FP, _FP, C.POINTER(NLOptions), _P]),
     "sponge_nl_update": (_I, [_P, _P, _I, _I, _P, _FP, _FP, _I, _I, _P, _P, _P]),
     "sponge_nl_check_overflow": (_I, [_P, C.POINTER(_I)]),
+    "sponge_nl_get_options": (_I, [_P, C.POINTER(NLOptions)]),
     "sponge_nl_export_half_list": (_I, [_P, _P, _P, _I]),
     "sponge_nl_atom_group": (_I, [_P, C.POINTER(_P)]),
     "sponge_nl_info": (_I, [_P, C.POINTER(C.c_longlong), C.POINTER(_I)]),
@@ -263,6 +264,11 @@ class NeighborList:
         f = C.c_int()
         rc = self.lib.sponge_nl_check_overflow(self.h, C.byref(f))
         return rc, f.value
+
+    def options(self):
+        o = NLOptions()
+        _check(self.lib.sponge_nl_get_options(self.h, C.byref(o)), "sponge_nl_get_options")
+        return {k: getattr(o, k) for k, _ in NLOptions._fields_}
 
     def export_half_list(self, counts, serial, max_numbers):
         _check(self.lib.sponge_nl_export_half_list(self.h, _ptr(counts), _ptr(serial),
@@ -673,21 +679,30 @@ class MDSystem:
         _check(self.lib.sponge_md_set_reciprocal_overlap(self.h, int(mode)),
                "sponge_md_set_reciprocal_overlap")
 
-    def set_surrogate(self, rsur, period=4, check_interval=8, bound=0.05):
+    def set_surrogate(self, rsur, period=4, check_interval=8, bound=0.01):
         _check(self.lib.sponge_md_set_surrogate(self.h, rsur.h if rsur is not None else None,
                                                 int(period), int(check_interval), float(bound)),
                "sponge_md_set_surrogate")
 
     def surrogate_stats(self):
-        out = (C.c_longlong * 3)()
+        out = (C.c_longlong * 4)()
         err = C.c_double()
         _check(self.lib.sponge_md_surrogate_stats(self.h, out, C.byref(err)),
                "sponge_md_surrogate_stats")
-        return dict(surrogate_steps=out[0], checks=out[1], fallbacks=out[2], last_error=err.value)
+        return dict(surrogate_steps=out[0], checks=out[1], fallbacks=out[2],
+                    switched_off=bool(out[3]), last_error=err.value)
 
     def collect_frames(self, rsur, n_frames, stride):
         _check(self.lib.sponge_md_collect_frames(self.h, rsur.h, int(n_frames), int(stride)),
                "sponge_md_collect_frames")
+
+    def nl_options(self):
+        """Options of the driver's current neighbor list (an overflow
+        re-initialises it with grown limits)."""
+        nl, _, _ = self.handles()
+        o = NLOptions()
+        _check(self.lib.sponge_nl_get_options(nl, C.byref(o)), "sponge_nl_get_options")
+        return {k: getattr(o, k) for k, _ in NLOptions._fields_}
 
     def handles(self):
         nl, lj, pm = C.c_void_p(), C.c_void_p(), C.c_void_p()

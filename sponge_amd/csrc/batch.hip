@@ -23,6 +23,7 @@ struct sponge_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int graph_steps = 0;
+    std::vector<int> generation;  // each system's list generation at the capture
 };
 
 namespace {
@@ -68,6 +69,8 @@ int capture(sponge_batch* b, int k) {
     b->graph = g;
     SA_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     b->graph_steps = k;
+    b->generation.clear();
+    for (sponge_md* m : b->md) b->generation.push_back(md_generation(m));
     return 0;
 }
 
@@ -157,8 +160,17 @@ int sponge_batch_run(sponge_batch* b, int steps, int steps_per_launch) {
         SA_CHECK(hipEventRecord(b->after, b->stream));
         for (sponge_md* m : b->md) {
             SA_CHECK(hipStreamWaitEvent(md_stream(m), b->after, 0));
+            const long long first = md_step_count(m);
             md_advance(m, launches * steps_per_launch);
+            // each system's neighbor-list overflow check over the replayed steps
+            const int rc = md_overflow_check(m, first, md_step_count(m) - 1);
+            if (rc) return rc;
         }
+        for (size_t i = 0; i < b->md.size(); ++i)
+            if (md_generation(b->md[i]) != b->generation[i]) {
+                batch_free_graph(b);  // a re-initialised list: capture again
+                break;
+            }
     }
     for (sponge_md* m : b->md)
         if (rest) {

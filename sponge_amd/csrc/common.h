@@ -107,20 +107,22 @@ __device__ __forceinline__ T wave_sum(T v) {
 // Scaled complementary error function erfcx(x) = exp(x^2) erfc(x) on
 // 0 <= x <= kErfcxMax, as t P(t) with t = 1 / (1 + 0.4 x) and P of degree 7,
 // coefficients fitted by iteratively reweighted least squares to minimise the
-// largest relative error (2.9e-8 for the fit, 3.2e-7 evaluated in FP32).
-// The Ewald direct term then needs one exponential, exp(-x^2), for both
-// erfc(x) and its derivative.  The rcp is the hardware reciprocal (1 ulp).
+// largest relative error (2.9e-8 for the fit in FP64, under 4e-7 evaluated in
+// FP32; generator and pins: tests/golden/make_erfcx_fit.py,
+// tests/test_erfcx_fit.py).  The Ewald direct term then needs one
+// exponential, exp(-x^2), for both erfc(x) and its derivative.  The rcp is the
+// hardware reciprocal (1 ulp).
 constexpr float kErfcxMax = 4.5f;
 __device__ __forceinline__ float erfcx_poly(float x) {
     const float t = __builtin_amdgcn_rcpf(1.0f + 0.4f * x);
-    float p = 7.614944875e-02f;
-    p = fmaf(p, t, -4.040236473e-01f);
-    p = fmaf(p, t, 7.048523426e-01f);
-    p = fmaf(p, t, -3.981665671e-01f);
-    p = fmaf(p, t, 4.256525636e-01f);
-    p = fmaf(p, t, 1.327622086e-01f);
-    p = fmaf(p, t, 2.379626185e-01f);
-    p = fmaf(p, t, 2.248110324e-01f);
+    float p = 7.614902268e-02f;
+    p = fmaf(p, t, -4.040216792e-01f);
+    p = fmaf(p, t, 7.048484256e-01f);
+    p = fmaf(p, t, -3.981623732e-01f);
+    p = fmaf(p, t, 4.256499008e-01f);
+    p = fmaf(p, t, 1.327632050e-01f);
+    p = fmaf(p, t, 2.379624228e-01f);
+    p = fmaf(p, t, 2.248110469e-01f);
     return p * t;
 }
 

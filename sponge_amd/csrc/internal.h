@@ -194,7 +194,17 @@ hipStream_t md_stream(const sponge_md* m);
 bool md_graphable(const sponge_md* m);
 int md_record_steps(sponge_md* m, int k);
 void md_advance(sponge_md* m, int k);
+// the driver's overflow check over steps first..last (see sponge_md_run)
+int md_overflow_check(sponge_md* m, long long first, long long last);
+long long md_step_count(const sponge_md* m);
+int md_generation(const sponge_md* m);  // changes when the list is re-initialised
 int md_state_arrays(sponge_md* m, float** crd, float** charge);
+
+// the list's sticky overflow flags, read (synchronising) and cleared
+int nl_take_flags(sponge_nl* nl, int* flags);
+// Check_Overflow's recovery: *nl replaced by a fresh list with 1.1x the
+// limits named by flags (the caller rebuilds anything that held the old one)
+int nl_reinitialize(sponge_nl** nl, int flags);
 
 int nl_pack_for_pairs(sponge_nl* nl, const float* crd, const float* charge, const int* type,
                       float4* pos, int* ptype, int* pgid, hipStream_t s);

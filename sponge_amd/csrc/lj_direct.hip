@@ -324,7 +324,7 @@ __global__ void __launch_bounds__(1024) k_lj_cells(
     const int* __restrict__ list_count, const unsigned short* __restrict__ list, int cap,
     const float4* __restrict__ pos, const int* __restrict__ ptype, const int* __restrict__ pgid,
     const float2* __restrict__ AB_sq, int T, LT6 c, LT6 r, float rc2_thr, float beta, int stage,
-    float gl, int Nx, int Ny, int Nz, float cutoff,
+    float gl, int Nx, int Ny, int Nz, float reach,
     float4* __restrict__ facc, float2* __restrict__ eacc, float* __restrict__ vacc) {
     extern __shared__ float4 cpos[];
     int* ctyp = reinterpret_cast<int*>(cpos + stage);
@@ -335,25 +335,30 @@ __global__ void __launch_bounds__(1024) k_lj_cells(
     const int cell = blockIdx.x;
     const int a0 = start[cell], a1 = start[cell + 1];
     if (a0 == a1) return;
-    // Force-only steps in an orthorhombic box wider than twice (cutoff + half
-    // the cell + 1 A) on every axis: the partners are staged as their images
-    // nearest the cell's centre C.  A pair within the cutoff then has its
-    // minimum image there (every other image is more than L/2 - (rc + h) > 0
-    // further), so the displacement is a plain difference; it differs from
-    // the reference arithmetic only by the rounding of the shifted
-    // coordinates (a few ulp of a coordinate), and pairs beyond the cutoff
-    // stay beyond it.
+    // Force-only steps in an orthorhombic box wider than twice (half the
+    // cell + reach) on every axis, reach = cutoff + the largest distance a row
+    // atom may have moved since the list was built (skin * skin_permit under
+    // the automatic refresh; reach < 0 when the list gives no such bound):
+    // the partners are staged as their images nearest the cell's centre C.
+    // A row atom then lies within h/2 + drift of C and a partner within the
+    // cutoff of it within h/2 + reach of C on each axis, less than L/2, so
+    // that partner's nearest image to C is its minimum image from the row
+    // atom and the displacement is a plain difference.  It differs from the
+    // reference arithmetic only by the rounding of the shifted coordinates
+    // (a few ulp of a coordinate), and pairs beyond the cutoff stay beyond it
+    // (no image is nearer than the minimum image).
     float3 C = make_float3(0.f, 0.f, 0.f);
     bool shift = false;
-    if (FAST && ORTHO) {
+    if (FAST && ORTHO && reach >= 0.f) {
         const int ix = cell / (Ny * Nz), iy = (cell / Nz) % Ny, iz = cell % Nz;
         const float lx = ix * gl, hx = ix == Nx - 1 ? c.a11 : (ix + 1) * gl;
         const float ly = iy * gl, hy = iy == Ny - 1 ? c.a22 : (iy + 1) * gl;
         const float lz = iz * gl, hz = iz == Nz - 1 ? c.a33 : (iz + 1) * gl;
         C = make_float3(0.5f * (lx + hx), 0.5f * (ly + hy), 0.5f * (lz + hz));
-        shift = 0.5f * c.a11 > cutoff + 0.5f * (hx - lx) + 1.f &&
-                0.5f * c.a22 > cutoff + 0.5f * (hy - ly) + 1.f &&
-                0.5f * c.a33 > cutoff + 0.5f * (hz - lz) + 1.f;
+        // 0.01 A of margin against the rounding of C and of the coordinates
+        shift = 0.5f * c.a11 > reach + 0.5f * (hx - lx) + 0.01f &&
+                0.5f * c.a22 > reach + 0.5f * (hy - ly) + 0.01f &&
+                0.5f * c.a33 > reach + 0.5f * (hz - lz) + 0.01f;
     }
     const int nr = nranges[cell];
     for (int k = threadIdx.x; k < nr; k += blockDim.x) r_first[k] = g_rfirst[(size_t)cell * max_ranges + k];
@@ -464,19 +469,27 @@ __global__ void k_sum_f32(int n, const float* __restrict__ a, double* out) {
     if ((threadIdx.x & (kWave - 1)) == 0) unsafeAtomicAdd(out, s);
 }
 
+// allow_shift: the caller updated the list (CONDITIONAL or FORCED) on these
+// coordinates, so every atom is within the refresh bound of its build cell
+// (the step driver); the public entry point makes no such assumption.
 template <bool ORTHO>
 int launch_rows(sponge_lj* lj, sponge_nl* nl, LT6 c, LT6 r, float beta, int need_e, int need_v,
-                hipStream_t s) {
+                hipStream_t s, bool allow_shift = false) {
     ScopedTimer timer("lj_pairs", s);
     const size_t sh = (sizeof(float4) + ((need_e || need_v) ? 2 : 1) * sizeof(int)) * lj->stage +
                       sizeof(float2) * lj->T * lj->T;
     const bool fast = beta * lj->cutoff <= kErfcxMax && !need_e && !need_v;
+    // how far a row atom can be from its build-time cell: the automatic
+    // refresh rebuilds once any atom moved more than skin * skin_permit
+    const float reach = allow_shift && nl->opt.refresh_interval <= 0
+                            ? lj->cutoff + nl->skin * nl->opt.skin_permit
+                            : -1.f;
 #define SA_LJ_ROWS(E, V, F)                                                                         \
     hipLaunchKernelGGL((k_lj_cells<ORTHO, E, V, F>), dim3(nl->G), dim3(1024), sh, s, nl->d_cell_start, \
                        nl->d_nranges, nl->max_ranges, nl->d_rfirst, nl->d_roff, nl->d_list_count,    \
                        nl->d_list, nl->cap, lj->d_pos, lj->d_ptype, lj->d_pgid, lj->d_AB_sq, lj->T,  \
                        c, r, lj->rc2_thr, beta, lj->stage, nl->grid_length, nl->Nx, nl->Ny,       \
-                       nl->Nz, lj->cutoff, lj->d_facc, lj->d_eacc, lj->d_vacc)
+                       nl->Nz, reach, lj->d_facc, lj->d_eacc, lj->d_vacc)
     if (fast) SA_LJ_ROWS(false, false, true);
     else if (need_e && need_v) SA_LJ_ROWS(true, true, false);
     else if (need_e) SA_LJ_ROWS(true, false, false);
@@ -643,9 +656,9 @@ int sa::lj_direct_force_md(sponge_lj* lj, sponge_nl* nl, const float* crd, const
     int rc = nl_pack_for_pairs(nl, crd, charge, lj->d_type, lj->d_pos, lj->d_ptype, lj->d_pgid, s);
     if (rc) return rc;
     if (lt_is_orthorhombic(c))
-        rc = launch_rows<true>(lj, nl, c, r, pme_beta, need_atom_energy, 0, s);
+        rc = launch_rows<true>(lj, nl, c, r, pme_beta, need_atom_energy, 0, s, true);
     else
-        rc = launch_rows<false>(lj, nl, c, r, pme_beta, need_atom_energy, 0, s);
+        rc = launch_rows<false>(lj, nl, c, r, pme_beta, need_atom_energy, 0, s, true);
     if (rc) return rc;
     return finalize(lj, lj->N, nl->d_sidx, need_atom_energy, 0, frc, atom_energy, nullptr,
                     atom_direct_pme_energy, s, 1, reset_flag);

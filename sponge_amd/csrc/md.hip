@@ -63,6 +63,8 @@ struct sponge_md {
     int sur_period = 0, sur_check = 0;
     float sur_bound = 0;
     long long sur_steps = 0;
+    unsigned long long sur_fallbacks_seen = 0;  // fallbacks read back by the host
+    int sur_disabled = 0;          // a check failed: plain PME until the surrogate is set again
     int collect = 0;               // store a training frame on this step
     sponge_rsur* collect_into = nullptr;
     float* d_rec = nullptr;        // [3N] PME reciprocal force of a checked / collected step
@@ -79,6 +81,8 @@ struct sponge_md {
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int overlap_mode = -1;  // 1 on, 0 off, -1 from the atom count
     float* d_exfrc = nullptr;  // [N][3] excluded-pair correction computed beside the pair work
+    int nl_generation = 0;  // bumped when an overflow re-initialises the list
+    int overflow_flags = 0;  // every overflow flag seen by the driver's checks
 };
 
 namespace {
@@ -548,6 +552,55 @@ int md_step(sponge_md* m, long long step, int need_energy, StepKind* kind_out = 
     return 0;
 }
 
+// After steps first..last of the learned schedule: when one of them was a
+// checked step, the fallback counter is read (a host synchronisation); a new
+// fallback switches the surrogate off (plain PME from the next step on) until
+// sponge_md_set_surrogate installs a model again.
+int md_check_surrogate(sponge_md* m, long long first, long long last) {
+    if (!m->rsur || m->sur_check <= 0 || m->sur_period < 2) return 0;
+    bool checked = false;
+    for (long long t = first; t <= last && !checked; ++t)
+        checked = t % m->sur_period == 1 && (t / m->sur_period) % m->sur_check == 0;
+    if (!checked) return 0;
+    unsigned long long cnt[2] = {0, 0};
+    SA_CHECK(hipMemcpyAsync(cnt, m->d_scount, sizeof(cnt), hipMemcpyDeviceToHost, m->stream));
+    SA_CHECK(hipStreamSynchronize(m->stream));
+    if (cnt[1] > m->sur_fallbacks_seen) {
+        m->sur_fallbacks_seen = cnt[1];
+        m->rsur = nullptr;
+        m->sur_disabled = 1;
+        m->anchor_valid = false;
+        capture_destroy(m->capture);
+        m->capture_len = 0;
+    }
+    return 0;
+}
+
+// NEIGHBOR_LIST::Check_Overflow at the reference's cadence (neighbor_list.cpp:
+// 1027-1103), once steps first..last have been issued: when one of them has
+// (step + 1) % check_overflow_interval == 0 the list's flags are read (a host
+// synchronisation).  An overflow re-initialises the list with 1.1x the limit
+// that overflowed (it is rebuilt by the next step's update, and any captured
+// step that held the old list is dropped), or is an error with
+// throw_error_when_overflow or when a cell's stream outgrew the 16-bit rows.
+int md_check_overflow(sponge_md* m, long long first, long long last) {
+    const int iv = m->nl->opt.check_overflow_interval;
+    if (iv <= 0 || (last + 1) / iv * iv < first + 1) return 0;
+    int flags = 0;
+    int rc = nl_take_flags(m->nl, &flags);
+    if (rc || !flags) return rc;
+    m->overflow_flags |= flags;
+    if ((flags & SPONGE_AMD_NL_STREAM_OVERFLOW) || m->nl->opt.throw_error_when_overflow)
+        return SPONGE_AMD_ERROR_OVERFLOW;
+    rc = nl_reinitialize(&m->nl, flags);
+    if (rc) return rc;
+    capture_destroy(m->capture);
+    m->capture_len = 0;
+    m->need_cleared = false;
+    m->nl_generation++;
+    return 0;
+}
+
 }  // namespace
 
 extern "C" int sponge_md_initial(sponge_md** out, int atom_numbers, const float* crd,
@@ -727,20 +780,26 @@ extern "C" int sponge_md_run(sponge_md* m, int steps, int energy_interval, int u
                 }
                 m->capture_len = cyc;
             }
-            const int rc = capture_launch(m->capture, m->stream);
+            int rc = capture_launch(m->capture, m->stream);
             if (rc) return rc;
             for (StepKind kd : m->capture_kinds)
                 if (kd != StepKind::Pme) m->sur_steps++;
             m->step += cyc;
             k += cyc;
+            rc = md_check_overflow(m, st, st + cyc - 1);
+            if (!rc) rc = md_check_surrogate(m, st, st + cyc - 1);
+            if (rc) return rc;
         } else {
             const int need_e = energy_interval > 0 && st % energy_interval == 0;
             StepKind kd = StepKind::Pme;
-            const int rc = md_step(m, st, need_e, &kd);
+            int rc = md_step(m, st, need_e, &kd);
             if (rc) return rc;
             if (kd != StepKind::Pme) m->sur_steps++;
             m->step++;
             k++;
+            rc = md_check_overflow(m, st, st);
+            if (!rc && kd == StepKind::Checked) rc = md_check_surrogate(m, st, st);
+            if (rc) return rc;
         }
     }
     return 0;
@@ -793,6 +852,13 @@ int md_record_steps(sponge_md* m, int k) {
 
 void md_advance(sponge_md* m, int k) { m->step += k; }
 
+int md_overflow_check(sponge_md* m, long long first, long long last) {
+    return md_check_overflow(m, first, last);
+}
+
+long long md_step_count(const sponge_md* m) { return m->step; }
+int md_generation(const sponge_md* m) { return m->nl_generation; }
+
 int md_state_arrays(sponge_md* m, float** crd, float** charge) {
     *crd = m->d_crd;
     *charge = m->d_charge;
@@ -833,6 +899,12 @@ extern "C" int sponge_md_set_surrogate(sponge_md* m, sponge_rsur* rsur, int peri
     if (!rc) rc = ensure_surrogate_buffers(m);
     if (rc) return rc;
     rsur_set_stream(rsur, m->stream);
+    // fallbacks so far, so that only a new one switches the surrogate off
+    unsigned long long cnt[2] = {0, 0};
+    SA_CHECK(hipMemcpyAsync(cnt, m->d_scount, sizeof(cnt), hipMemcpyDeviceToHost, m->stream));
+    SA_CHECK(hipStreamSynchronize(m->stream));
+    m->sur_fallbacks_seen = cnt[1];
+    m->sur_disabled = 0;
     m->rsur = rsur;
     m->anchor_valid = false;
     m->sur_period = period;
@@ -854,6 +926,7 @@ extern "C" int sponge_md_surrogate_stats(sponge_md* m, long long* out, double* l
         out[0] = m->sur_steps;
         out[1] = (long long)cnt[0];
         out[2] = (long long)cnt[1];
+        out[3] = m->sur_disabled;
     }
     if (last_error) *last_error = err;
     return 0;
@@ -873,6 +946,7 @@ extern "C" int sponge_md_collect_frames(sponge_md* m, sponge_rsur* rsur, int n_f
         for (int k = 0; k < stride && !rc; ++k) {
             m->collect = k == stride - 1;
             rc = md_step(m, m->step, 0);
+            if (!rc) rc = md_check_overflow(m, m->step, m->step);
             if (!rc) m->step++;
         }
     }

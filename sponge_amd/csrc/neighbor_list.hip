@@ -245,9 +245,10 @@ __global__ void __launch_bounds__(512) k_nl_search_cells(
     for (int k = threadIdx.x; k <= nr; k += blockDim.x) g_roff[(size_t)cell * (max_ranges + 1) + k] = r_off[k];
     const int a0 = start[cell], a1 = start[cell + 1];
     if (total > 65535) {
-        // positions no longer fit 16 bits: empty rows and a raised flag
+        // positions no longer fit 16 bits: empty rows and a raised flag, which
+        // the step driver turns into an error at its next overflow check
         for (int p = a0 + (int)threadIdx.x; p < a1; p += blockDim.x) list_count[p] = 0;
-        if (threadIdx.x == 0) atomicOr(flags, SPONGE_AMD_NL_GRID_OVERFLOW | SPONGE_AMD_NL_LIST_OVERFLOW);
+        if (threadIdx.x == 0) atomicOr(flags, SPONGE_AMD_NL_STREAM_OVERFLOW | SPONGE_AMD_NL_LIST_OVERFLOW);
         return;
     }
     const int staged = total < stage ? total : stage;
@@ -661,8 +662,7 @@ int sa::nl_update_impl(sponge_nl* nl, const int* atom_local, int local_atom_numb
     return 0;
 }
 
-extern "C" int sponge_nl_check_overflow(sponge_nl* nl, int* flags) {
-    if (!nl) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+int sa::nl_take_flags(sponge_nl* nl, int* flags) {
     int h = 0;
     SA_CHECK(hipMemcpyAsync(&h, nl->d_flags, sizeof(int), hipMemcpyDeviceToHost, nl->stream));
     SA_CHECK(hipStreamSynchronize(nl->stream));
@@ -670,8 +670,44 @@ extern "C" int sponge_nl_check_overflow(sponge_nl* nl, int* flags) {
         int rc = fill_async(nl->d_flags, 0, 1, nl->stream);
         if (rc) return rc;
     }
+    *flags = h;
+    return 0;
+}
+
+extern "C" int sponge_nl_check_overflow(sponge_nl* nl, int* flags) {
+    if (!nl) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    int h = 0;
+    const int rc = nl_take_flags(nl, &h);
+    if (rc) return rc;
     if (flags) *flags = h;
-    if (h && nl->opt.throw_error_when_overflow) return SPONGE_AMD_ERROR_OVERFLOW;
+    if ((h & SPONGE_AMD_NL_STREAM_OVERFLOW) || (h && nl->opt.throw_error_when_overflow))
+        return SPONGE_AMD_ERROR_OVERFLOW;
+    return 0;
+}
+
+extern "C" int sponge_nl_get_options(sponge_nl* nl, sponge_nl_options* opt) {
+    if (!nl || !opt) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    *opt = nl->opt;
+    return 0;
+}
+
+// Check_Overflow's recovery (neighbor_list.cpp:1027-1103): a new list of the
+// same atoms, cutoff, skin and box with 1.1x the limits whose flags were
+// raised; the old one is released.  The new list is built on its first update.
+int sa::nl_reinitialize(sponge_nl** nl, int flags) {
+    sponge_nl* old = *nl;
+    sponge_nl_options o = old->opt;
+    if (flags & SPONGE_AMD_NL_LIST_OVERFLOW)
+        o.max_neighbor_numbers = (int)(o.max_neighbor_numbers * 1.1);
+    if (flags & SPONGE_AMD_NL_GRID_OVERFLOW)
+        o.max_atom_in_grid_numbers = (int)(o.max_atom_in_grid_numbers * 1.1);
+    sponge_nl* fresh = nullptr;
+    SA_CHECK(hipStreamSynchronize(old->stream));
+    const int rc = sponge_nl_initial(&fresh, old->N, old->cutoff, old->skin, &old->cell0.a11,
+                                     &old->rcell0.a11, &o, old->stream);
+    if (rc) return rc;
+    sponge_nl_clear(old);
+    *nl = fresh;
     return 0;
 }
 

@@ -1131,26 +1131,26 @@ extern "C" void sponge_pme_default_options(sponge_pme_options* o) {
 }
 
 extern "C" float sponge_amd_get_beta(float cutoff, float tolerance) {
-    // Get_Beta (PM_force.cpp:246-274): float variables, double erfc
-    float beta = 0.f, low, high, tempf;
-    int ilow, ihigh;
-    high = 1.0;
-    ihigh = 1;
-    while (1) {
-        tempf = erfc(high * cutoff) / cutoff;
-        if (tempf <= tolerance) break;
-        high *= 2;
-        ihigh++;
+    // Get_Beta (PM_force.cpp:246-274).  The direct-space tail erfc(b rc) / rc
+    // falls as b grows: an upper end is found by doubling from b = 1, then the
+    // bracket [0, upper] is halved 50 times more than the doublings needed,
+    // and the last midpoint tried is beta.  The tail is evaluated with the
+    // library erfc and kept in float, as are b and the bracket, so that beta
+    // comes out bit-identical to the reference's.
+    auto tail = [cutoff](float b) -> float { return erfc(b * cutoff) / cutoff; };
+    float upper = 1.0f;
+    int doublings = 0;
+    while (!(tail(upper) <= tolerance)) {
+        upper *= 2.0f;
+        ++doublings;
     }
-    ihigh += 50;
-    low = 0.0;
-    for (ilow = 1; ilow < ihigh; ilow++) {
-        beta = (low + high) / 2;
-        tempf = erfc(beta * cutoff) / cutoff;
-        if (tempf >= tolerance) low = beta;
-        else high = beta;
+    float lower = 0.0f, mid = 0.0f;
+    for (int halving = 0; halving < doublings + 50; ++halving) {
+        mid = (lower + upper) / 2;
+        if (tail(mid) >= tolerance) lower = mid;
+        else upper = mid;
     }
-    return beta;
+    return mid;
 }
 
 static int check_2357(int n) {

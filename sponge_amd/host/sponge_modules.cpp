@@ -7,6 +7,7 @@
 #include <cstring>
 #include <fstream>
 #include <sstream>
+#include <string>
 
 namespace sponge_amd {
 
@@ -186,11 +187,19 @@ void NEIGHBOR_LIST::Initial(CONTROLLER* controller, int atom_numbers_, float cut
         o.throw_error_when_overflow =
             controller->Get_Bool("neighbor_list", "throw_error_when_overflow", false) ? 1 : 0;
     }
+    Initial_With_Options(o, atom_numbers_, cutoff_, skin_, boundary, stream_);
+}
+
+void NEIGHBOR_LIST::Initial_With_Options(const sponge_nl_options& o, int atom_numbers_,
+                                         float cutoff_, float skin_, const Boundary boundary,
+                                         void* stream_) {
     atom_numbers = atom_numbers_;
     cutoff = cutoff_;
     skin = skin_;
     stream = stream_;
+    boundary0 = boundary;
     max_neighbor_numbers = o.max_neighbor_numbers;
+    max_atom_in_grid_numbers = o.max_atom_in_grid_numbers;
     check_overflow_interval = o.check_overflow_interval;
     throw_error_when_overflow = o.throw_error_when_overflow != 0;
     check(sponge_nl_initial(&handle, atom_numbers, cutoff, skin, boundary.cell, boundary.rcell, &o,
@@ -210,18 +219,37 @@ void NEIGHBOR_LIST::Update(int* atom_local, int local_atom_numbers, int ghost_nu
 }
 
 void NEIGHBOR_LIST::Check_Overflow(CONTROLLER* controller, int steps) {
-    (void)controller;
     if (!is_initialized || check_overflow_interval <= 0 || (steps + 1) % check_overflow_interval != 0)
         return;
     int flags = 0;
     const int rc = sponge_nl_check_overflow(handle, &flags);
+    if (rc == SPONGE_AMD_ERROR_OVERFLOW) {
+        Throw_SPONGE_Error(SPONGE_AMD_ERROR_OVERFLOW, "NEIGHBOR_LIST::Check_Overflow",
+                           "Reason:\n\tOverflow occured in neighbor searching");
+        return;
+    }
     if (rc) {
         check(rc, "NEIGHBOR_LIST::Check_Overflow");
         return;
     }
-    if (flags)
-        Throw_SPONGE_Error(SPONGE_AMD_ERROR_OVERFLOW, "NEIGHBOR_LIST::Check_Overflow",
-                           "Reason:\n\tOverflow occured in neighbor searching");
+    if (!flags) return;
+    // the reference's recovery: the module re-initialised with 1.1x the
+    // limit that overflowed, the new limit written back to the commands
+    sponge_nl_options o;
+    check(sponge_nl_get_options(handle, &o), "NEIGHBOR_LIST::Check_Overflow");
+    auto grow = [&](int& v, const char* key) {
+        v = (int)(v * 1.1);
+        if (controller) controller->commands[key] = std::to_string(v);
+        printf("Overflow occured in neighbor searching.\nSPONGE will re-initialize the neighbor "
+               "list module with %s = %d\n\n", key, v);
+    };
+    if (flags & SPONGE_AMD_NL_GRID_OVERFLOW)
+        grow(o.max_atom_in_grid_numbers, "neighbor_list_max_atom_in_grid_numbers");
+    if (flags & SPONGE_AMD_NL_LIST_OVERFLOW)
+        grow(o.max_neighbor_numbers, "neighbor_list_max_neighbor_numbers");
+    const Boundary b = boundary0;
+    Clear();
+    Initial_With_Options(o, atom_numbers, cutoff, skin, b, stream);
 }
 
 void* NEIGHBOR_LIST::Device_List() {

@@ -55,9 +55,11 @@ struct NEIGHBOR_LIST {
     float cutoff = 0, skin = 0;
     int atom_numbers = 0;
     int max_neighbor_numbers = 1200;
+    int max_atom_in_grid_numbers = 150;
     int check_overflow_interval = 150;
     bool throw_error_when_overflow = false;
     void* stream = nullptr;
+    Boundary boundary0{};  // the box of Initial (a re-initialisation reuses it)
 
     // NEIGHBOR_LIST::Initial (neighbor_list.cpp:871-972)
     void Initial(CONTROLLER* controller, int atom_numbers, float cutoff, float skin,
@@ -67,11 +69,18 @@ struct NEIGHBOR_LIST {
                 const Boundary boundary, int step, int update, int* excluded_list_start = nullptr,
                 int* excluded_list = nullptr, int* excluded_numbers = nullptr);
     // NEIGHBOR_LIST::Check_Overflow (neighbor_list.cpp:1027-1103): every
-    // check_overflow_interval steps; an overflow is an error here
+    // check_overflow_interval steps; on an overflow the module is
+    // re-initialised with 1.1x the limit that overflowed (the new value also
+    // written to the controller's commands), or, with
+    // throw_error_when_overflow, the error handler gets spongeErrorOverflow
     void Check_Overflow(CONTROLLER* controller, int steps);
     // the ATOM_GROUP d_nl device array for modules that read the list directly
     void* Device_List();
     void Clear();
+
+   private:
+    void Initial_With_Options(const sponge_nl_options& o, int atom_numbers, float cutoff,
+                              float skin, const Boundary boundary, void* stream);
 };
 
 struct LENNARD_JONES_INFORMATION {

@@ -37,3 +37,17 @@ def test_host_only_helpers_match_reference_procedures():
     for rc, beta in ((8.0, 0.348639), (10.0, 0.275106), (12.0, 0.226635)):
         assert abs(lib.sponge_amd_get_beta(rc, 1e-5) - beta) < 2e-6
     assert [lib.sponge_amd_get_fft_parameter(L) for L in (26.9, 62.1, 99.0)] == [28, 64, 100]
+
+
+def test_get_beta_bitwise_equal_to_oracle_restatement():
+    """The library's Get_Beta gives the oracle restatement's float, bit for
+    bit, over a sweep of cutoffs and tolerances (the same bisection steps)."""
+    import numpy as np
+    from oracle import oracle as O
+    from sponge_amd import abi
+    lib = abi.load()
+    for rc in np.arange(6.0, 14.01, 0.25, dtype=np.float32):
+        for tol in (1e-4, 1e-5, 3e-6, 1e-6, 1e-8):
+            a = np.float32(lib.sponge_amd_get_beta(float(rc), tol))
+            b = np.float32(O.lib().orc_get_beta(float(rc), tol))
+            assert a.tobytes() == b.tobytes(), (rc, tol, a, b)

@@ -115,7 +115,10 @@ def test_mlp_forward_against_oracle(dev, cfg1):
     want = R.predict(p64, mu.astype(np.float64), isd.astype(np.float64), feat, basis, s.charge)
     e_bf, e64 = rel_rms(got, want_bf), rel_rms(got, want)
     print(f"forward: rel rms vs bf16-rounded oracle {e_bf:.2e}, vs FP64 oracle {e64:.2e}")
-    assert e_bf < 1e-3
+    # the oracle with BF16-rounded operands differs from the MFMA path only in
+    # FP32 accumulation order: a wrong fragment layout in any layer would give
+    # errors of order one
+    assert e_bf < 1e-4
     assert e64 < 2e-2
     # a second call accumulates (frc +=)
     rs.predict(frc)
@@ -155,7 +158,7 @@ def test_gradient_against_oracle(dev, cfg1):
         n = int(np.prod(shape))
         e = rel_rms(g[o:o + n], g64[o:o + n])
         print(f"grad {name}: rel rms {e:.2e}")
-        assert e < 5e-2, name
+        assert e < 1e-2, name
         o += n
 
 
@@ -217,17 +220,29 @@ def test_md_learned_schedule(dev, trained):
     err = st1["last_error"]
     print(f"last checked step: rel RMS error {err:.4f}")
     assert 0 < err < 0.5
-    # a bound below the error: every check falls back to PME
+    assert not st1["switched_off"]
+    # a bound below the error: the first check falls back to PME and switches
+    # the surrogate off; every later step is plain PME
     md.set_surrogate(rs, period=4, check_interval=2, bound=1e-9)
     md.run(64, use_graph=False)
     st2 = md.surrogate_stats()
-    assert st2["checks"] - st1["checks"] == 8 and st2["fallbacks"] - st1["fallbacks"] == 8
+    assert st2["checks"] - st1["checks"] == 1 and st2["fallbacks"] - st1["fallbacks"] == 1
+    assert st2["surrogate_steps"] - st1["surrogate_steps"] == 1 and st2["switched_off"]
     # graph replay of whole 8-step cycles
     md.set_surrogate(rs, period=4, check_interval=2, bound=1e9)
+    assert not md.surrogate_stats()["switched_off"]
     md.run(64, use_graph=True)
     st3 = md.surrogate_stats()
     assert st3["surrogate_steps"] - st2["surrogate_steps"] == 48
     assert st3["checks"] - st2["checks"] == 8 and st3["fallbacks"] == st2["fallbacks"]
+    # under graph replay a failed check switches the surrogate off after the
+    # cycle that held it (at most 6 surrogate steps, one of them checked)
+    md.set_surrogate(rs, period=4, check_interval=2, bound=1e-9)
+    md.run(64, use_graph=True)
+    st4 = md.surrogate_stats()
+    assert 1 <= st4["surrogate_steps"] - st3["surrogate_steps"] <= 6
+    assert st4["checks"] - st3["checks"] == 1 and st4["fallbacks"] - st3["fallbacks"] == 1
+    assert st4["switched_off"]
     md.run(1, energy_interval=1)
     e = md.energies()
     assert np.isfinite(e["potential"]) and np.isfinite(e["kinetic"])
