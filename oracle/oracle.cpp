@@ -1400,3 +1400,223 @@ ORC_API int orc_num_threads(void) {
     return 1;
 #endif
 }
+
+// ---------------------------------------------------------------------------
+// NVE leap-frog + SETTLE of rigid waters: one MD_Iteration in the reference's
+// NVE order (SPONGE/main.cpp:1705-1777): remember each triangle's B - A and
+// C - A, leap-frog every atom (MD_Iteration_Leap_Frog, SPONGE/MD_core/
+// nve.hpp:3-16: v += dt F / m, x += dt v), then SETTLE of every triangle
+// (settle_triangle + SETTLE_DO_TRIANGLE, SPONGE/constrain/settle.cpp:100-258)
+// with the NVE factors v_factor = x_factor = 1.
+
+// Canonical triangle of side lengths rAB, rAC, rBC with its centre of mass at
+// the origin, following Get_Rabcde_From_SSS (settle.cpp:51-94): out =
+// {ra, rb, rc, rd, re}, A = (0, ra), B = (rc, rb), C = (rd, re).
+// Pinned by: the side lengths and the centre of mass of the canonical
+// triangle (tests/test_oracle_settle.py).
+ORC_API void orc_settle_geometry(float rAB, float rAC, float rBC, float mA, float mB, float mC,
+                                 float* out) {
+    const float m_sum = mA + mB + mC;
+    // A at the origin, B on the -x axis, C above the x axis (law of cosines)
+    const float cos_a = (rBC * rBC - rAC * rAC - rAB * rAB) / (2 * rAC * rAB);
+    const float sin_a = sqrtf(1.0f - cos_a * cos_a);
+    float px[3] = {0.0f, -rAB, rAC * cos_a};
+    float py[3] = {0.0f, 0.0f, rAC * sin_a};
+    const float ox = (px[1] * mB + px[2] * mC) / m_sum;
+    const float oy = py[2] * mC / m_sum;
+    for (int k = 0; k < 3; ++k) {
+        px[k] -= ox;
+        py[k] -= oy;
+    }
+    // rotate A onto the y axis
+    const float c = 1.0f / sqrtf(1.0f + px[0] * px[0] / py[0] / py[0]);
+    const float s = c * px[0] / py[0];
+    float ra = px[0] * s + py[0] * c;
+    float rc = px[1] * c - py[1] * s, rb = px[1] * s + py[1] * c;
+    float rd = px[2] * c - py[2] * s, re = px[2] * s + py[2] * c;
+    if (ra < 0) {  // reflect y so that A lies on +y
+        ra = -ra;
+        rb = -rb;
+        re = -re;
+    }
+    out[0] = ra;
+    out[1] = rb;
+    out[2] = rc;
+    out[3] = rd;
+    out[4] = re;
+}
+
+namespace {
+inline V3 vcross(V3 a, V3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+inline V3 vunit(V3 a) { return vscale(1.0f / sqrtf(a.x * a.x + a.y * a.y + a.z * a.z), a); }
+// x ex + y ey + z ez, component by component in the reference's order
+inline V3 from_frame(V3 p, V3 ex, V3 ey, V3 ez) {
+    return {p.x * ex.x + p.y * ey.x + p.z * ez.x, p.x * ex.y + p.y * ey.y + p.z * ez.y,
+            p.x * ex.z + p.y * ey.z + p.z * ez.z};
+}
+}  // namespace
+
+// One NVE step in place: crd, vel (float[3N]) from frc (float[3N]) and the
+// inverse masses; waters are (A, B, C) triples of atom ids, g the canonical
+// triangle of orc_settle_geometry, mass the masses SETTLE weights with.
+// Pinned by: the FP64 SHAKE solution of the same constraint equations
+// (orc64_nve_shake), constraint lengths, conservation of each water's linear
+// momentum by the correction (tests/test_oracle_settle.py).
+ORC_API void orc_nve_settle(int N, float* crd, float* vel, const float* frc, const float* inv_mass,
+                            const float* mass, int n_water, const int* water, const float* g,
+                            const float* cell6, const float* rcell6, float dt) {
+    const LT cell = ld_lt(cell6), rcell = ld_lt(rcell6);
+    std::vector<V3> old_b(n_water), old_c(n_water);
+    for (int w = 0; w < n_water; ++w) {  // remember_triangle_BA_CA (settle.cpp:5-25)
+        const int a = water[3 * w], b = water[3 * w + 1], c = water[3 * w + 2];
+        old_b[w] = disp_pbc(ld3(crd, b), ld3(crd, a), cell, rcell);
+        old_c[w] = disp_pbc(ld3(crd, c), ld3(crd, a), cell, rcell);
+    }
+    for (int i = 0; i < N; ++i) {  // MD_Iteration_Leap_Frog
+        const V3 acc = vscale(inv_mass[i], ld3(frc, i));
+        const V3 v = vadd(ld3(vel, i), vscale(dt, acc));
+        const V3 x = vadd(ld3(crd, i), vscale(dt, v));
+        for (int d = 0; d < 3; ++d) {
+            vel[3 * i + d] = (&v.x)[d];
+            crd[3 * i + d] = (&x.x)[d];
+        }
+    }
+    const float ra = g[0], rb = g[1], rc = g[2], rd = g[3], re = g[4];
+    const float v_scale = 1.0f / dt / 1.0f;  // exp_gamma / dt / half_exp_gamma_plus_half
+    for (int w = 0; w < n_water; ++w) {
+        const int ia = water[3 * w], ib = water[3 * w + 1], ic = water[3 * w + 2];
+        const float mA = mass[ia], mB = mass[ib], mC = mass[ic];
+        // settle_triangle: positions relative to the centre of mass, formed
+        // through the absolute position of A as the reference does
+        V3 pa = ld3(crd, ia);
+        V3 pb = disp_pbc(ld3(crd, ib), pa, cell, rcell);
+        V3 pc = disp_pbc(ld3(crd, ic), pa, cell, rcell);
+        const V3 com = vadd(vscale(1.0f / (mA + mB + mC), vadd(vscale(mB, pb), vscale(mC, pc))), pa);
+        pa = vsub(pa, com);
+        pb = vadd(pb, pa);
+        pc = vadd(pc, pa);
+        // SETTLE_DO_TRIANGLE: frame from the old plane and the new A
+        const V3 b0 = old_b[w], c0 = old_c[w];
+        V3 ez = vcross(b0, c0);
+        V3 ex = vcross(pa, ez);
+        V3 ey = vcross(ez, ex);
+        ex = vunit(ex);
+        ey = vunit(ey);
+        ez = vunit(ez);
+        const float b0x = vdot(ex, b0), b0y = vdot(ey, b0);
+        const float c0x = vdot(ex, c0), c0y = vdot(ey, c0);
+        const float a1z = vdot(ez, pa);
+        const V3 b1 = {vdot(ex, pb), vdot(ey, pb), vdot(ez, pb)};
+        const V3 c1 = {vdot(ex, pc), vdot(ey, pc), vdot(ez, pc)};
+        const float sin_phi = a1z / ra;
+        const float cos_phi = sqrtf(1.0f - sin_phi * sin_phi);
+        const float sin_psi = (b1.z - c1.z - (rb - re) * sin_phi) / ((rd - rc) * cos_phi);
+        const float cos_psi = sqrtf(1.0f - sin_psi * sin_psi);
+        const V3 a2 = {0.0f, ra * cos_phi, a1z};
+        const V3 b2 = {rc * cos_psi, rb * cos_phi + rc * sin_psi * sin_phi, b1.z};
+        const V3 c2 = {rd * cos_psi, re * cos_phi + rd * sin_psi * sin_phi, c1.z};
+        const float alpha = b2.x * b0x + c2.x * c0x + b2.y * b0y + c2.y * c0y;
+        const float beta = -b2.x * b0y - c2.x * c0y + b2.y * b0x + c2.y * c0x;
+        const float gamma = b1.y * b0x - b1.x * b0y + c1.y * c0x - c1.x * c0y;
+        const float ab2 = alpha * alpha + beta * beta;
+        const float sin_t = (alpha * gamma - beta * sqrtf(ab2 - gamma * gamma)) / ab2;
+        const float cos_t = sqrtf(1.0f - sin_t * sin_t);
+        const V3 a3 = {-a2.y * sin_t, a2.y * cos_t, a2.z};
+        const V3 b3 = {b2.x * cos_t - b2.y * sin_t, b2.x * sin_t + b2.y * cos_t, b2.z};
+        const V3 c3 = {c2.x * cos_t - c2.y * sin_t, c2.x * sin_t + c2.y * cos_t, c2.z};
+        const V3 wa = from_frame(a3, ex, ey, ez);
+        const V3 wb = from_frame(b3, ex, ey, ez);
+        const V3 wc = from_frame(c3, ex, ey, ez);
+        const V3 dva = vscale(v_scale, vsub(wa, pa));
+        const V3 dvb = vscale(v_scale, vsub(wb, pb));
+        const V3 dvc = vscale(v_scale, vsub(wc, pc));
+        const V3 na = vadd(wa, com), nb = vadd(wb, com), nc = vadd(wc, com);
+        const V3 va = vadd(ld3(vel, ia), dva), vb = vadd(ld3(vel, ib), dvb),
+                 vc = vadd(ld3(vel, ic), dvc);
+        for (int d = 0; d < 3; ++d) {
+            crd[3 * ia + d] = (&na.x)[d];
+            crd[3 * ib + d] = (&nb.x)[d];
+            crd[3 * ic + d] = (&nc.x)[d];
+            vel[3 * ia + d] = (&va.x)[d];
+            vel[3 * ib + d] = (&vb.x)[d];
+            vel[3 * ic + d] = (&vc.x)[d];
+        }
+    }
+}
+
+// FP64 oracle of the same step: leap-frog, then the constraint equations of
+// each water (|A-B| = dAB, |A-C| = dAC, |B-C| = dBC with the correction of
+// each atom along the old bond vectors, weighted by the inverse masses --
+// the equations SETTLE solves in closed form) solved by SHAKE iteration to
+// a relative tolerance tol; velocities corrected by the displacement / dt.
+// Returns the largest iteration count used (max_iter when not converged).
+// Pinned by: exact constraint lengths after the step and the closed form of
+// a force-free rotating water (tests/test_oracle_settle.py).
+ORC_API int orc64_nve_shake(int N, double* crd, double* vel, const double* frc,
+                            const double* inv_mass, int n_water, const int* water, double dAB,
+                            double dAC, double dBC, const double* box, double dt, int max_iter,
+                            double tol) {
+    auto mimg = [&](double d, int k) { return d - box[k] * std::floor(d / box[k] + 0.5); };
+    std::vector<double> old(9 * (size_t)n_water);
+    for (int w = 0; w < n_water; ++w) {
+        const int id[3] = {water[3 * w], water[3 * w + 1], water[3 * w + 2]};
+        // old bond vectors A-B, A-C, B-C
+        for (int k = 0; k < 3; ++k) {
+            const double ab = mimg(crd[3 * id[0] + k] - crd[3 * id[1] + k], k);
+            const double ac = mimg(crd[3 * id[0] + k] - crd[3 * id[2] + k], k);
+            old[9 * w + k] = ab;
+            old[9 * w + 3 + k] = ac;
+            old[9 * w + 6 + k] = ac - ab;  // B - C = (A - C) - (A - B)
+        }
+    }
+    for (int i = 0; i < N; ++i)
+        for (int k = 0; k < 3; ++k) {
+            vel[3 * i + k] += dt * inv_mass[i] * frc[3 * i + k];
+            crd[3 * i + k] += dt * vel[3 * i + k];
+        }
+    int worst = 0;
+    for (int w = 0; w < n_water; ++w) {
+        const int id[3] = {water[3 * w], water[3 * w + 1], water[3 * w + 2]};
+        // unwrapped positions relative to A's
+        double p[3][3], p0[3][3];
+        for (int k = 0; k < 3; ++k) {
+            p[0][k] = crd[3 * id[0] + k];
+            p[1][k] = p[0][k] + mimg(crd[3 * id[1] + k] - p[0][k], k);
+            p[2][k] = p[0][k] + mimg(crd[3 * id[2] + k] - p[0][k], k);
+        }
+        std::memcpy(p0, p, sizeof(p));
+        const int ca[3] = {0, 0, 1}, cb[3] = {1, 2, 2};
+        const double d2[3] = {dAB * dAB, dAC * dAC, dBC * dBC};
+        int it = 0;
+        for (; it < max_iter; ++it) {
+            double worst_rel = 0;
+            for (int c = 0; c < 3; ++c) {
+                const int a = ca[c], b = cb[c];
+                const double* o = &old[9 * w + 3 * c];
+                double r[3], r2 = 0, ro = 0;
+                for (int k = 0; k < 3; ++k) {
+                    r[k] = p[a][k] - p[b][k];
+                    r2 += r[k] * r[k];
+                    ro += r[k] * o[k];
+                }
+                worst_rel = std::max(worst_rel, std::fabs(r2 - d2[c]) / d2[c]);
+                const double ima = inv_mass[id[a]], imb = inv_mass[id[b]];
+                const double gg = (d2[c] - r2) / (2.0 * (ima + imb) * ro);
+                for (int k = 0; k < 3; ++k) {
+                    p[a][k] += gg * ima * o[k];
+                    p[b][k] -= gg * imb * o[k];
+                }
+            }
+            if (worst_rel < tol) break;
+        }
+        worst = std::max(worst, it);
+        for (int a = 0; a < 3; ++a)
+            for (int k = 0; k < 3; ++k) {
+                vel[3 * id[a] + k] += (p[a][k] - p0[a][k]) / dt;
+                crd[3 * id[a] + k] = p[a][k];
+            }
+    }
+    return worst;
+}

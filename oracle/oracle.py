@@ -78,6 +78,13 @@ def _load(name):
     lib.orc_sum_f32.restype = C.c_double
     lib.orc_sum_f32.argtypes = [_f32p, C.c_int]
     lib.orc_num_threads.restype = C.c_int
+    lib.orc_settle_geometry.argtypes = [C.c_float] * 6 + [_f32p]
+    lib.orc_nve_settle.argtypes = [C.c_int, _f32p, _f32p, _f32p, _f32p, _f32p, C.c_int, _i32p,
+                                   _f32p, _f32p, _f32p, C.c_float]
+    lib.orc64_nve_shake.restype = C.c_int
+    lib.orc64_nve_shake.argtypes = [C.c_int, _f64p, _f64p, _f64p, _f64p, C.c_int, _i32p,
+                                    C.c_double, C.c_double, C.c_double, _f64p, C.c_double,
+                                    C.c_int, C.c_double]
     return lib
 
 
@@ -345,3 +352,58 @@ def pme64(crd, charge, box, beta, fft):
     lib().orc64_pme(N, crd.reshape(-1), f64(charge), f64(box), float(beta), *[int(x) for x in fft],
                     frc, C.byref(e), vir)
     return dict(frc=frc.reshape(N, 3), reciprocal=e.value, virial=vir)
+
+
+TIME_CONVERSION = 20.455  # ps -> internal time units, SPONGE/common.h:138
+
+
+def water_triples(sys_):
+    """(A, B, C) = (O, H1, H2) atom ids of every water (the waters come first)."""
+    return np.arange(3 * sys_.n_waters, dtype=np.int32).reshape(-1, 3)
+
+
+def water_sides(sys_, crd=None):
+    """Side lengths |OH1|, |OH2|, |H1H2| of the first water, in double (the
+    step driver's rigid geometry comes from the same coordinates)."""
+    crd = np.asarray(sys_.crd if crd is None else crd, np.float64).reshape(-1, 3)
+    a, b, c = crd[0], crd[1], crd[2]
+    return (float(np.linalg.norm(a - b)), float(np.linalg.norm(a - c)),
+            float(np.linalg.norm(b - c)))
+
+
+def settle_geometry(sides, masses):
+    g = np.zeros(5, np.float32)
+    lib().orc_settle_geometry(*[float(x) for x in sides], *[float(m) for m in masses], g)
+    return g
+
+
+def nve_settle(sys_, crd, vel, frc, dt_ps=None, sides=None):
+    """One NVE leap-frog + SETTLE step in the reference's FP32 arithmetic:
+    returns the new (crd, vel)."""
+    N = sys_.n_atoms
+    dt = np.float32((sys_.dt_ps if dt_ps is None else dt_ps) * np.float32(TIME_CONVERSION))
+    m = f32(sys_.mass)
+    im = np.where(m > 0, np.float32(1.0) / np.where(m > 0, m, 1), 0).astype(np.float32)
+    w = water_triples(sys_)
+    g = settle_geometry(sides or water_sides(sys_), m[:3] if len(w) else (16.0, 1.008, 1.008))
+    c = f32(crd).reshape(-1).copy()
+    v = f32(vel).reshape(-1).copy()
+    lib().orc_nve_settle(N, c, v, f32(frc).reshape(-1), im, m, len(w), i32(w).reshape(-1), g,
+                         sys_.cell(), sys_.rcell(), float(dt))
+    return c.reshape(N, 3), v.reshape(N, 3)
+
+
+def nve_shake64(sys_, crd, vel, frc, dt_ps=None, sides=None, max_iter=500, tol=1e-13):
+    """FP64 leap-frog + SHAKE-iterated constraints of the same step: returns
+    (crd, vel, iterations)."""
+    N = sys_.n_atoms
+    dt = (sys_.dt_ps if dt_ps is None else dt_ps) * TIME_CONVERSION
+    m = np.asarray(sys_.mass, np.float64)
+    im = np.where(m > 0, 1.0 / np.where(m > 0, m, 1), 0.0)
+    w = water_triples(sys_)
+    dab, dac, dbc = sides or water_sides(sys_)
+    c = f64(crd).reshape(-1).copy()
+    v = f64(vel).reshape(-1).copy()
+    it = lib().orc64_nve_shake(N, c, v, f64(frc).reshape(-1), im, len(w), i32(w).reshape(-1),
+                               dab, dac, dbc, f64(sys_.box), dt, max_iter, tol)
+    return c.reshape(N, 3), v.reshape(N, 3), it

@@ -292,6 +292,43 @@ def test_pme_reciprocal(dev, cfg1, pme_ref):
     check_total("virial_reciprocal", vir.cpu().numpy(), ref["virial"], ref64["virial"])
 
 
+@pytest.mark.parametrize("update_interval", [2, 4])
+def test_pme_impulse_update_interval(dev, cfg1, update_interval):
+    """PME update_interval (the reference's impulse schedule, PM_force.cpp:
+    1014-1022, 1082, 1135-1140): on steps with step % update_interval == 0 the
+    force gets update_interval x F_reciprocal, on the others nothing; the
+    energy of a skipped step is the last evaluation's.  Each step is given
+    other coordinates, so a skipped step that recomputed would show."""
+    import torch
+    from oracle import oracle as O
+    from sponge_amd import abi
+    s = cfg1
+    ui = update_interval
+    beta = O.get_beta(s.cutoff, s.tolerance)
+    p = O.PME(s, s.fft, beta)
+    plain = O.PME(s, s.fft, beta)
+    box = abi.Box(s.box)
+    pm = abi.ParticleMesh(s.n_atoms, box, s.cutoff, fftx=s.fft[0], ffty=s.fft[1], fftz=s.fft[2],
+                          update_interval=ui)
+    for step in range(ui + 2):
+        crd = _perturbed(s, 100 + step, 0.05)
+        d = DevSystem(s, dev, crd)
+        frc = d.zeros(d.N, 3)
+        pot = d.zeros(1)
+        pm.reciprocal_force(d.crd, d.charge, frc, step=step, potential=pot)
+        torch.cuda.synchronize()
+        ref = p.reciprocal(crd=crd, need_virial=False, step=step, update_interval=ui)
+        f = frc.cpu().numpy()
+        print(f"case: update_interval {ui} step {step}")
+        if step % ui == 0:
+            assert rel_rms(f, ref["frc"]) < FORCE_TOL
+            once = plain.reciprocal(crd=crd, need_energy=False, need_virial=False)["frc"]
+            assert rel_rms(f, ui * once.astype(np.float64)) < FORCE_TOL
+        else:
+            assert not ref["frc"].any() and not f.any()
+        assert rel(float(pot.cpu()[0]), ref["potential"]) < 1e-6
+
+
 def test_pme_spread_dense_bins_and_repeat(dev, cfg1):
     """All atoms packed into a 2.5 A cube: the spread's per-brick bins
     overflow and the overflow list carries the rest.  A second evaluation

@@ -227,7 +227,13 @@ def test_md_learned_schedule(dev, trained):
     md.run(64, use_graph=False)
     st2 = md.surrogate_stats()
     assert st2["checks"] - st1["checks"] == 1 and st2["fallbacks"] - st1["fallbacks"] == 1
-    assert st2["surrogate_steps"] - st1["surrogate_steps"] == 1 and st2["switched_off"]
+    # the checked step comes first or after one unchecked period, depending on
+    # the phase of the step counter: at most 3 unchecked steps before it
+    assert 1 <= st2["surrogate_steps"] - st1["surrogate_steps"] <= 4 and st2["switched_off"]
+    md.run(32, use_graph=False)
+    st2b = md.surrogate_stats()
+    assert st2b["surrogate_steps"] == st2["surrogate_steps"] and st2b["checks"] == st2["checks"]
+    st2 = st2b
     # graph replay of whole 8-step cycles
     md.set_surrogate(rs, period=4, check_interval=2, bound=1e9)
     assert not md.surrogate_stats()["switched_off"]
@@ -236,13 +242,17 @@ def test_md_learned_schedule(dev, trained):
     assert st3["surrogate_steps"] - st2["surrogate_steps"] == 48
     assert st3["checks"] - st2["checks"] == 8 and st3["fallbacks"] == st2["fallbacks"]
     # under graph replay a failed check switches the surrogate off after the
-    # cycle that held it (at most 6 surrogate steps, one of them checked)
+    # cycle that held it: up to 3 unchecked eager steps until the step counter
+    # reaches a cycle boundary, then one captured cycle of 6 surrogate steps,
+    # one of them checked
     md.set_surrogate(rs, period=4, check_interval=2, bound=1e-9)
     md.run(64, use_graph=True)
     st4 = md.surrogate_stats()
-    assert 1 <= st4["surrogate_steps"] - st3["surrogate_steps"] <= 6
+    assert 1 <= st4["surrogate_steps"] - st3["surrogate_steps"] <= 9
     assert st4["checks"] - st3["checks"] == 1 and st4["fallbacks"] - st3["fallbacks"] == 1
     assert st4["switched_off"]
+    md.run(32, use_graph=True)
+    assert md.surrogate_stats()["surrogate_steps"] == st4["surrogate_steps"]
     md.run(1, energy_interval=1)
     e = md.energies()
     assert np.isfinite(e["potential"]) and np.isfinite(e["kinetic"])
