@@ -204,6 +204,31 @@ int sponge_lj_pme_direct_force_list(sponge_lj* lj, const int* counts, const int*
                                     float* atom_energy, int need_virial, float* atom_virial,
                                     float* atom_direct_pme_energy);
 
+/* Allocation-free form of sponge_lj_pme_direct_force_list.  It has no
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No module handle: the LJ tables are the caller's device
+ * arrays (d_LJ_A, d_LJ_B of T(T+1)/2 entries in the in-memory convention,
+ * d_atom_LJ_type of atom_numbers entries), and the pair kernel's working
+ * arrays live in a fixed device array of the library (one per GPU, ~17 MB,
+ * held until the process ends).  The call allocates nothing and frees nothing.
+ * Outputs as sponge_lj_pme_direct_force_list: frc, atom_energy, atom_virial
+ * accumulated; atom_LJ_energy and atom_direct_pme_energy (either may be NULL)
+ * overwritten when need_atom_energy.  Limits: atom_numbers <=
+ * SPONGE_AMD_NOALLOC_MAX_ATOMS and atom_type_numbers <=
+ * SPONGE_AMD_NOALLOC_MAX_TYPES, else SPONGE_AMD_ERROR_OVERFLOW.  Every call
+ * on one GPU shares that GPU's working array: calls that overlap on one GPU
+ * (several streams or host threads) race and must be serialised by the
+ * caller.  Asynchronous on `stream` (NULL = the default stream). */
+#define SPONGE_AMD_NOALLOC_MAX_ATOMS 262144
+#define SPONGE_AMD_NOALLOC_MAX_TYPES 64
+int sponge_lj_pme_direct_force_list_noalloc(
+    int atom_numbers, int atom_type_numbers, const float* d_LJ_A, const float* d_LJ_B,
+    const int* d_atom_LJ_type, float cutoff, const int* counts, const int* serial,
+    int max_numbers, const float* crd, const float* charge, float* frc, const float* cell,
+    const float* rcell, float pme_beta, int need_atom_energy, float* atom_energy, int need_virial,
+    float* atom_virial, float* atom_LJ_energy, float* atom_direct_pme_energy, void* stream);
+
 /* Long_Range_Correction (Lennard_Jones_force.cpp:287-308): d_potential[0] +=
  * factor/volume; diagonal of d_virial[0] += 2*factor/volume. */
 int sponge_lj_long_range_correction(sponge_lj* lj, int need_pressure, float* d_virial,

@@ -80,6 +80,9 @@ SIGNATURES = {
                                         _I, _P, _P]),
     "sponge_lj_pme_direct_force_list": (_I, [_P, _P, _P, _I, _I, _P, _P, _P, _FP, _FP, _F, _I, _P,
                                              _I, _P, _P]),
+    "sponge_lj_pme_direct_force_list_noalloc": (_I, [_I, _I, _P, _P, _P, _F, _P, _P, _I, _P, _P,
+                                                     _P, _FP, _FP, _F, _I, _P, _I, _P, _P, _P,
+                                                     _P]),
     "sponge_lj_long_range_correction": (_I, [_P, _I, _P, _I, _P, _F]),
     "sponge_lj_step_print": (_I, [_P, _FP, _FP]),
     "sponge_lj_atom_energy": (_P, [_P]),
@@ -716,3 +719,23 @@ class MDSystem:
             self.h = None
 
     __del__ = clear
+
+
+NOALLOC_MAX_ATOMS = 262144
+NOALLOC_MAX_TYPES = 64
+
+
+def lj_direct_force_list_noalloc(n_types, lj_A, lj_B, lj_type, cutoff, counts, serial,
+                                 max_numbers, crd, charge, frc, box, beta, atom_energy=None,
+                                 atom_virial=None, lj_energy=None, direct_energy=None,
+                                 stream=None):
+    """sponge_lj_pme_direct_force_list_noalloc: the direct force over a
+    reference-format half list with no module handle and no device
+    allocation (every argument array is a caller-owned device tensor)."""
+    lib = load()
+    _check(lib.sponge_lj_pme_direct_force_list_noalloc(
+        int(crd.shape[0]), int(n_types), _ptr(lj_A), _ptr(lj_B), _ptr(lj_type), float(cutoff),
+        _ptr(counts), _ptr(serial), int(max_numbers), _ptr(crd), _ptr(charge), _ptr(frc),
+        *box.args(), float(beta), int(atom_energy is not None), _ptr(atom_energy),
+        int(atom_virial is not None), _ptr(atom_virial), _ptr(lj_energy), _ptr(direct_energy),
+        _stream_ptr(stream)), "sponge_lj_pme_direct_force_list_noalloc")

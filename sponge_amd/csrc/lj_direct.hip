@@ -21,6 +21,7 @@
 // (no atomics); a final pass scatters the rows into the caller's AoS arrays.
 // The kernel over a caller-provided reference-format half list (k_lj_pairs)
 // keeps the reference's column updates, as hardware float atomics.
+#include <mutex>
 #include <vector>
 
 #include "internal.h"
@@ -766,5 +767,134 @@ extern "C" int sponge_lj_clear(sponge_lj* lj) {
     (void)hipStreamSynchronize(lj->stream);
     lj_free(lj);
     delete lj;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free form of sponge_lj_pme_direct_force_list for callers that
+// cannot allocate device memory (code under HIP graph capture, serving loops):
+// no module handle, the LJ tables come from the caller's device arrays, and
+// the pair kernel's working arrays live in one file-scope device array,
+// `scratch`, of a fixed maximum size.  Each GPU holds its own copy for the
+// life of the process; its device address is taken once per device with
+// hipGetSymbolAddress (the array's name is a host-side stand-in, never a
+// device address).  Every call on a GPU shares that GPU's array: calls that
+// overlap on one GPU (other streams, other host threads) race.
+namespace {
+
+constexpr int kNoAllocMaxAtoms = SPONGE_AMD_NOALLOC_MAX_ATOMS;
+constexpr int kNoAllocMaxTypes = SPONGE_AMD_NOALLOC_MAX_TYPES;
+constexpr int kNoAllocMaxPairs = kNoAllocMaxTypes * (kNoAllocMaxTypes + 1) / 2;
+constexpr int kMaxDevices = 64;
+
+// byte offsets of the working arrays (each 256-byte aligned)
+constexpr size_t kNaAlign = 256;
+constexpr size_t na_round(size_t b) { return (b + kNaAlign - 1) / kNaAlign * kNaAlign; }
+constexpr size_t kNaPos = 0;                                                    // float4[N]
+constexpr size_t kNaFacc = kNaPos + na_round(sizeof(float4) * kNoAllocMaxAtoms);  // float4[N]
+constexpr size_t kNaEacc = kNaFacc + na_round(sizeof(float4) * kNoAllocMaxAtoms); // float2[N]
+constexpr size_t kNaVacc = kNaEacc + na_round(sizeof(float2) * kNoAllocMaxAtoms); // float[6N]
+constexpr size_t kNaType = kNaVacc + na_round(sizeof(float) * 6 * kNoAllocMaxAtoms); // int[N]
+constexpr size_t kNaAB = kNaType + na_round(sizeof(int) * kNoAllocMaxAtoms);      // float2[P]
+constexpr size_t kNaBytes = kNaAB + na_round(sizeof(float2) * kNoAllocMaxPairs);
+
+__device__ __align__(256) unsigned char scratch[kNaBytes];
+
+std::mutex g_na_mutex;
+void* g_na_ptr[kMaxDevices] = {};
+
+// The device address of this device's scratch, looked up on the first call
+// on the device.
+int noalloc_scratch(unsigned char** out) {
+    int dev = 0;
+    SA_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return SPONGE_AMD_ERROR_NOT_IMPLEMENTED;
+    std::lock_guard<std::mutex> lock(g_na_mutex);
+    if (!g_na_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return (int)e;
+        if (!p) return SPONGE_AMD_ERROR_MALLOC_FAILED;
+        g_na_ptr[dev] = p;
+    }
+    *out = static_cast<unsigned char*>(g_na_ptr[dev]);
+    return 0;
+}
+
+// Interleaved (A, B) table of the caller's two device arrays.
+__global__ void k_lj_interleave_ab(int P, const float* __restrict__ A, const float* __restrict__ B,
+                                   float2* __restrict__ AB) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < P) AB[k] = make_float2(A[k], B[k]);
+}
+
+}  // namespace
+
+extern "C" int sponge_lj_pme_direct_force_list_noalloc(
+    int atom_numbers, int atom_type_numbers, const float* d_LJ_A, const float* d_LJ_B,
+    const int* d_atom_LJ_type, float cutoff, const int* counts, const int* serial,
+    int max_numbers, const float* crd, const float* charge, float* frc, const float* cell,
+    const float* rcell, float pme_beta, int need_atom_energy, float* atom_energy, int need_virial,
+    float* atom_virial, float* atom_LJ_energy, float* atom_direct_pme_energy, void* stream) {
+    if (!d_LJ_A || !d_LJ_B || !d_atom_LJ_type || !counts || !serial || !crd || !charge || !frc ||
+        !cell || !rcell || max_numbers <= 0 || atom_numbers < 0 || atom_type_numbers <= 0 ||
+        !(cutoff > 0))
+        return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    if ((need_atom_energy && !atom_energy) || (need_virial && !atom_virial))
+        return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    // sizes beyond the fixed scratch
+    if (atom_numbers > kNoAllocMaxAtoms || atom_type_numbers > kNoAllocMaxTypes)
+        return SPONGE_AMD_ERROR_OVERFLOW;
+    if (atom_numbers == 0) return 0;
+    unsigned char* base = nullptr;
+    int rc = noalloc_scratch(&base);
+    if (rc) return rc;
+    const int N = atom_numbers;
+    const int P = atom_type_numbers * (atom_type_numbers + 1) / 2;
+    float4* pos = reinterpret_cast<float4*>(base + kNaPos);
+    float4* facc = reinterpret_cast<float4*>(base + kNaFacc);
+    float2* eacc = reinterpret_cast<float2*>(base + kNaEacc);
+    float* vacc = reinterpret_cast<float*>(base + kNaVacc);
+    int* ptype = reinterpret_cast<int*>(base + kNaType);
+    float2* AB = reinterpret_cast<float2*>(base + kNaAB);
+    hipStream_t s = (hipStream_t)stream;
+    const LT6 c = lt_from(cell), r = lt_from(rcell);
+    const float rc2_thr = sqrt_threshold(cutoff);
+    hipLaunchKernelGGL(k_lj_interleave_ab, dim3(grid_for(P, 256)), dim3(256), 0, s, P, d_LJ_A,
+                       d_LJ_B, AB);
+    SA_LAUNCH_CHECK();
+    // positions, types and cleared force accumulators (every row written)
+    hipLaunchKernelGGL(k_lj_pack_local, dim3(grid_for(N, 256)), dim3(256), 0, s, N, crd, charge,
+                       d_atom_LJ_type, pos, ptype, facc);
+    SA_LAUNCH_CHECK();
+    const bool ortho = lt_is_orthorhombic(c);
+    dim3 grid(grid_for(N, 4)), block(256);
+#define SA_NA_PAIRS(O, E, V)                                                                  \
+    hipLaunchKernelGGL((k_lj_pairs<O, E, V>), grid, block, 0, s, N, counts, serial, max_numbers, \
+                       pos, ptype, AB, c, r, rc2_thr, pme_beta, facc, eacc, vacc)
+#define SA_NA_PAIRS_EV(O)                                       \
+    if (need_atom_energy && need_virial) SA_NA_PAIRS(O, true, true); \
+    else if (need_atom_energy) SA_NA_PAIRS(O, true, false);     \
+    else if (need_virial) SA_NA_PAIRS(O, false, true);          \
+    else SA_NA_PAIRS(O, false, false)
+    if (ortho) {
+        SA_NA_PAIRS_EV(true);
+    } else {
+        SA_NA_PAIRS_EV(false);
+    }
+#undef SA_NA_PAIRS_EV
+#undef SA_NA_PAIRS
+    SA_LAUNCH_CHECK();
+    dim3 fgrid(grid_for(N, 256));
+#define SA_NA_FIN(E, V)                                                                          \
+    hipLaunchKernelGGL((k_lj_finalize<E, V>), fgrid, block, 0, s, N, nullptr, facc, eacc, vacc, \
+                       frc, atom_energy, atom_LJ_energy, atom_direct_pme_energy, atom_virial, 0,  \
+                       nullptr)
+    if (need_atom_energy && need_virial) SA_NA_FIN(true, true);
+    else if (need_atom_energy) SA_NA_FIN(true, false);
+    else if (need_virial) SA_NA_FIN(false, true);
+    else SA_NA_FIN(false, false);
+#undef SA_NA_FIN
+    SA_LAUNCH_CHECK();
     return 0;
 }
