@@ -185,8 +185,9 @@ int sponge_lj_long_range_factor(sponge_lj* lj, float* factor);
  * 326-384) over the list held by nl.  frc, atom_energy, atom_virial are
  * accumulated; atom_direct_pme_energy (PM's d_direct_atom_energy) and the
  * module's own LJ energy array are overwritten when need_atom_energy.
- * solvent_numbers must be 0 (the NVE configurations run without the
- * solvent_LJ specialisation). */
+ * With solvent_numbers > 0 the last solvent_numbers atoms are the solvent
+ * tail (Lennard_Jones_force.cpp:55-58): only the pairs with a member below it
+ * are computed here, the tail's own pairs by sponge_solvent_lj_direct_force. */
 int sponge_lj_pme_direct_force(sponge_lj* lj, sponge_nl* nl, int atom_numbers,
                                int local_atom_numbers, int solvent_numbers, int ghost_numbers,
                                const float* crd, const float* charge, float* frc,
@@ -243,6 +244,53 @@ int sponge_lj_step_print(sponge_lj* lj, float* LJ_short, float* LJ_long);
 float* sponge_lj_atom_energy(sponge_lj* lj);
 
 int sponge_lj_clear(sponge_lj* lj);
+
+/* ------------------------------------------------------------------------- */
+/* solvent_LJ: SOLVENT_LENNARD_JONES                                          */
+/* (SPONGE/Lennard_Jones_force/solvent_LJ.h:9-50)                             */
+/* ------------------------------------------------------------------------- */
+
+typedef struct sponge_solvent_lj sponge_solvent_lj;
+
+/* SOLVENT_LENNARD_JONES::Initial(controller, lj, lj_soft, md_info,
+ * default_enable) (solvent_LJ.cpp:149-248) over the LJ module `lj`.
+ * res_start (residue_numbers entries) and mass (atom_numbers) are host
+ * arrays; residues are contiguous atom ranges (the reference's update groups
+ * of a water / ion system).  solvent_LJ is the mdin key: -1 when absent (then
+ * on when default_enable -- the reference passes mode >= NVT -- and there are
+ * at least 10 residues), 0 false, 1 true.  The water size is the last
+ * residue's (3 or 4 atoms, else off); the solvent is the run of trailing
+ * residues of that size whose first three atoms have water masses (O in
+ * (15.9, 16.1), H in (1.007, 1.009)).  A module that is off computes nothing,
+ * as the reference's. */
+int sponge_solvent_lj_initial(sponge_solvent_lj** out, sponge_lj* lj, int atom_numbers,
+                              int residue_numbers, const int* res_start, const float* mass,
+                              int solvent_LJ, int default_enable, void* stream);
+
+/* The public fields is_initialized, solvent_numbers, solvent_start and
+ * water_points (solvent_LJ.h:13-17). */
+int sponge_solvent_lj_info(sponge_solvent_lj* h, int* enabled, int* solvent_numbers,
+                           int* solvent_start_residue, int* water_points);
+
+/* SOLVENT_LENNARD_JONES::LJ_PME_Direct_Force_With_Atom_Energy_And_Virial(
+ * atom_numbers, residue_numbers, d_res_start, crd, charge, frc, boundary, nl,
+ * pme_beta, need_atom_energy, atom_energy, need_virial, atom_lj_virial,
+ * atom_direct_pme_energy) (solvent_LJ.cpp:24-147, 256-432) over the list
+ * held by nl: for every water w of the tail, the pairs of each of its atoms
+ * with every partner of its OXYGEN's half list (the hydrogens' own lists are
+ * not used), strict r < cutoff; frc, atom_energy, atom_virial, the LJ
+ * module's per-atom LJ energy and atom_direct_pme_energy accumulated, pair
+ * energies and virials on the oxygen.  Call after sponge_lj_pme_direct_force
+ * with solvent_numbers = this module's solvent_numbers, which computes every
+ * pair with a non-solvent member. */
+int sponge_solvent_lj_direct_force(sponge_solvent_lj* h, sponge_nl* nl, int atom_numbers,
+                                   int residue_numbers, const float* crd, const float* charge,
+                                   float* frc, const float* cell, const float* rcell,
+                                   float pme_beta, int need_atom_energy, float* atom_energy,
+                                   int need_virial, float* atom_virial,
+                                   float* atom_direct_pme_energy);
+
+int sponge_solvent_lj_clear(sponge_solvent_lj* h);
 
 /* ------------------------------------------------------------------------- */
 /* PME: Particle_Mesh (SPONGE/PM_force/PM_force.h:152-209)                    */

@@ -23,6 +23,7 @@
 // Pinning status is recorded next to each function ("pinned by: ...").
 
 #include <algorithm>
+#include <chrono>
 #include <cfloat>
 #include <cmath>
 #include <complex>
@@ -901,6 +902,20 @@ static void sum_to_float_via_double(const float* a, int n, float* out) {
 // {reciprocal, self (incl. neutralizing), unused}.
 // Pinned by: orc64_ewald_recip (exact Ewald sum, FP64) as the grid refines,
 // numpy.fft for the transform, and partition-of-unity checks of the weights.
+namespace {
+// wall time of the pieces of the last orc_pme_reciprocal evaluation:
+// [0] base points + spread, [1] R2C + BC + C2R, [2] gather (the CPU
+// baseline's per-module split)
+double g_recip_seconds[3] = {0, 0, 0};
+inline double wall_now() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+}  // namespace
+
+ORC_API void orc_recip_seconds(double* out3) {
+    for (int k = 0; k < 3; ++k) out3[k] = g_recip_seconds[k];
+}
+
 ORC_API void orc_pme_reciprocal(orc_pme* p, const float* crd, const float* charge, float* frc,
                                 int need_virial, int need_energy, float* virial6,
                                 float* potential, int step, int update_interval,
@@ -910,6 +925,7 @@ ORC_API void orc_pme_reciprocal(orc_pme* p, const float* crd, const float* charg
     float reciprocal_ene = 0, self_ene = 0;
     if (step % update_interval == 0) {
         std::fill(p->Q.begin(), p->Q.end(), 0.f);
+        const double t_spread0 = wall_now();
         // PME_Atom_Near (PM_force.cpp:699-762)
         ORC_PRAGMA(omp parallel for)
         for (int atom = 0; atom < N; ++atom) {
@@ -970,6 +986,7 @@ ORC_API void orc_pme_reciprocal(orc_pme* p, const float* crd, const float* charg
                 }
             }
         }
+        const double t_fft0 = wall_now();
         // R2C (unnormalised, half spectrum along z)
         std::vector<cd> full((size_t)p->nall);
         for (int i = 0; i < p->nall; ++i) full[i] = p->Q[i];
@@ -1011,6 +1028,7 @@ ORC_API void orc_pme_reciprocal(orc_pme* p, const float* crd, const float* charg
                 }
         fft3d_c(full.data(), fx, fy, fz, +1);
         for (int i = 0; i < p->nall; ++i) p->FBCFQ[i] = (float)full[i].real();
+        const double t_gather0 = wall_now();
         // PME_Final (PM_force.cpp:829-893)
         ORC_PRAGMA(omp parallel for)
         for (int atom = 0; atom < N; ++atom) {
@@ -1042,6 +1060,10 @@ ORC_API void orc_pme_reciprocal(orc_pme* p, const float* crd, const float* charg
             p->force_backup[3 * atom + 1] += nv.y;
             p->force_backup[3 * atom + 2] += nv.z;
         }
+        const double t_end = wall_now();
+        g_recip_seconds[0] = t_fft0 - t_spread0;
+        g_recip_seconds[1] = t_gather0 - t_fft0;
+        g_recip_seconds[2] = t_end - t_gather0;
         // device_add_force (PM_force.cpp:1014-1022)
         for (int i = 0; i < 3 * N; ++i) frc[i] = frc[i] + (float)update_interval * p->force_backup[i];
     }
@@ -1393,6 +1415,14 @@ ORC_API double orc_sum_f32(const float* a, int n) {
     return s;
 }
 
+ORC_API void orc_set_num_threads(int n) {
+#ifdef _OPENMP
+    omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
 ORC_API int orc_num_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();
@@ -1619,4 +1649,149 @@ ORC_API int orc64_nve_shake(int N, double* crd, double* vel, const double* frc,
             }
     }
     return worst;
+}
+
+// orc64_direct over a half list (counts[N], serial[N*max_nb]) that holds
+// every non-excluded pair within the cutoff -- orc_nl_build's list, equal to
+// the brute-force set (tests/test_oracle.py) -- for systems where the O(N^2)
+// enumeration of orc64_direct is too slow.  Same sums in double.
+// Pinned by: orc64_direct on the same system (tests/test_oracle.py).
+ORC_API void orc64_direct_list(int N, const double* crd, const double* charge, const int* lj_type,
+                               const double* LJ_A, const double* LJ_B, const double* box,
+                               double cutoff, double beta, const int* counts, const int* serial,
+                               int max_nb, double* frc, double* energies2, double* virial6) {
+    double elj = 0, ec = 0;
+    double vir[6] = {0, 0, 0, 0, 0, 0};
+    std::vector<double> f((size_t)3 * N, 0.0);
+    const double tsp = 2.0 / sqrt(M_PI);
+#pragma omp parallel
+    {
+        std::vector<double> fl((size_t)3 * N, 0.0);
+        double lelj = 0, lec = 0, lv[6] = {0, 0, 0, 0, 0, 0};
+#pragma omp for schedule(dynamic, 64)
+        for (int i = 0; i < N; ++i) {
+            const int* row = serial + (size_t)i * max_nb;
+            for (int k = 0; k < counts[i]; ++k) {
+                const int j = row[k];
+                double d[3];
+                disp64(&crd[3 * j], &crd[3 * i], box, d);
+                const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+                const double r = sqrt(r2);
+                if (r >= cutoff) continue;
+                const int tp = lj_type_pair(lj_type[i], lj_type[j]);
+                const double A = LJ_A[tp], B = LJ_B[tp];
+                const double r6 = 1.0 / (r2 * r2 * r2);
+                const double qq = charge[i] * charge[j];
+                const double br = beta * r;
+                const double fr =
+                    (B - A * r6) * r6 / r2 - qq / (r2 * r) * (br * tsp * exp(-br * br) + erfc(br));
+                lelj += (A / 12.0 * r6 - B / 6.0) * r6;
+                lec += qq * erfc(br) / r;
+                const double g[3] = {fr * d[0], fr * d[1], fr * d[2]};
+                for (int c = 0; c < 3; ++c) {
+                    fl[3 * i + c] += g[c];
+                    fl[3 * j + c] -= g[c];
+                }
+                lv[0] -= g[0] * d[0];
+                lv[1] -= g[0] * d[1] + g[1] * d[0];
+                lv[2] -= g[1] * d[1];
+                lv[3] -= g[0] * d[2] + g[2] * d[0];
+                lv[4] -= g[1] * d[2] + g[2] * d[1];
+                lv[5] -= g[2] * d[2];
+            }
+        }
+#pragma omp critical
+        {
+            for (size_t k = 0; k < f.size(); ++k) f[k] += fl[k];
+            elj += lelj;
+            ec += lec;
+            for (int k = 0; k < 6; ++k) vir[k] += lv[k];
+        }
+    }
+    for (size_t k = 0; k < f.size(); ++k) frc[k] += f[k];
+    energies2[0] = elj;
+    energies2[1] = ec;
+    for (int k = 0; k < 6; ++k) virial6[k] = vir[k];
+}
+
+// solvent_LJ, restating Lennard_Jones_And_Direct_Coulomb_Device<WAT_POINTS>
+// of SPONGE/Lennard_Jones_force/solvent_LJ.cpp:24-147: for every water of the
+// solvent tail (atoms from solvent_atom on, water_points atoms per water,
+// oxygen first) and every j of the OXYGEN's half list, the pairs (a, j) of
+// each atom a of the water with the generic pair functions; forces on a and
+// j; energies (LJ + Coulomb, LJ, Coulomb) and virial on the oxygen.
+// Accumulates (+=) like orc_lj_direct; called after orc_lj_direct with
+// solvent_n = local_n - solvent_atom.
+// Pinned by: on a list fresh from a build the two together equal the plain
+// orc_lj_direct (every pair within the cutoff of two waters is in the
+// oxygens' lists); on a stale list, an FP64 sum over the pair set spelled out
+// in numpy (tests/test_oracle_solvent.py).
+ORC_API void orc_solvent_lj(int local_n, int solvent_atom, int water_points, const int* counts,
+                            const int* serial, int max_nb, const float* crd, const float* charge,
+                            const int* lj_type, const float* LJ_A, const float* LJ_B,
+                            const float* cell6, const float* rcell6, float cutoff, float pme_beta,
+                            int need_energy, int need_virial, float* frc, float* atom_energy,
+                            float* atom_virial, float* atom_direct_cf_energy,
+                            float* atom_LJ_ene) {
+    const LT cell = ld_lt(cell6), rcell = ld_lt(rcell6);
+    const int n_water = (local_n - solvent_atom) / water_points;
+    ORC_PRAGMA(omp parallel for schedule(dynamic))
+    for (int w = 0; w < n_water; ++w) {
+        const int oxygen = solvent_atom + w * water_points;
+        V3 own[4] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        LT virial = {0, 0, 0, 0, 0, 0};
+        float e_lj = 0, e_c = 0;
+        const int* row = serial + (size_t)oxygen * max_nb;
+        for (int k = 0; k < counts[oxygen]; ++k) {
+            const int atom_j = row[k];
+            const float ij_factor = atom_j < local_n ? 1.0f : 0.5f;
+            const V3 r2 = ld3(crd, atom_j);
+            const float q2 = charge[atom_j];
+            const int t2 = lj_type[atom_j];
+            V3 on_j = {0, 0, 0};
+            for (int a = 0; a < water_points; ++a) {
+                const int atom_a = oxygen + a;
+                const V3 dr = disp_pbc(r2, ld3(crd, atom_a), cell, rcell);
+                const float dr_abs = sqrtf(dr.x * dr.x + dr.y * dr.y + dr.z * dr.z);
+                if (!(dr_abs < cutoff)) continue;
+                const int tp = lj_type_pair(lj_type[atom_a], t2);
+                const float A = LJ_A[tp], B = LJ_B[tp];
+                const float q1 = charge[atom_a];
+                float frc_abs = (B - A * powf(dr_abs, -6.0f)) * powf(dr_abs, -8.0f);
+                const float beta_dr = pme_beta * dr_abs;
+                frc_abs -= q1 * q2 * powf(dr_abs, -3.0f) *
+                           (beta_dr * TWO_DIVIDED_BY_SQRT_PI * expf(-beta_dr * beta_dr) +
+                            erfcf(beta_dr));
+                const V3 frc_lin = vscale(frc_abs, dr);
+                own[a] = vadd(own[a], frc_lin);
+                on_j = vsub(on_j, frc_lin);
+                if (need_virial) virial = lt_sub(virial, lt_scale(ij_factor, virial_fd(frc_lin, dr)));
+                if (need_energy) {
+                    const float dr_6 = powf(dr_abs, -6.0f);
+                    e_lj += ij_factor * ((0.083333333f * A * dr_6 - 0.166666667f * B) * dr_6);
+                    e_c += ij_factor * (q1 * q2 * erfcf(pme_beta * dr_abs) / dr_abs);
+                }
+            }
+            if (atom_j < local_n) {
+                ORC_ATOMIC frc[3 * atom_j] += on_j.x;
+                ORC_ATOMIC frc[3 * atom_j + 1] += on_j.y;
+                ORC_ATOMIC frc[3 * atom_j + 2] += on_j.z;
+            }
+        }
+        for (int a = 0; a < water_points; ++a) {
+            ORC_ATOMIC frc[3 * (oxygen + a)] += own[a].x;
+            ORC_ATOMIC frc[3 * (oxygen + a) + 1] += own[a].y;
+            ORC_ATOMIC frc[3 * (oxygen + a) + 2] += own[a].z;
+        }
+        if (need_energy) {
+            atom_energy[oxygen] += e_lj + e_c;
+            if (atom_LJ_ene) atom_LJ_ene[oxygen] += e_lj;
+            if (atom_direct_cf_energy) atom_direct_cf_energy[oxygen] += e_c;
+        }
+        if (need_virial) {
+            float* v = atom_virial + 6 * oxygen;
+            v[0] += virial.a11; v[1] += virial.a21; v[2] += virial.a22;
+            v[3] += virial.a31; v[4] += virial.a32; v[5] += virial.a33;
+        }
+    }
 }

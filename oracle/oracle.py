@@ -78,6 +78,14 @@ def _load(name):
     lib.orc_sum_f32.restype = C.c_double
     lib.orc_sum_f32.argtypes = [_f32p, C.c_int]
     lib.orc_num_threads.restype = C.c_int
+    lib.orc_set_num_threads.argtypes = [C.c_int]
+    lib.orc_recip_seconds.argtypes = [_f64p]
+    lib.orc64_direct_list.argtypes = [C.c_int, _f64p, _f64p, _i32p, _f64p, _f64p, _f64p,
+                                      C.c_double, C.c_double, _i32p, _i32p, C.c_int, _f64p,
+                                      _f64p, _f64p]
+    lib.orc_solvent_lj.argtypes = [C.c_int, C.c_int, C.c_int, _i32p, _i32p, C.c_int, _f32p, _f32p,
+                                   _i32p, _f32p, _f32p, _f32p, _f32p, C.c_float, C.c_float,
+                                   C.c_int, C.c_int, _f32p, _f32p, _f32p, _f32p, _f32p]
     lib.orc_settle_geometry.argtypes = [C.c_float] * 6 + [_f32p]
     lib.orc_nve_settle.argtypes = [C.c_int, _f32p, _f32p, _f32p, _f32p, _f32p, C.c_int, _i32p,
                                    _f32p, _f32p, _f32p, C.c_float]
@@ -171,7 +179,7 @@ def pair_array(counts, serial):
 
 
 def lj_direct(sys_, counts, serial, beta, crd=None, need_energy=True, need_virial=True,
-              fast=False):
+              fast=False, solvent_n=0):
     crd = sys_.crd if crd is None else crd
     N = len(crd)
     max_nb = serial.shape[1]
@@ -180,10 +188,31 @@ def lj_direct(sys_, counts, serial, beta, crd=None, need_energy=True, need_viria
     v = np.zeros(6 * N, np.float32)
     edir = np.zeros(N, np.float32)
     elj = np.zeros(N, np.float32)
-    lib(fast).orc_lj_direct(N, 0, i32(counts), i32(serial.reshape(-1)), max_nb, f32(crd),
+    lib(fast).orc_lj_direct(N, int(solvent_n), i32(counts), i32(serial.reshape(-1)), max_nb, f32(crd),
                             sys_.charge, i32(sys_.lj_type), sys_.lj_A, sys_.lj_B, sys_.cell(),
                             sys_.rcell(), sys_.cutoff, beta, int(need_energy), int(need_virial),
                             frc, e, v, edir, elj)
+    return dict(frc=frc.reshape(N, 3), atom_energy=e, atom_virial=v.reshape(N, 6),
+                direct_energy=edir, lj_energy=elj)
+
+
+def solvent_lj(sys_, counts, serial, beta, solvent_atom, water_points=3, crd=None,
+               need_energy=True, need_virial=True):
+    """The solvent_LJ pairs of the water tail (from atom solvent_atom on)
+    over the reference-format half list; same output layout as lj_direct."""
+    crd = sys_.crd if crd is None else crd
+    N = len(crd)
+    max_nb = serial.shape[1]
+    frc = np.zeros(3 * N, np.float32)
+    e = np.zeros(N, np.float32)
+    v = np.zeros(6 * N, np.float32)
+    edir = np.zeros(N, np.float32)
+    elj = np.zeros(N, np.float32)
+    lib().orc_solvent_lj(N, int(solvent_atom), int(water_points), i32(counts),
+                         i32(serial.reshape(-1)), max_nb, f32(crd), sys_.charge,
+                         i32(sys_.lj_type), sys_.lj_A, sys_.lj_B, sys_.cell(), sys_.rcell(),
+                         sys_.cutoff, beta, int(need_energy), int(need_virial), frc, e, v, edir,
+                         elj)
     return dict(frc=frc.reshape(N, 3), atom_energy=e, atom_virial=v.reshape(N, 6),
                 direct_energy=edir, lj_energy=elj)
 
@@ -275,6 +304,13 @@ class PME:
         return dict(frc=frc.reshape(self.N, 3), virial=vir, potential=float(pot[0]),
                     reciprocal=float(en[0]), self_energy=float(en[1]))
 
+    def last_seconds(self):
+        """Wall time of the last evaluation's pieces: spread, FFT
+        convolution, gather."""
+        t = np.zeros(3)
+        self.lib.orc_recip_seconds(t)
+        return dict(spread=t[0], fft_convolution=t[1], gather=t[2])
+
     def grids(self):
         fx, fy, fz = self.fft
         nall = fx * fy * fz
@@ -322,6 +358,22 @@ def direct64(sys_, beta, crd=None, cutoff=None):
     return dict(frc=frc.reshape(N, 3), lj=en[0], coulomb=en[1], virial=vir)
 
 
+def direct64_list(sys_, beta, counts, serial, crd=None):
+    """FP64 direct sums over a half list holding every pair within the
+    cutoff (orc_nl_build's), for systems too large for direct64."""
+    crd = sys_.crd if crd is None else crd
+    N = len(crd)
+    frc = np.zeros(3 * N)
+    en = np.zeros(2)
+    vir = np.zeros(6)
+    serial = np.ascontiguousarray(serial, np.int32).reshape(N, -1)
+    lib().orc64_direct_list(N, f64(crd).reshape(-1), f64(sys_.charge), i32(sys_.lj_type),
+                            f64(sys_.lj_A), f64(sys_.lj_B), f64(sys_.box), float(sys_.cutoff),
+                            float(beta), i32(counts), serial.reshape(-1), serial.shape[1], frc,
+                            en, vir)
+    return dict(frc=frc.reshape(N, 3), lj=en[0], coulomb=en[1], virial=vir)
+
+
 def excluded64(sys_, beta, crd=None):
     crd = sys_.crd if crd is None else crd
     N = len(crd)
@@ -358,15 +410,17 @@ TIME_CONVERSION = 20.455  # ps -> internal time units, SPONGE/common.h:138
 
 
 def water_triples(sys_):
-    """(A, B, C) = (O, H1, H2) atom ids of every water (the waters come first)."""
-    return np.arange(3 * sys_.n_waters, dtype=np.int32).reshape(-1, 3)
+    """(A, B, C) = (O, H1, H2) atom ids of every water."""
+    w0 = getattr(sys_, "water_start", 0)
+    return np.arange(w0, w0 + 3 * sys_.n_waters, dtype=np.int32).reshape(-1, 3)
 
 
 def water_sides(sys_, crd=None):
     """Side lengths |OH1|, |OH2|, |H1H2| of the first water, in double (the
     step driver's rigid geometry comes from the same coordinates)."""
     crd = np.asarray(sys_.crd if crd is None else crd, np.float64).reshape(-1, 3)
-    a, b, c = crd[0], crd[1], crd[2]
+    w0 = getattr(sys_, "water_start", 0)
+    a, b, c = crd[w0], crd[w0 + 1], crd[w0 + 2]
     return (float(np.linalg.norm(a - b)), float(np.linalg.norm(a - c)),
             float(np.linalg.norm(b - c)))
 
@@ -385,7 +439,7 @@ def nve_settle(sys_, crd, vel, frc, dt_ps=None, sides=None):
     m = f32(sys_.mass)
     im = np.where(m > 0, np.float32(1.0) / np.where(m > 0, m, 1), 0).astype(np.float32)
     w = water_triples(sys_)
-    g = settle_geometry(sides or water_sides(sys_), m[:3] if len(w) else (16.0, 1.008, 1.008))
+    g = settle_geometry(sides or water_sides(sys_), m[w[0]] if len(w) else (16.0, 1.008, 1.008))
     c = f32(crd).reshape(-1).copy()
     v = f32(vel).reshape(-1).copy()
     lib().orc_nve_settle(N, c, v, f32(frc).reshape(-1), im, m, len(w), i32(w).reshape(-1), g,

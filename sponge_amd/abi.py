@@ -54,6 +54,7 @@ _P = C.c_void_p
 _F = C.c_float
 _I = C.c_int
 _FP = C.POINTER(C.c_float)
+_IP = C.POINTER(C.c_int)
 
 # name -> (restype, argtypes)
 SIGNATURES = {
@@ -83,6 +84,11 @@ SIGNATURES = {
     "sponge_lj_pme_direct_force_list_noalloc": (_I, [_I, _I, _P, _P, _P, _F, _P, _P, _I, _P, _P,
                                                      _P, _FP, _FP, _F, _I, _P, _I, _P, _P, _P,
                                                      _P]),
+    "sponge_solvent_lj_initial": (_I, [C.POINTER(C.c_void_p), _P, _I, _I, _IP, _FP, _I, _I, _P]),
+    "sponge_solvent_lj_info": (_I, [_P, _IP, _IP, _IP, _IP]),
+    "sponge_solvent_lj_direct_force": (_I, [_P, _P, _I, _I, _P, _P, _P, _FP, _FP, _F, _I, _P, _I,
+                                            _P, _P]),
+    "sponge_solvent_lj_clear": (_I, [_P]),
     "sponge_lj_long_range_correction": (_I, [_P, _I, _P, _I, _P, _F]),
     "sponge_lj_step_print": (_I, [_P, _FP, _FP]),
     "sponge_lj_atom_energy": (_P, [_P]),
@@ -312,9 +318,10 @@ class LennardJones:
         return f.value
 
     def direct_force(self, nl, crd, charge, frc, box, beta, atom_energy=None, atom_virial=None,
-                     direct_energy=None):
+                     direct_energy=None, solvent_numbers=0):
         _check(self.lib.sponge_lj_pme_direct_force(
-            self.h, nl.h, self.N, self.N, 0, 0, _ptr(crd), _ptr(charge), _ptr(frc), *box.args(),
+            self.h, nl.h, self.N, self.N, int(solvent_numbers), 0, _ptr(crd), _ptr(charge),
+            _ptr(frc), *box.args(),
             float(beta), int(atom_energy is not None), _ptr(atom_energy),
             int(atom_virial is not None), _ptr(atom_virial), _ptr(direct_energy)),
             "sponge_lj_pme_direct_force")
@@ -343,6 +350,47 @@ class LennardJones:
     def clear(self):
         if self.h:
             self.lib.sponge_lj_clear(self.h)
+            self.h = None
+
+    __del__ = clear
+
+
+class SolventLJ:
+    """SOLVENT_LENNARD_JONES (SPONGE/Lennard_Jones_force/solvent_LJ.h:9-50)
+    over a LennardJones module: the water tail's own pairs."""
+
+    def __init__(self, lj, s, solvent_LJ=-1, default_enable=True, stream=None):
+        self.lib = load()
+        self.lj = lj
+        rs = np.ascontiguousarray(s.residue_starts(), np.int32)
+        m = np.ascontiguousarray(s.mass, np.float32)
+        self.R = len(rs)
+        h = C.c_void_p()
+        _check(self.lib.sponge_solvent_lj_initial(
+            C.byref(h), lj.h, int(s.n_atoms), self.R, rs.ctypes.data_as(_IP),
+            m.ctypes.data_as(_FP), int(solvent_LJ), int(bool(default_enable)),
+            _stream_ptr(stream)), "sponge_solvent_lj_initial")
+        self.h = h
+        self.N = int(s.n_atoms)
+
+    def info(self):
+        v = [C.c_int() for _ in range(4)]
+        _check(self.lib.sponge_solvent_lj_info(self.h, *[C.byref(x) for x in v]),
+               "sponge_solvent_lj_info")
+        return dict(enabled=bool(v[0].value), solvent_numbers=v[1].value,
+                    solvent_start=v[2].value, water_points=v[3].value)
+
+    def direct_force(self, nl, crd, charge, frc, box, beta, atom_energy=None, atom_virial=None,
+                     direct_energy=None):
+        _check(self.lib.sponge_solvent_lj_direct_force(
+            self.h, nl.h, self.N, self.R, _ptr(crd), _ptr(charge), _ptr(frc), *box.args(),
+            float(beta), int(atom_energy is not None), _ptr(atom_energy),
+            int(atom_virial is not None), _ptr(atom_virial), _ptr(direct_energy)),
+            "sponge_solvent_lj_direct_force")
+
+    def clear(self):
+        if self.h:
+            self.lib.sponge_solvent_lj_clear(self.h)
             self.h = None
 
     __del__ = clear
@@ -618,7 +666,8 @@ class MDSystem:
                       arr(s.lj_type, np.int32), arr(s.excl_start, np.int32),
                       arr(s.excl_num, np.int32),
                       arr(s.excl_list if len(s.excl_list) else np.zeros(1), np.int32),
-                      arr(np.arange(3 * s.n_waters), np.int32), arr(s.box)]
+                      arr(np.arange(s.water_start, s.water_start + 3 * s.n_waters), np.int32),
+                      arr(s.box)]
         k = self._keep
         h = C.c_void_p()
         _check(self.lib.sponge_md_initial(

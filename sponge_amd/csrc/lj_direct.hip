@@ -25,13 +25,12 @@
 #include <vector>
 
 #include "internal.h"
+#include "pair_term.h"
 #include "timing.h"
 
 using namespace sa;
 
 namespace {
-
-constexpr float kTwoOverSqrtPi = 1.1283791670218446f;  // Lennard_Jones_force.h:8
 
 template <bool ORTHO, bool NEED_E, bool NEED_V>
 __global__ void __launch_bounds__(256) k_lj_pairs(
@@ -48,8 +47,8 @@ __global__ void __launch_bounds__(256) k_lj_pairs(
     const int n = list_count[p];
     const int* row = list + (size_t)p * max_nb;
     float fx = 0.f, fy = 0.f, fz = 0.f;
-    float elj = 0.f, ec = 0.f;
-    float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f, v5 = 0.f;
+    double elj = 0, ec = 0;  // energy and virial terms summed in FP64 per lane
+    double v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
     for (int k = lane; k < n; k += kWave) {
         const int q = row[k];
         const float4 pj4 = pos[q];
@@ -58,14 +57,13 @@ __global__ void __launch_bounds__(256) k_lj_pairs(
         if (r2 < rc2_thr) {
             const float2 ab = AB[lj_pair_index(ti, ptype[q])];
             const float rabs = sqrtf(r2);
-            const float rinv = 1.0f / rabs;
-            const float r2inv = rinv * rinv;
-            const float r6inv = r2inv * r2inv * r2inv;
+            float pw3, pw6, pw8;
+            ref_powers(rabs, pw3, pw6, pw8);
             const float qq = pi4.w * pj4.w;
             const float br = beta * rabs;
             const float erfc_br = erfcf(br);
-            const float flj = (ab.y - ab.x * r6inv) * r6inv * r2inv;
-            const float fc = qq * rinv * r2inv * (br * kTwoOverSqrtPi * expf(-br * br) + erfc_br);
+            const float flj = (ab.y - ab.x * pw6) * pw8;
+            const float fc = qq * pw3 * (br * kTwoOverSqrtPi * expf(-br * br) + erfc_br);
             const float f = flj - fc;
             const float lx = f * d.x, ly = f * d.y, lz = f * d.z;
             fx += lx;
@@ -83,8 +81,8 @@ __global__ void __launch_bounds__(256) k_lj_pairs(
                 v5 -= lz * d.z;
             }
             if (NEED_E) {
-                elj += (0.083333333f * ab.x * r6inv - 0.166666667f * ab.y) * r6inv;
-                ec += qq * erfc_br * rinv;
+                elj += (0.083333333f * ab.x * pw6 - 0.166666667f * ab.y) * pw6;
+                ec += qq * erfc_br / rabs;
             }
         }
     }
@@ -128,87 +126,6 @@ __global__ void __launch_bounds__(256) k_lj_pairs(
 // virials are counted in the row of the lower global id only, the
 // reference's half-list row.
 //
-// Arithmetic per in-cutoff pair on force-only steps (FAST, when beta *
-// cutoff <= kErfcxMax): 1/r from the hardware rsqrt and erfc(beta r) as
-// exp(-(beta r)^2) erfcx(beta r) with the polynomial erfcx, so one
-// exponential serves erfc and the force bracket.  Steps that need energies or
-// the virial use the reference's sqrtf, division and erfcf.
-constexpr int kPairStage = 3584;
-constexpr int kMaxLdsTypes = 64;  // the square (A, B) table lives in LDS: up to 64 LJ types
-
-// The periodic image of x nearest the point C (orthorhombic box).
-__device__ __forceinline__ float3 near_image(float3 x, float3 C, const LT6& c, const LT6& r) {
-    return make_float3(x.x - c.a11 * rintf((x.x - C.x) * r.a11),
-                       x.y - c.a22 * rintf((x.y - C.y) * r.a22),
-                       x.z - c.a33 * rintf((x.z - C.z) * r.a33));
-}
-
-// Partner data of a row entry: from the block's LDS stage (every position of
-// the stream staged) or, for the rare stream longer than the stage, from the
-// sorted arrays in HBM.  The two sources are separate code paths chosen once
-// per block, so that LDS reads stay ds_read and HBM reads global loads.
-struct PairSrc {
-    const float4* pos;
-    const int* typ;
-    const int* gid;
-};
-
-template <bool STAGED>
-__device__ __forceinline__ int src_index(int t, int nr, const int* r_first, const int* r_off) {
-    if (STAGED) return t;
-    return stream_to_sorted(t, nr, r_first, r_off);
-}
-
-// Force (and energy / virial terms) of one pair, branch-free: pairs at or
-// beyond the cutoff contribute exact zeros (selects, so the non-finite
-// values of a far or padding pair never reach the sums).
-template <bool NEED_E, bool NEED_V, bool FAST>
-struct PairTerm {
-    float lx, ly, lz, elj, ec;
-    bool in;
-    __device__ __forceinline__ PairTerm(const float3& d, float r2, float qi, float qj, float2 ab,
-                                        float rc2_thr, float beta) {
-        in = r2 < rc2_thr;
-        // FAST: hardware rsqrt (1 ulp); otherwise the reference's sqrtf and
-        // division, which energies and virials need: the LJ energy is a
-        // difference of the r^-12 and r^-6 terms and amplifies a biased 1/r
-        float rinv, rabs;
-        if (FAST) {
-            rinv = __builtin_amdgcn_rsqf(r2);
-            rabs = r2 * rinv;
-        } else {
-            rabs = sqrtf(r2);
-            rinv = 1.0f / rabs;
-        }
-        const float r2inv = rinv * rinv;
-        const float r6inv = r2inv * r2inv * r2inv;
-        const float qq = qi * qj;
-        const float br = beta * rabs;
-        float erfc_br, bracket;
-        if (FAST) {
-            const float ex = __expf(-br * br);
-            const float ecx = erfcx_poly(br);
-            erfc_br = ex * ecx;
-            bracket = ex * fmaf(br, kTwoOverSqrtPi, ecx);
-        } else {
-            erfc_br = erfcf(br);
-            bracket = fmaf(br * kTwoOverSqrtPi, expf(-br * br), erfc_br);
-        }
-        const float flj = (ab.y - ab.x * r6inv) * r6inv * r2inv;
-        const float fc = qq * rinv * r2inv * bracket;
-        const float f = in ? flj - fc : 0.f;
-        lx = f * d.x;
-        ly = f * d.y;
-        lz = f * d.z;
-        elj = 0.f;
-        ec = 0.f;
-        if (NEED_E) {
-            elj = in ? (0.083333333f * ab.x * r6inv - 0.166666667f * ab.y) * r6inv : 0.f;
-            ec = in ? qq * erfc_br * rinv : 0.f;
-        }
-    }
-};
-
 // One row (wavefront-wide): two adjacent partner entries per lane and pass
 // (one 32-bit load: rows start 4-byte aligned since the row capacity is
 // even), with the next pass's pair loaded before the current one is used so
@@ -216,19 +133,26 @@ struct PairTerm {
 // SHIFT: the staged partner positions are the images nearest the cell's
 // centre C (see k_lj_cells), so the displacement is a plain difference with
 // the row atom's own image nearest C.
-template <bool ORTHO, bool NEED_E, bool NEED_V, bool FAST, bool STAGED, bool SHIFT = false>
+// SOLV: the solvent_LJ split of the reference (Lennard_Jones_force.cpp:55-58
+// with solvent_numbers > 0): atoms from solvent_start on are the solvent
+// tail, whose pairs with each other the solvent module evaluates; a solvent
+// row here keeps only its partners below solvent_start.
+template <bool ORTHO, bool NEED_E, bool NEED_V, bool FAST, bool STAGED, bool SHIFT = false,
+          bool SOLV = false>
 __device__ __forceinline__ void lj_row(int p, int lane, int n, const unsigned short* __restrict__ row,
                                        PairSrc src, int nr, const int* r_first, const int* r_off,
                                        const float4* __restrict__ pos, const float2* trow, int gi,
                                        const LT6& c, const LT6& r, float rc2_thr, float beta,
                                        float4* __restrict__ facc, float2* __restrict__ eacc,
-                                       float* __restrict__ vacc, float3 C = float3{0.f, 0.f, 0.f}) {
+                                       float* __restrict__ vacc, float3 C = float3{0.f, 0.f, 0.f},
+                                       int solvent_start = 0) {
+    const bool solvent_row = SOLV && gi >= solvent_start;
     const float4 pi4 = pos[p];
     const float3 pi = SHIFT ? near_image(make_float3(pi4.x, pi4.y, pi4.z), C, c, r)
                             : make_float3(pi4.x, pi4.y, pi4.z);
     float fx = 0.f, fy = 0.f, fz = 0.f;
-    float elj = 0.f, ec = 0.f;
-    float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f, v5 = 0.f;
+    double elj = 0, ec = 0;  // energy and virial terms summed in FP64 per lane
+    double v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
     const unsigned* row2 = reinterpret_cast<const unsigned*>(row);
     const int npairs = (n + 1) >> 1;  // 32-bit words holding the row
     const int last = npairs > 0 ? npairs - 1 : 0;
@@ -262,6 +186,10 @@ __device__ __forceinline__ void lj_row(int p, int lane, int n, const unsigned sh
         // an absent entry is given r2 = the cutoff threshold: out of range
         r2a = has_a ? r2a : rc2_thr;
         r2b = has_b ? r2b : rc2_thr;
+        if (SOLV && solvent_row) {
+            r2a = src.gid[qa] < solvent_start ? r2a : rc2_thr;
+            r2b = src.gid[qb] < solvent_start ? r2b : rc2_thr;
+        }
         const PairTerm<NEED_E, NEED_V, FAST> A(da, r2a, pi4.w, pa.w, aba, rc2_thr, beta);
         const PairTerm<NEED_E, NEED_V, FAST> B(db, r2b, pi4.w, pb.w, abb, rc2_thr, beta);
         fx += A.lx + B.lx;
@@ -318,7 +246,7 @@ __device__ __forceinline__ void lj_row(int p, int lane, int n, const unsigned sh
     }
 }
 
-template <bool ORTHO, bool NEED_E, bool NEED_V, bool FAST>
+template <bool ORTHO, bool NEED_E, bool NEED_V, bool FAST, bool SOLV = false>
 __global__ void __launch_bounds__(1024) k_lj_cells(
     const int* __restrict__ start, const int* __restrict__ nranges, int max_ranges,
     const int* __restrict__ g_rfirst, const int* __restrict__ g_roff,
@@ -326,11 +254,13 @@ __global__ void __launch_bounds__(1024) k_lj_cells(
     const float4* __restrict__ pos, const int* __restrict__ ptype, const int* __restrict__ pgid,
     const float2* __restrict__ AB_sq, int T, LT6 c, LT6 r, float rc2_thr, float beta, int stage,
     float gl, int Nx, int Ny, int Nz, float reach,
-    float4* __restrict__ facc, float2* __restrict__ eacc, float* __restrict__ vacc) {
+    float4* __restrict__ facc, float2* __restrict__ eacc, float* __restrict__ vacc,
+    int solvent_start) {
+    constexpr bool kGid = NEED_E || NEED_V || SOLV;  // global ids staged
     extern __shared__ float4 cpos[];
     int* ctyp = reinterpret_cast<int*>(cpos + stage);
     int* cgid = ctyp + stage;
-    float2* cab = reinterpret_cast<float2*>(ctyp + ((NEED_E || NEED_V) ? 2 : 1) * stage);
+    float2* cab = reinterpret_cast<float2*>(ctyp + (kGid ? 2 : 1) * stage);
     __shared__ int r_first[kMaxRanges];
     __shared__ int r_off[kMaxRanges + 1];
     const int cell = blockIdx.x;
@@ -377,29 +307,28 @@ __global__ void __launch_bounds__(1024) k_lj_cells(
             }
             cpos[t] = v;
             ctyp[t] = ptype[q];
-            if (NEED_E || NEED_V) cgid[t] = pgid[q];
+            if (kGid) cgid[t] = pgid[q];
         });
     __syncthreads();
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave, nwave = blockDim.x / kWave;
     for (int p = a0 + wave; p < a1; p += nwave) {
         const float2* trow = cab + ptype[p] * T;
-        const int gi = (NEED_E || NEED_V) ? pgid[p] : 0;
+        const int gi = kGid ? pgid[p] : 0;
         const int n = list_count[p];
         const unsigned short* row = list + (size_t)p * cap;
         if (staged && shift)
-            lj_row<ORTHO, NEED_E, NEED_V, FAST, true, true>(p, lane, n, row,
-                                                            PairSrc{cpos, ctyp, cgid}, nr, r_first,
-                                                            r_off, pos, trow, gi, c, r, rc2_thr,
-                                                            beta, facc, eacc, vacc, C);
+            lj_row<ORTHO, NEED_E, NEED_V, FAST, true, true, SOLV>(
+                p, lane, n, row, PairSrc{cpos, ctyp, cgid}, nr, r_first, r_off, pos, trow, gi, c,
+                r, rc2_thr, beta, facc, eacc, vacc, C, solvent_start);
         else if (staged)
-            lj_row<ORTHO, NEED_E, NEED_V, FAST, true>(p, lane, n, row, PairSrc{cpos, ctyp, cgid}, nr,
-                                                      r_first, r_off, pos, trow, gi, c, r, rc2_thr,
-                                                      beta, facc, eacc, vacc);
+            lj_row<ORTHO, NEED_E, NEED_V, FAST, true, false, SOLV>(
+                p, lane, n, row, PairSrc{cpos, ctyp, cgid}, nr, r_first, r_off, pos, trow, gi, c,
+                r, rc2_thr, beta, facc, eacc, vacc, C, solvent_start);
         else
-            lj_row<ORTHO, NEED_E, NEED_V, FAST, false>(p, lane, n, row, PairSrc{pos, ptype, pgid}, nr,
-                                                       r_first, r_off, pos, trow, gi, c, r, rc2_thr,
-                                                       beta, facc, eacc, vacc);
+            lj_row<ORTHO, NEED_E, NEED_V, FAST, false, false, SOLV>(
+                p, lane, n, row, PairSrc{pos, ptype, pgid}, nr, r_first, r_off, pos, trow, gi, c,
+                r, rc2_thr, beta, facc, eacc, vacc, C, solvent_start);
     }
 }
 
@@ -475,27 +404,37 @@ __global__ void k_sum_f32(int n, const float* __restrict__ a, double* out) {
 // (the step driver); the public entry point makes no such assumption.
 template <bool ORTHO>
 int launch_rows(sponge_lj* lj, sponge_nl* nl, LT6 c, LT6 r, float beta, int need_e, int need_v,
-                hipStream_t s, bool allow_shift = false) {
+                hipStream_t s, bool allow_shift = false, int solvent_start = -1) {
     ScopedTimer timer("lj_pairs", s);
-    const size_t sh = (sizeof(float4) + ((need_e || need_v) ? 2 : 1) * sizeof(int)) * lj->stage +
-                      sizeof(float2) * lj->T * lj->T;
+    const bool solv = solvent_start >= 0 && solvent_start < lj->N;
+    const size_t sh =
+        (sizeof(float4) + ((need_e || need_v || solv) ? 2 : 1) * sizeof(int)) * lj->stage +
+        sizeof(float2) * lj->T * lj->T;
     const bool fast = beta * lj->cutoff <= kErfcxMax && !need_e && !need_v;
     // how far a row atom can be from its build-time cell: the automatic
     // refresh rebuilds once any atom moved more than skin * skin_permit
     const float reach = allow_shift && nl->opt.refresh_interval <= 0
                             ? lj->cutoff + nl->skin * nl->opt.skin_permit
                             : -1.f;
-#define SA_LJ_ROWS(E, V, F)                                                                         \
-    hipLaunchKernelGGL((k_lj_cells<ORTHO, E, V, F>), dim3(nl->G), dim3(1024), sh, s, nl->d_cell_start, \
-                       nl->d_nranges, nl->max_ranges, nl->d_rfirst, nl->d_roff, nl->d_list_count,    \
-                       nl->d_list, nl->cap, lj->d_pos, lj->d_ptype, lj->d_pgid, lj->d_AB_sq, lj->T,  \
-                       c, r, lj->rc2_thr, beta, lj->stage, nl->grid_length, nl->Nx, nl->Ny,       \
-                       nl->Nz, reach, lj->d_facc, lj->d_eacc, lj->d_vacc)
-    if (fast) SA_LJ_ROWS(false, false, true);
-    else if (need_e && need_v) SA_LJ_ROWS(true, true, false);
-    else if (need_e) SA_LJ_ROWS(true, false, false);
-    else if (need_v) SA_LJ_ROWS(false, true, false);
-    else SA_LJ_ROWS(false, false, false);
+#define SA_LJ_ROWS(E, V, F, SV)                                                                 \
+    hipLaunchKernelGGL((k_lj_cells<ORTHO, E, V, F, SV>), dim3(nl->G), dim3(1024), sh, s,          \
+                       nl->d_cell_start, nl->d_nranges, nl->max_ranges, nl->d_rfirst, nl->d_roff, \
+                       nl->d_list_count, nl->d_list, nl->cap, lj->d_pos, lj->d_ptype, lj->d_pgid, \
+                       lj->d_AB_sq, lj->T, c, r, lj->rc2_thr, beta, lj->stage, nl->grid_length,   \
+                       nl->Nx, nl->Ny, nl->Nz, reach, lj->d_facc, lj->d_eacc, lj->d_vacc,         \
+                       solvent_start)
+#define SA_LJ_ROWS_EVF(SV)                                      \
+    if (fast) SA_LJ_ROWS(false, false, true, SV);               \
+    else if (need_e && need_v) SA_LJ_ROWS(true, true, false, SV); \
+    else if (need_e) SA_LJ_ROWS(true, false, false, SV);        \
+    else if (need_v) SA_LJ_ROWS(false, true, false, SV);        \
+    else SA_LJ_ROWS(false, false, false, SV)
+    if (solv) {
+        SA_LJ_ROWS_EVF(true);
+    } else {
+        SA_LJ_ROWS_EVF(false);
+    }
+#undef SA_LJ_ROWS_EVF
 #undef SA_LJ_ROWS
     SA_LAUNCH_CHECK();
     return 0;
@@ -675,7 +614,9 @@ extern "C" int sponge_lj_pme_direct_force(sponge_lj* lj, sponge_nl* nl, int atom
                                           float* atom_direct_pme_energy) {
     if (!lj || !nl || !crd || !charge || !frc || !cell || !rcell)
         return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
-    if (ghost_numbers != 0 || solvent_numbers != 0) return SPONGE_AMD_ERROR_NOT_IMPLEMENTED;
+    if (ghost_numbers != 0) return SPONGE_AMD_ERROR_NOT_IMPLEMENTED;
+    if (solvent_numbers < 0 || solvent_numbers > local_atom_numbers)
+        return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
     if (atom_numbers == 0 || local_atom_numbers == 0) return 0;  // :351
     if (local_atom_numbers != lj->N || nl->N != lj->N) return SPONGE_AMD_ERROR_CONFLICTING_COMMAND;
     if (!nl->built) return SPONGE_AMD_ERROR_MISSING_COMMAND;
@@ -685,10 +626,15 @@ extern "C" int sponge_lj_pme_direct_force(sponge_lj* lj, sponge_nl* nl, int atom
     const LT6 c = lt_from(cell), r = lt_from(rcell);
     int rc = nl_pack_for_pairs(nl, crd, charge, lj->d_type, lj->d_pos, lj->d_ptype, lj->d_pgid, s);
     if (rc) return rc;
+    // rows from local_atom_numbers - solvent_numbers on are the solvent tail:
+    // only their pairs with the atoms below it are computed here
+    const int solvent_start = solvent_numbers > 0 ? lj->N - solvent_numbers : -1;
     if (lt_is_orthorhombic(c))
-        rc = launch_rows<true>(lj, nl, c, r, pme_beta, need_atom_energy, need_virial, s);
+        rc = launch_rows<true>(lj, nl, c, r, pme_beta, need_atom_energy, need_virial, s, false,
+                               solvent_start);
     else
-        rc = launch_rows<false>(lj, nl, c, r, pme_beta, need_atom_energy, need_virial, s);
+        rc = launch_rows<false>(lj, nl, c, r, pme_beta, need_atom_energy, need_virial, s, false,
+                                solvent_start);
     if (rc) return rc;
     return finalize(lj, lj->N, nl->d_sidx, need_atom_energy, need_virial, frc, atom_energy,
                     atom_virial, atom_direct_pme_energy, s);

@@ -1,0 +1,112 @@
+// Pair arithmetic shared by the LJ + PME direct pair kernels (the plain
+// module and the solvent_LJ specialisation): the reference's pair functions
+// (SPONGE/Lennard_Jones_force/Lennard_Jones_force.h:45-75) evaluated
+// branch-free on one pair, and the partner sources of a cell's candidate
+// stream (LDS stage or sorted HBM arrays).
+#pragma once
+
+#include "internal.h"
+
+namespace sa {
+
+constexpr float kTwoOverSqrtPi = 1.1283791670218446f;  // Lennard_Jones_force.h:8
+
+// Arithmetic per in-cutoff pair on force-only steps (FAST, when beta *
+// cutoff <= kErfcxMax): 1/r from the hardware rsqrt and erfc(beta r) as
+// exp(-(beta r)^2) erfcx(beta r) with the polynomial erfcx, so one
+// exponential serves erfc and the force bracket.  Steps that need energies or
+// the virial use the reference's sqrtf, division and erfcf.
+constexpr int kPairStage = 3584;
+constexpr int kMaxLdsTypes = 64;  // the square (A, B) table lives in LDS: up to 64 LJ types
+
+// The periodic image of x nearest the point C (orthorhombic box).
+__device__ __forceinline__ float3 near_image(float3 x, float3 C, const LT6& c, const LT6& r) {
+    return make_float3(x.x - c.a11 * rintf((x.x - C.x) * r.a11),
+                       x.y - c.a22 * rintf((x.y - C.y) * r.a22),
+                       x.z - c.a33 * rintf((x.z - C.z) * r.a33));
+}
+
+// Partner data of a row entry: from the block's LDS stage (every position of
+// the stream staged) or, for the rare stream longer than the stage, from the
+// sorted arrays in HBM.  The two sources are separate code paths chosen once
+// per block, so that LDS reads stay ds_read and HBM reads global loads.
+struct PairSrc {
+    const float4* pos;
+    const int* typ;
+    const int* gid;
+};
+
+template <bool STAGED>
+__device__ __forceinline__ int src_index(int t, int nr, const int* r_first, const int* r_off) {
+    if (STAGED) return t;
+    return stream_to_sorted(t, nr, r_first, r_off);
+}
+
+// r^-3, r^-6 and r^-8 each rounded once to FP32 from the FP32 distance, as
+// the reference's powf (Lennard_Jones_force.h:45-75): the products are taken
+// in FP64.  Repeated FP32 squaring of 1/r instead biases r^-6 by a few ulp,
+// which the LJ energy (a difference of the r^-12 and r^-6 sums) turns into a
+// ~2e-6 relative error of the total.
+__device__ __forceinline__ void ref_powers(float rabs, float& pw3, float& pw6, float& pw8) {
+    const double ri = 1.0 / (double)rabs;
+    const double ri2 = ri * ri;
+    const double ri6 = ri2 * ri2 * ri2;
+    pw3 = (float)(ri2 * ri);
+    pw6 = (float)ri6;
+    pw8 = (float)(ri6 * ri2);
+}
+
+// Force (and energy / virial terms) of one pair, branch-free: pairs at or
+// beyond the cutoff contribute exact zeros (selects, so the non-finite
+// values of a far or padding pair never reach the sums).
+template <bool NEED_E, bool NEED_V, bool FAST>
+struct PairTerm {
+    float lx, ly, lz, elj, ec;
+    bool in;
+    __device__ __forceinline__ PairTerm(const float3& d, float r2, float qi, float qj, float2 ab,
+                                        float rc2_thr, float beta) {
+        in = r2 < rc2_thr;
+        // FAST: hardware rsqrt (1 ulp); otherwise the reference's sqrtf and
+        // division, which energies and virials need: the LJ energy is a
+        // difference of the r^-12 and r^-6 terms and amplifies a biased 1/r
+        float rinv, rabs, pw3, pw6, pw8;
+        if (FAST) {
+            rinv = __builtin_amdgcn_rsqf(r2);
+            rabs = r2 * rinv;
+            const float r2inv = rinv * rinv;
+            pw6 = r2inv * r2inv * r2inv;
+            pw8 = pw6 * r2inv;
+            pw3 = rinv * r2inv;
+        } else {
+            rabs = sqrtf(r2);
+            rinv = 1.0f / rabs;
+            ref_powers(rabs, pw3, pw6, pw8);
+        }
+        const float qq = qi * qj;
+        const float br = beta * rabs;
+        float erfc_br, bracket;
+        if (FAST) {
+            const float ex = __expf(-br * br);
+            const float ecx = erfcx_poly(br);
+            erfc_br = ex * ecx;
+            bracket = ex * fmaf(br, kTwoOverSqrtPi, ecx);
+        } else {
+            erfc_br = erfcf(br);
+            bracket = fmaf(br * kTwoOverSqrtPi, expf(-br * br), erfc_br);
+        }
+        const float flj = (ab.y - ab.x * pw6) * pw8;
+        const float fc = qq * pw3 * bracket;
+        const float f = in ? flj - fc : 0.f;
+        lx = f * d.x;
+        ly = f * d.y;
+        lz = f * d.z;
+        elj = 0.f;
+        ec = 0.f;
+        if (NEED_E) {
+            elj = in ? (0.083333333f * ab.x * pw6 - 0.166666667f * ab.y) * pw6 : 0.f;
+            ec = in ? qq * erfc_br / rabs : 0.f;
+        }
+    }
+};
+
+}  // namespace sa

@@ -66,6 +66,7 @@ class System:
     tolerance: float = 1e-5
     dt_ps: float = 0.002
     seed: int = 0
+    water_start: int = 0          # first atom of the waters (3 atoms each, O first)
 
     @property
     def n_atoms(self) -> int:
@@ -75,6 +76,15 @@ class System:
     def n_types(self) -> int:
         t = int(round((math.sqrt(8 * len(self.lj_A) + 1) - 1) / 2))
         return t
+
+    def residue_starts(self) -> np.ndarray:
+        """First atom of every residue: each water (3 atoms) and each ion
+        (1 atom) is one residue, in atom order."""
+        N, w0, nw = self.n_atoms, self.water_start, self.n_waters
+        starts = [i for i in range(w0)]
+        starts += list(range(w0, w0 + 3 * nw, 3))
+        starts += list(range(w0 + 3 * nw, N))
+        return np.asarray(starts, np.int32)
 
     def cell(self) -> np.ndarray:
         """LTMatrix3 {a11,a21,a22,a31,a32,a33} of the box (float32)."""
@@ -263,6 +273,29 @@ def build_water_ion_box(name, n_sites_grid, box_len, n_sites=None, n_na=0, n_cl=
                   lj_type=lj_type, lj_A=A, lj_B=B, excl_start=excl_start,
                   excl_num=excl_num, excl_list=excl_list, n_waters=nw,
                   cutoff=float(cutoff), skin=float(skin), fft=fft, seed=seed)
+
+
+def solvent_tail(s: System) -> System:
+    """The same system with the ions first and the waters after them: the
+    layout the reference's solvent_LJ expects (solute, then the water tail,
+    SPONGE/Lennard_Jones_force/solvent_LJ.cpp:195-217)."""
+    nw, N = s.n_waters, s.n_atoms
+    order = np.concatenate([np.arange(3 * nw, N), np.arange(3 * nw)])  # new -> old
+    new_of = np.empty(N, np.int64)
+    new_of[order] = np.arange(N)
+    excl_num = s.excl_num[order].astype(np.int32)
+    lists = []
+    for old in order:
+        a, n = s.excl_start[old], s.excl_num[old]
+        lists.append(np.sort(new_of[s.excl_list[a:a + n]]).astype(np.int32))
+    excl_list = np.concatenate(lists) if lists else np.zeros(0, np.int32)
+    excl_start = np.concatenate([[0], np.cumsum(excl_num)[:-1]]).astype(np.int32)
+    return dataclasses.replace(s, name=s.name + "_tail", crd=s.crd[order].copy(),
+                               vel=s.vel[order].copy(), charge=s.charge[order].copy(),
+                               mass=s.mass[order].copy(), lj_type=s.lj_type[order].copy(),
+                               excl_start=excl_start, excl_num=excl_num,
+                               excl_list=excl_list.astype(np.int32),
+                               water_start=N - 3 * nw)
 
 
 def config(idx: int, seed: int = None) -> System:

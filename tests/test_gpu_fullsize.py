@@ -7,7 +7,7 @@ reciprocal forces within 1e-5 relative RMS."""
 import numpy as np
 import pytest
 
-from gpu_helpers import DevSystem, export_pairs, rel_rms
+from gpu_helpers import DevSystem, check_total, export_pairs, rel, rel_rms
 
 pytestmark = pytest.mark.gpu
 
@@ -53,6 +53,70 @@ def test_forces_95k_against_oracle(dev, cfg3):
     assert rel_rms(f_direct.cpu().numpy(), ref_direct["frc"]) < FORCE_TOL
     assert rel_rms(f_excl.cpu().numpy(), ref_excl["frc"]) < FORCE_TOL
     assert rel_rms(f_recip.cpu().numpy(), ref_recip["frc"]) < FORCE_TOL
+
+
+def test_energies_and_virial_95k_against_fp64(dev, cfg3):
+    """Energy-and-virial evaluation of the 95k box: LJ and direct-Coulomb
+    energies, the excluded-pair correction, the reciprocal energy and every
+    virial against the FP64 oracle (within 1e-6 relative, or no further than
+    the FP32 restatement of the reference is from it), per-atom forces within
+    1e-5 of both."""
+    import torch
+    from oracle import oracle as O
+    from sponge_amd import abi
+    s = cfg3
+    print(f"case: config 3 energies, {s.n_atoms} atoms", flush=True)
+    beta = O.get_beta(s.cutoff, s.tolerance)
+    c, ser, flags = O.nl_build(s)
+    assert flags == 0
+    ref = O.lj_direct(s, c, ser, beta)
+    ref64 = O.direct64_list(s, beta, c, ser)
+    ex = O.pme_excluded(s, beta)
+    ex64 = O.excluded64(s, beta)
+    rec = O.PME(s, s.fft, beta).reciprocal()
+    rec64 = O.pme64(s.crd, s.charge, s.box, beta, s.fft)
+
+    d = DevSystem(s, dev)
+    box = abi.Box(s.box)
+    nl = abi.NeighborList(d.N, s.cutoff, s.skin, box)
+    nl.update(d.crd, update=abi.NL_FORCED, excl=d.excl)
+    lj = abi.LennardJones(d.N, s.n_types, s.lj_A, s.lj_B, s.lj_type, s.cutoff)
+    pm = abi.ParticleMesh(d.N, box, s.cutoff, fftx=s.fft[0], ffty=s.fft[1], fftz=s.fft[2])
+    N = d.N
+    f, e, v, edir = d.zeros(N, 3), d.zeros(N), d.zeros(N, 6), d.zeros(N)
+    fx, ecorr, vx = d.zeros(N, 3), d.zeros(N), d.zeros(N, 6)
+    fr, vr, pot = d.zeros(N, 3), d.zeros(6), d.zeros(1)
+    lj.direct_force(nl, d.crd, d.charge, f, box, beta, e, v, edir)
+    pm.excluded_force(d.crd, d.charge, d.excl, fx, ecorr, vx)
+    pm.reciprocal_force(d.crd, d.charge, fr, step=0, virial=vr, potential=pot)
+    torch.cuda.synchronize()
+    lj_short, _ = lj.step_print()
+    st = pm.step_print()
+    f, e, v, edir = f.cpu().numpy(), e.cpu().numpy(), v.cpu().numpy(), edir.cpu().numpy()
+    # direct space
+    assert rel_rms(f, ref["frc"]) < FORCE_TOL and rel_rms(f, ref64["frc"]) < FORCE_TOL
+    out = [check_total("E_LJ", lj_short, ref["lj_energy"].sum(dtype=np.float64), ref64["lj"]),
+           check_total("E_coulomb_direct", edir.sum(dtype=np.float64),
+                       ref["direct_energy"].sum(dtype=np.float64), ref64["coulomb"]),
+           check_total("virial_direct", v.sum(0, dtype=np.float64),
+                       ref["atom_virial"].sum(0, dtype=np.float64), ref64["virial"])]
+    # excluded-pair correction
+    assert rel_rms(fx.cpu().numpy(), ex64["frc"]) < FORCE_TOL
+    out.append(check_total("E_correction", ecorr.cpu().numpy().sum(dtype=np.float64),
+                           ex["atom_energy"].sum(dtype=np.float64), ex64["energy"]))
+    out.append(check_total("virial_correction", vx.cpu().numpy().sum(0, dtype=np.float64),
+                           ex["atom_virial"].sum(0, dtype=np.float64), ex64["virial"]))
+    # reciprocal
+    assert rel_rms(fr.cpu().numpy(), rec64["frc"]) < FORCE_TOL
+    out.append(check_total("E_reciprocal", st["reciprocal"], rec["reciprocal"],
+                           rec64["reciprocal"]))
+    out.append(check_total("virial_reciprocal", vr.cpu().numpy(), rec["virial"], rec64["virial"]))
+    assert rel(float(pot.cpu()[0]), rec["potential"]) < 1e-6
+    for name, (dg, dr, dgr) in zip(["E_LJ", "E_coulomb_direct", "virial_direct", "E_correction",
+                                     "virial_correction", "E_reciprocal", "virial_reciprocal"],
+                                    out):
+        print(f"{name}: |gpu-fp64| {dg:.2e}  |fp32 restatement-fp64| {dr:.2e}  "
+              f"|gpu-restatement| {dgr:.2e}", flush=True)
 
 
 @pytest.mark.parametrize("use_graph", [False, True], ids=["eager", "graph"])

@@ -197,6 +197,19 @@ def test_direct_restatement_vs_fp64_cfg1():
     assert abs(r32["direct_energy"].sum(dtype=np.float64) - r64["coulomb"]) < 1e-5 * abs(r64["coulomb"])
 
 
+def test_direct64_list_equals_brute_force():
+    # the list form sums the same pairs as the O(N^2) enumeration
+    s = S.config(1)
+    beta = O.get_beta(s.cutoff, s.tolerance)
+    c, ser, _ = O.nl_build(s)
+    a = O.direct64(s, beta)
+    b = O.direct64_list(s, beta, c, ser)
+    assert np.abs(a["frc"] - b["frc"]).max() < 1e-9 * np.abs(a["frc"]).max()
+    assert abs(a["lj"] - b["lj"]) < 1e-10 * abs(a["lj"])
+    assert abs(a["coulomb"] - b["coulomb"]) < 1e-10 * abs(a["coulomb"])
+    assert np.abs(a["virial"] - b["virial"]).max() < 1e-10 * np.abs(a["virial"]).max()
+
+
 def test_direct_virial_is_minus_strain_derivative():
     """virial_aa = -dE/d(eps_aa) under an affine stretch of box and atoms."""
     s = S.small_water_box(n_side=4, spacing=3.1, cutoff=5.5)
