@@ -96,38 +96,72 @@ def pair_counts(crd, counts, serial, box, cutoff, chunk=1 << 22):
     return n_in, int(len(i_idx))
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(s, gpu_rebuild_interval, budget_s=20.0):
     """The oracle's baseline build (the reference CPU path restated with its
     OpenMP + atomic structure, -O3 -ffast-math) on the same coordinates:
     force evaluations of the full system, list builds amortised over the
-    rebuild interval the GPU run observed."""
+    rebuild interval the GPU run observed; every host core, then one thread,
+    with the per-module split (list build, LJ + PME direct, excluded-pair
+    correction, reciprocal spread / FFT convolution / gather)."""
     from oracle import oracle as O  # the CPU baseline leg only
     lib = O.lib(fast=True)
-    threads = lib.orc_num_threads()
+    all_threads = lib.orc_num_threads()
     beta = O.get_beta(s.cutoff, s.tolerance)
-    t0 = time.perf_counter()
-    c, ser, _ = O.nl_build(s, fast=True)
-    t_nl = time.perf_counter() - t0
-    pm = O.PME(s, s.fft, beta, fast=True)
-    n = 0
-    t_force = 0.0
-    start = time.perf_counter()
-    while True:
+
+    def run(threads, budget, max_evals):
+        lib.orc_set_num_threads(threads)
         t0 = time.perf_counter()
-        O.lj_direct(s, c, ser, beta, need_energy=False, need_virial=False, fast=True)
-        pm.reciprocal(need_energy=False, need_virial=False)
-        t_force += time.perf_counter() - t0
-        n += 1
-        if time.perf_counter() - start > budget_s or n >= 50:
-            break
-    per_step = t_force / n + t_nl / max(gpu_rebuild_interval, 1.0)
-    ns_day = 86.4 * s.dt_ps / per_step
-    return dict(value=ns_day, unit="ns/day", cores=threads, kind="port",
-                sample=f"{n} force evaluations (LJ + PME direct, PME reciprocal with the "
-                       f"oracle's own FFT) of the 95,000-atom system plus one list build "
-                       f"({t_nl:.2f} s) amortised over {gpu_rebuild_interval:.1f} steps; "
-                       f"{t_force / n * 1e3:.0f} ms per force evaluation",
-                ms_per_step=per_step * 1e3)
+        c, ser, _ = O.nl_build(s, fast=True)
+        t_nl = time.perf_counter() - t0
+        pm = O.PME(s, s.fft, beta, fast=True)
+        split = dict(direct=0.0, excluded=0.0, spread=0.0, fft_convolution=0.0, gather=0.0)
+        n = 0
+        start = time.perf_counter()
+        while True:
+            t0 = time.perf_counter()
+            O.lj_direct(s, c, ser, beta, need_energy=False, need_virial=False, fast=True)
+            t1 = time.perf_counter()
+            O.pme_excluded(s, beta, need_energy=False, need_virial=False)
+            t2 = time.perf_counter()
+            pm.reciprocal(need_energy=False, need_virial=False)
+            rs = pm.last_seconds()
+            split["direct"] += t1 - t0
+            split["excluded"] += t2 - t1
+            for k in ("spread", "fft_convolution", "gather"):
+                split[k] += rs[k]
+            n += 1
+            if time.perf_counter() - start > budget or n >= max_evals:
+                break
+        per_eval = sum(split.values()) / n
+        per_step = per_eval + t_nl / max(gpu_rebuild_interval, 1.0)
+        ms = {k: v * 1e3 / n for k, v in split.items()}
+        ms["list_build_per_build"] = t_nl * 1e3
+        ms["list_build_amortised"] = t_nl * 1e3 / max(gpu_rebuild_interval, 1.0)
+        return dict(ns_day=86.4 * s.dt_ps / per_step, ms_per_step=per_step * 1e3,
+                    force_evals_per_s=1.0 / per_eval, evals=n, threads=threads, ms_split=ms)
+
+    full = run(all_threads, budget_s, 50)
+    one = run(1, 0.0, 1)
+    lib.orc_set_num_threads(all_threads)
+    return dict(value=full["ns_day"], unit="ns/day", cores=all_threads, kind="port",
+                sample=f"{full['evals']} force evaluations (LJ + PME direct, excluded-pair "
+                       f"correction, PME reciprocal with the oracle's own FFT) of the "
+                       f"{s.n_atoms}-atom system on {all_threads} threads plus one list build "
+                       f"amortised over {gpu_rebuild_interval:.1f} steps; one evaluation and one "
+                       f"build on 1 thread",
+                ms_per_step=full["ms_per_step"], force_evals_per_s=full["force_evals_per_s"],
+                ms_split=full["ms_split"], cpu_model=cpu_model(), one_thread=one)
 
 
 def learned_form(md, s, torch, abi, n_train=40, n_heldout=16, stride=5, epochs=30, lr=2e-3,
@@ -269,6 +303,92 @@ def batched_form(torch, abi, S, steps=500, steps_per_launch=10, n_boxes=48):
     return out
 
 
+def plain_path(s, torch, abi, lib, steps, warmup, breakdown_steps, eager=False):
+    """The plain MD step of one system: fixed equilibration of the seeded
+    lattice (outside every timed region), FFT tiling autotune, W warmup
+    steps, K timed steps (graph replay, no timers), K more steps with the
+    pair-kernel timer, a short eager pass with every timer; pair statistics
+    of the list at the end."""
+    md = abi.MDSystem(s)
+    nl_h, _, _ = md.handles()
+    torch.cuda.synchronize()
+    # set-up of the synthetic input (outside the timed region, independent of
+    # --warmup): the seeded lattice start relaxes to the stated 300 K state
+    # under velocity rescaling, every step at first (so the energy the
+    # lattice releases cannot heat the box), then every 10 steps
+    md.equilibrate(EQUIL_STRONG, temperature=300.0, every=1)
+    md.equilibrate(EQUIL_STEPS - EQUIL_STRONG, temperature=300.0, every=10)
+    fft_tiling = md.autotune(5)
+    torch.cuda.synchronize()
+
+    use_graph = not eager
+    md.run(max(1, warmup), use_graph=use_graph)
+    torch.cuda.synchronize()
+    rb0 = C_rebuilds(lib, nl_h)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    md.run(steps, use_graph=use_graph)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    rb1 = C_rebuilds(lib, nl_h)
+
+    # pair-kernel timing pass: the same number of steps, events around the
+    # pair kernel in every replay
+    abi.set_kernel_timing_filter(["lj_pairs"])
+    abi.reset_kernel_timing()
+    abi.set_kernel_timing(True)
+    md.run(1, use_graph=use_graph)  # capture outside the timed pass
+    torch.cuda.synchronize()
+    abi.reset_kernel_timing()
+    t2 = time.perf_counter()
+    md.run(steps, use_graph=use_graph)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    lj_ms, lj_n = abi.kernel_timing("lj_pairs")
+    abi.set_kernel_timing(False)
+
+    # breakdown of the other kernels: a short eager pass with every timer
+    kernel = {}
+    if breakdown_steps > 0:
+        abi.set_kernel_timing_filter(None)
+        abi.reset_kernel_timing()
+        abi.set_kernel_timing(True)
+        md.run(breakdown_steps, use_graph=False)
+        torch.cuda.synchronize()
+        abi.set_kernel_timing(False)
+        kernel = {k: abi.kernel_timing(k) for k in TIMERS}
+
+    ms_per_step = (t1 - t0) * 1e3 / steps
+    rebuilds = rb1 - rb0
+    rebuild_interval = steps / max(rebuilds, 1)
+
+    # pair statistics of the current list
+    N = s.n_atoms
+    M = 1200
+    counts = torch.zeros(N, dtype=torch.int32, device="cuda")
+    serial = torch.zeros(N * M, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()  # the MD stream is not torch's: fill first
+    rc = lib.sponge_nl_export_half_list(nl_h, abi._ptr(counts), abi._ptr(serial), M)
+    assert rc == 0
+    crd, _, _ = md.state()
+    torch.cuda.synchronize()
+    n_in, n_list = pair_counts(crd, counts.cpu().numpy(), serial.view(N, M).cpu().numpy(), s.box,
+                               s.cutoff)
+    del counts, serial
+    assert lj_n == steps, (lj_n, steps)
+    lj_avg_s = lj_ms * 1e-3 / max(lj_n, 1)
+    flops = pair_kernel_flops(n_in, n_list)
+    achieved = flops / lj_avg_s / 1e12
+    return dict(md=md, crd=crd, ms_per_step=ms_per_step, ns_day=86.4 * s.dt_ps / (ms_per_step * 1e-3),
+                rebuild_interval=rebuild_interval, fft_tiling=list(fft_tiling),
+                lj_avg_us=lj_avg_s * 1e6, flops=flops, achieved=achieved, n_in=n_in,
+                n_list=n_list, ms_per_step_timing_pass=(t3 - t2) * 1e3 / steps,
+                kernel={k: (v[0] * 1e3 / v[1] if v[1] else None) for k, v in kernel.items()},
+                pme_hbm=pme_hbm_rates(kernel, N, s.fft), launch=(
+                    "hip graph replay, reciprocal half on a second stream" if use_graph
+                    else "eager, reciprocal half on a second stream"))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -282,6 +402,10 @@ def main():
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned multiple-time-step form")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched form")
+    ap.add_argument("--no-cfg2", action="store_true",
+                    help="skip the 24,000-atom sub-line (force evaluations/s, pair-kernel VALU "
+                         "fraction, spread/gather GB/s)")
+    ap.add_argument("--cfg2-steps", type=int, default=1000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -298,120 +422,73 @@ def main():
     from sponge_amd import abi
     from sponge_amd import system as S
 
-    s = S.config(args.config)
-    md = abi.MDSystem(s)
-    nl_h, lj_h, pm_h = md.handles()
     lib = abi.load()
-    torch.cuda.synchronize()
-
-    # set-up of the synthetic input (outside the timed region, independent of
-    # --warmup): the seeded lattice start relaxes to the stated 300 K state
-    # under velocity rescaling, every step at first (so the energy the
-    # lattice releases cannot heat the box), then every 10 steps
-    md.equilibrate(EQUIL_STRONG, temperature=300.0, every=1)
-    md.equilibrate(EQUIL_STEPS - EQUIL_STRONG, temperature=300.0, every=10)
-    # set-up: the FFT tiling autotuned on this system (not timed)
-    fft_tiling = md.autotune(5)
-    torch.cuda.synchronize()
-
-    # warmup: W untimed steps of the timed schedule (graph capture, clocks)
-    use_graph = not args.eager
-    md.run(max(1, args.warmup), use_graph=use_graph)
-    torch.cuda.synchronize()
-    rb0 = C_rebuilds(lib, nl_h)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    md.run(args.steps, use_graph=use_graph)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    rb1 = C_rebuilds(lib, nl_h)
-
-    # pair-kernel timing pass: the same number of steps, events around the
-    # pair kernel in every replay
-    abi.set_kernel_timing_filter(["lj_pairs"])
-    abi.reset_kernel_timing()
-    abi.set_kernel_timing(True)
-    md.run(1, use_graph=use_graph)  # capture outside the timed pass
-    torch.cuda.synchronize()
-    abi.reset_kernel_timing()
-    t2 = time.perf_counter()
-    md.run(args.steps, use_graph=use_graph)
-    torch.cuda.synchronize()
-    t3 = time.perf_counter()
-    lj_ms, lj_n = abi.kernel_timing("lj_pairs")
-    abi.set_kernel_timing(False)
-
-    # breakdown of the other kernels: a short eager pass with every timer
-    kernel = {}
-    if args.breakdown_steps > 0:
-        abi.set_kernel_timing_filter(None)
-        abi.reset_kernel_timing()
-        abi.set_kernel_timing(True)
-        md.run(args.breakdown_steps, use_graph=False)
-        torch.cuda.synchronize()
-        abi.set_kernel_timing(False)
-        kernel = {k: abi.kernel_timing(k) for k in TIMERS}
-
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    ns_day = 86.4 * s.dt_ps / (ms_per_step * 1e-3)
-    rebuilds = rb1 - rb0
-    rebuild_interval = args.steps / max(rebuilds, 1)
-
-    # pair statistics of the current list
+    s = S.config(args.config)
+    r = plain_path(s, torch, abi, lib, args.steps, args.warmup, args.breakdown_steps, args.eager)
+    md = r["md"]
     N = s.n_atoms
-    M = 1200
-    counts = torch.zeros(N, dtype=torch.int32, device="cuda")
-    serial = torch.zeros(N * M, dtype=torch.int32, device="cuda")
-    torch.cuda.synchronize()  # the MD stream is not torch's: fill first
-    rc = lib.sponge_nl_export_half_list(nl_h, abi._ptr(counts), abi._ptr(serial), M)
-    assert rc == 0
-    crd, _, _ = md.state()
-    torch.cuda.synchronize()
-    n_in, n_list = pair_counts(crd, counts.cpu().numpy(), serial.view(N, M).cpu().numpy(), s.box,
-                               s.cutoff)
-
-    assert lj_n == args.steps, (lj_n, args.steps)
-    lj_avg_s = lj_ms * 1e-3 / max(lj_n, 1)
-    flops = pair_kernel_flops(n_in, n_list)
-    achieved = flops / lj_avg_s / 1e12
     traffic = PAIR_KERNEL_TRAFFIC_BYTES if args.config == 3 else None
-    roofline = dict(bound="valu", achieved=achieved, peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
-                    frac=achieved / FP32_PEAK_TFLOPS, traffic=traffic,
+    roofline = dict(bound="valu", achieved=r["achieved"], peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
+                    frac=r["achieved"] / FP32_PEAK_TFLOPS, traffic=traffic,
                     traffic_note="HBM bytes per launch from rocprofv3 PMC runs (copy-calibrated)",
-                    kernel="lj_pairs", flop_per_launch=flops, pairs_in_cutoff=n_in,
-                    list_entries=n_list, avg_launch_us=lj_avg_s * 1e6,
-                    frac_of_unpacked_peak=achieved / FP32_PEAK_TFLOPS_UNPACKED,
+                    kernel="lj_pairs", flop_per_launch=r["flops"], pairs_in_cutoff=r["n_in"],
+                    list_entries=r["n_list"], avg_launch_us=r["lj_avg_us"],
+                    frac_of_unpacked_peak=r["achieved"] / FP32_PEAK_TFLOPS_UNPACKED,
                     timing="HIP events on the MD stream around every pair-kernel launch of a "
                            "second pass of the same steps (timers split the graph at the pair "
                            "kernel and keep the step on one stream)",
-                    ms_per_step_timing_pass=(t3 - t2) * 1e3 / args.steps)
+                    ms_per_step_timing_pass=r["ms_per_step_timing_pass"])
 
-    out = dict(metric=METRIC, value=ns_day, unit="ns/day", n_gpus=1, steps=args.steps,
-               warmup=args.warmup, ms_per_step=ms_per_step, higher_is_better=True,
+    out = dict(metric=METRIC, value=r["ns_day"], unit="ns/day", n_gpus=1, steps=args.steps,
+               warmup=args.warmup, ms_per_step=r["ms_per_step"], higher_is_better=True,
                scaling="weak", vs_baseline=None, dtype="f32", data="synthetic",
                config=dict(workload="water_ions_95k NVE step (31000 rigid TIP3P-like waters + "
                                     "1000 Na+ + 1000 Cl-, L=99 A, rc=10 A, skin=2 A, PME 96^3 "
-                                    "order 4, dt=2 fs, SETTLE)",
+                                    "order 4, dt=2 fs, SETTLE)" if args.config == 3 else s.name,
                            atoms=N, box_A=float(s.box[0]), cutoff_A=s.cutoff, skin_A=s.skin,
                            pme_grid=list(s.fft), dt_fs=s.dt_ps * 1e3, parallelism="1 GPU",
                            equilibration_steps=EQUIL_STEPS),
                roofline=roofline,
-               force_evals_per_s=1e3 / ms_per_step,
-               list_rebuild_interval_steps=rebuild_interval,
-               launch=("hip graph replay, reciprocal half on a second stream" if use_graph
-                       else "eager, reciprocal half on a second stream"),
-               pme_fft_tiling=list(fft_tiling),
-               kernel_avg_us_eager_pass={k: (v[0] * 1e3 / v[1] if v[1] else None)
-                                         for k, v in kernel.items()},
-               pme_hbm=pme_hbm_rates(kernel, N, s.fft))
+               force_evals_per_s=1e3 / r["ms_per_step"],
+               list_rebuild_interval_steps=r["rebuild_interval"],
+               launch=r["launch"],
+               pme_fft_tiling=r["fft_tiling"],
+               kernel_avg_us_eager_pass=r["kernel"],
+               pme_hbm=r["pme_hbm"])
     if not args.no_learned:
         out["learned"] = learned_form(md, s, torch, abi)
+    crd = r["crd"]
+    md.clear()
+    del r
+    if not args.no_cfg2 and args.config == 3:
+        s2 = S.config(2)
+        r2 = plain_path(s2, torch, abi, lib, args.cfg2_steps, args.warmup, args.breakdown_steps,
+                        args.eager)
+        hbm = r2["pme_hbm"]
+        out["cfg2"] = dict(
+            workload="water_24k NVE step (8000 rigid waters, L=62.1 A, rc=10 A, skin=2 A, "
+                     "PME 64^3 order 4, dt=2 fs, SETTLE)",
+            atoms=s2.n_atoms, steps=args.cfg2_steps, ms_per_step=r2["ms_per_step"],
+            ns_day=r2["ns_day"], force_evals_per_s=1e3 / r2["ms_per_step"],
+            list_rebuild_interval_steps=r2["rebuild_interval"],
+            pair_kernel=dict(avg_launch_us=r2["lj_avg_us"], achieved_TFLOP_s=r2["achieved"],
+                             valu_frac_of_fp32_peak=r2["achieved"] / FP32_PEAK_TFLOPS,
+                             valu_frac_of_unpacked_peak=r2["achieved"] / FP32_PEAK_TFLOPS_UNPACKED,
+                             flop_per_launch=r2["flops"], pairs_in_cutoff=r2["n_in"],
+                             list_entries=r2["n_list"]),
+            spread_GB_s=hbm.get("spread", {}).get("GB_s"),
+            gather_GB_s=hbm.get("gather", {}).get("GB_s"),
+            fft_convolution_GB_s=hbm.get("fft_convolution", {}).get("GB_s"),
+            pme_hbm=hbm, kernel_avg_us_eager_pass=r2["kernel"])
+        r2["md"].clear()
+        del r2
     if not args.no_batched:
         out["batched"] = batched_form(torch, abi, S)
     if not args.no_cpu_baseline:
         import dataclasses
         s_now = dataclasses.replace(s, crd=crd)
-        out["cpu_baseline"] = cpu_baseline(s_now, rebuild_interval, args.cpu_budget)
+        out["cpu_baseline"] = cpu_baseline(s_now, out["list_rebuild_interval_steps"],
+                                           args.cpu_budget)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))
