@@ -564,6 +564,11 @@ int sponge_md_set_state(sponge_md* md, const float* crd, const float* vel);
 /* The module handles the driver owns (for timing and list export). */
 int sponge_md_handles(sponge_md* md, sponge_nl** nl, sponge_lj** lj, sponge_pme** pme);
 
+/* The driver's list is rebuilt from the coordinates of the next step, alone or
+ * batched (NEIGHBOR_LIST::Update with update = FORCED, neighbor_list.cpp:
+ * 983-998, for the list the driver holds). */
+int sponge_md_request_list_update(sponge_md* md);
+
 /* sponge_pme_autotune on the driver's PME module and current coordinates;
  * tiles[3] (host, may be NULL) gets the choice.  Synchronises. */
 int sponge_md_autotune(sponge_md* md, int reps, int* tiles);

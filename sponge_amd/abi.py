@@ -147,6 +147,7 @@ SIGNATURES = {
     "sponge_md_energies": (_I, [_P, C.POINTER(C.c_double)]),
     "sponge_md_get_state": (_I, [_P, _FP, _FP, _FP]),
     "sponge_md_set_state": (_I, [_P, _FP, _FP]),
+    "sponge_md_request_list_update": (_I, [_P]),
     "sponge_md_handles": (_I, [_P, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),
     "sponge_md_clear": (_I, [_P]),
 }
@@ -755,6 +756,18 @@ class MDSystem:
         o = NLOptions()
         _check(self.lib.sponge_nl_get_options(nl, C.byref(o)), "sponge_nl_get_options")
         return {k: getattr(o, k) for k, _ in NLOptions._fields_}
+
+    def request_list_update(self):
+        """The next step (alone or batched) rebuilds the list."""
+        _check(self.lib.sponge_md_request_list_update(self.h), "sponge_md_request_list_update")
+
+    def pme_grid(self):
+        """The PME grid of the driver's reciprocal module."""
+        _, _, pm = self.handles()
+        b = C.c_float()
+        f = (C.c_int * 3)()
+        _check(self.lib.sponge_pme_info(pm, C.byref(b), f), "sponge_pme_info")
+        return tuple(f)
 
     def handles(self):
         nl, lj, pm = C.c_void_p(), C.c_void_p(), C.c_void_p()

@@ -1,29 +1,57 @@
 // Batched driver: many independent systems (each a sponge_md with its own
-// stream, neighbor list, PME grid and tuned FFT tiling) stepped together by
-// one HIP graph.  The capture forks every system's stream from one origin
-// stream, records k MD steps of each system on its own stream and joins them
-// back, so one graph launch advances every system by k steps and the
-// systems' kernels run concurrently as independent branches of the graph.
-// No counterpart in the reference (one process runs one system there).
+// neighbor list, PME grid and tuned FFT tiling) advanced together with ONE
+// dispatch per kernel of the MD step.
+//
+// Every kernel of the force-only step is a device body behind SA_LAUNCH
+// (batch_launch.h).  The driver records one step of each system -- the same
+// host code path that launches it alone, with a LaunchRecorder installed --
+// and checks that every system produced the same sequence of kernels.  For
+// each position of that sequence it keeps, in one device buffer, the prefix
+// of the systems' workgroup counts and each system's packed arguments; one
+// launch of k_batched then covers every system (workgroup b runs the body for
+// the system whose range holds b).  k steps of these launches are captured
+// into one HIP graph and replayed.  No counterpart in the reference (one
+// process runs one system there).
+#include <algorithm>
 #include <map>
 #include <tuple>
 #include <vector>
 
 #include "internal.h"
+#include "batch_launch.h"
 #include "timing.h"
 
 using namespace sa;
 
+thread_local sa::LaunchRecorder* sa::g_recorder = nullptr;
+
+namespace {
+
+// one kernel of the recorded step, over every system
+struct BatchPos {
+    const void* fn = nullptr;
+    dim3 block;
+    size_t shmem = 0;      // the largest of the systems'
+    int total_blocks = 0;  // sum over the systems
+    int stride = 0;        // bytes per system in the argument array
+    size_t start_off = 0;  // device buffer offsets
+    size_t args_off = 0;
+};
+
+}  // namespace
+
 struct sponge_batch {
     std::vector<sponge_md*> md;
-    hipStream_t stream = nullptr;  // origin of the capture and of the launches
-    hipEvent_t fork = nullptr;
-    std::vector<hipEvent_t> done;  // per system: its work before a launch
-    hipEvent_t after = nullptr;    // end of a launch
+    hipStream_t stream = nullptr;  // the batched launches
+    std::vector<hipEvent_t> done;  // per system: its earlier work
+    hipEvent_t after = nullptr;    // end of a run
+    std::vector<BatchPos> pos;
+    char* d_buf = nullptr;         // starts and argument tuples of every position
+    bool recorded = false;
+    std::vector<int> generation;   // each system's list generation at the recording
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int graph_steps = 0;
-    std::vector<int> generation;  // each system's list generation at the capture
 };
 
 namespace {
@@ -36,29 +64,143 @@ void batch_free_graph(sponge_batch* b) {
     b->graph_steps = 0;
 }
 
-void batch_free(sponge_batch* b) {
+void batch_free_record(sponge_batch* b) {
     batch_free_graph(b);
+    if (b->d_buf) (void)hipFree(b->d_buf);
+    b->d_buf = nullptr;
+    b->pos.clear();
+    b->recorded = false;
+}
+
+void batch_free(sponge_batch* b) {
+    batch_free_record(b);
     for (auto e : b->done)
         if (e) (void)hipEventDestroy(e);
     b->done.clear();
-    if (b->fork) (void)hipEventDestroy(b->fork);
     if (b->after) (void)hipEventDestroy(b->after);
     if (b->stream) (void)hipStreamDestroy(b->stream);
-    b->fork = b->after = nullptr;
+    b->after = nullptr;
     b->stream = nullptr;
 }
 
-int capture(sponge_batch* b, int k) {
+// One step of system m recorded instead of launched.  The system's stream is
+// captured meanwhile: a kernel or copy of the step that does not go through
+// SA_LAUNCH would land in that capture, and is reported.
+int record_system(sponge_md* m, LaunchRecorder& rec) {
+    hipStream_t s = md_stream(m);
+    SA_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    g_recorder = &rec;
+    const int rc = md_record_steps(m, 1);
+    g_recorder = nullptr;
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    size_t stray = 0;
+    if (g) {
+        (void)hipGraphGetNodes(g, nullptr, &stray);
+        (void)hipGraphDestroy(g);
+    }
+    (void)hipGetLastError();
+    if (rc) return rc;
+    if (e != hipSuccess) return (int)e;
+    return stray ? SPONGE_AMD_ERROR_NOT_IMPLEMENTED : 0;
+}
+
+int batch_record(sponge_batch* b) {
+    batch_free_record(b);
+    const int n = (int)b->md.size();
+    // the tile spread's workgroup size must be one for every system
+    int waves = 1 << 30;
+    for (sponge_md* m : b->md) {
+        sponge_nl* nl = nullptr;
+        sponge_lj* lj = nullptr;
+        sponge_pme* pme = nullptr;
+        const int rc = sponge_md_handles(m, &nl, &lj, &pme);
+        if (rc) return rc;
+        waves = std::min(waves, pme_spread_waves(pme));
+    }
+    std::vector<LaunchRecorder> rec(n);
+    for (int i = 0; i < n; ++i) {
+        sponge_md* m = b->md[i];
+        sponge_nl* nl = nullptr;
+        sponge_lj* lj = nullptr;
+        sponge_pme* pme = nullptr;
+        int rc = sponge_md_handles(m, &nl, &lj, &pme);
+        if (!rc) pme_set_spread_waves(pme, waves);
+        if (!rc) rc = record_system(m, rec[i]);
+        if (rc) return rc;
+    }
+    // every system must have recorded the same kernels with the same blocks
+    const size_t np = rec[0].launches.size();
+    for (int i = 1; i < n; ++i) {
+        if (rec[i].launches.size() != np) return SPONGE_AMD_ERROR_NOT_IMPLEMENTED;
+        for (size_t p = 0; p < np; ++p) {
+            const RecordedLaunch& a = rec[0].launches[p];
+            const RecordedLaunch& c = rec[i].launches[p];
+            if (a.fn != c.fn || a.block.x != c.block.x || a.block.y != c.block.y ||
+                a.block.z != c.block.z || a.args.size() != c.args.size())
+                return SPONGE_AMD_ERROR_NOT_IMPLEMENTED;
+        }
+    }
+    // one device buffer: per position the block prefix [n+1] and the tuples
+    std::vector<char> host;
+    auto align16 = [](size_t x) { return (x + 15) / 16 * 16; };
+    b->pos.resize(np);
+    for (size_t p = 0; p < np; ++p) {
+        BatchPos& bp = b->pos[p];
+        const RecordedLaunch& r0 = rec[0].launches[p];
+        bp.fn = r0.fn;
+        bp.block = r0.block;
+        bp.stride = (int)align16(r0.args.size());
+        std::vector<int> start(n + 1, 0);
+        for (int i = 0; i < n; ++i) {
+            const RecordedLaunch& r = rec[i].launches[p];
+            start[i + 1] = start[i] + r.blocks;
+            bp.shmem = std::max(bp.shmem, r.shmem);
+        }
+        bp.total_blocks = start[n];
+        bp.start_off = align16(host.size());
+        host.resize(bp.start_off + sizeof(int) * (n + 1));
+        std::memcpy(host.data() + bp.start_off, start.data(), sizeof(int) * (n + 1));
+        bp.args_off = align16(host.size());
+        host.resize(bp.args_off + (size_t)bp.stride * n, 0);
+        for (int i = 0; i < n; ++i) {
+            const RecordedLaunch& r = rec[i].launches[p];
+            std::memcpy(host.data() + bp.args_off + (size_t)bp.stride * i, r.args.data(),
+                        r.args.size());
+        }
+    }
+    if (host.empty()) return SPONGE_AMD_ERROR_NOT_IMPLEMENTED;
+    if (hipMalloc(&b->d_buf, host.size()) != hipSuccess) {
+        b->d_buf = nullptr;
+        return SPONGE_AMD_ERROR_MALLOC_FAILED;
+    }
+    SA_CHECK(hipMemcpyAsync(b->d_buf, host.data(), host.size(), hipMemcpyHostToDevice, b->stream));
+    SA_CHECK(hipStreamSynchronize(b->stream));
+    b->generation.clear();
+    for (sponge_md* m : b->md) b->generation.push_back(md_generation(m));
+    b->recorded = true;
+    return 0;
+}
+
+// k batched steps on b->stream (eager, or into the capture in progress)
+int batch_issue(sponge_batch* b, int k) {
+    const int n = (int)b->md.size();
+    for (int step = 0; step < k; ++step)
+        for (const BatchPos& bp : b->pos) {
+            const int* start = reinterpret_cast<const int*>(b->d_buf + bp.start_off);
+            const char* args = b->d_buf + bp.args_off;
+            int nn = n, stride = bp.stride;
+            void* kargs[] = {(void*)&start, (void*)&nn, (void*)&args, (void*)&stride};
+            SA_CHECK(hipLaunchKernel(bp.fn, dim3(bp.total_blocks), bp.block, kargs, bp.shmem,
+                                     b->stream));
+        }
+    return 0;
+}
+
+int batch_capture(sponge_batch* b, int k) {
     batch_free_graph(b);
     SA_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    int rc = (int)hipEventRecord(b->fork, b->stream);
-    for (size_t i = 0; i < b->md.size() && !rc; ++i) {
-        hipStream_t s = md_stream(b->md[i]);
-        rc = (int)hipStreamWaitEvent(s, b->fork, 0);
-        if (!rc) rc = md_record_steps(b->md[i], k);
-        if (!rc) rc = (int)hipEventRecord(b->done[i], s);
-        if (!rc) rc = (int)hipStreamWaitEvent(b->stream, b->done[i], 0);
-    }
+    const int rc = batch_issue(b, k);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(b->stream, &g);
     if (rc || e != hipSuccess) {
@@ -69,8 +211,6 @@ int capture(sponge_batch* b, int k) {
     b->graph = g;
     SA_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     b->graph_steps = k;
-    b->generation.clear();
-    for (sponge_md* m : b->md) b->generation.push_back(md_generation(m));
     return 0;
 }
 
@@ -87,7 +227,6 @@ int sponge_batch_initial(sponge_batch** out, int n_systems, sponge_md* const* sy
     b->md.assign(systems, systems + n_systems);
     b->done.assign(n_systems, nullptr);
     bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess &&
-              hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess &&
               hipEventCreateWithFlags(&b->after, hipEventDisableTiming) == hipSuccess;
     for (int i = 0; i < n_systems && ok; ++i)
         ok = hipEventCreateWithFlags(&b->done[i], hipEventDisableTiming) == hipSuccess;
@@ -129,54 +268,80 @@ int sponge_batch_autotune(sponge_batch* b, int reps, int* tiles) {
         if (tiles)
             for (int a = 0; a < 3; ++a) tiles[3 * i + a] = t[a];
     }
-    batch_free_graph(b);
+    batch_free_record(b);  // the tilings are launch arguments: record again
     return 0;
 }
 
 int sponge_batch_run(sponge_batch* b, int steps, int steps_per_launch) {
     if (!b || steps < 0 || steps_per_launch < 1) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
-    if (g_timing) return SPONGE_AMD_ERROR_CONFLICTING_COMMAND;  // timers cannot cut this capture
-    for (sponge_md* m : b->md)
-        if (!md_graphable(m)) {
-            // first steps build the lists; other schedules run system by system
-            for (sponge_md* q : b->md) {
-                const int rc = sponge_md_run(q, steps, 0, 0);
-                if (rc) return rc;
-            }
-            return 0;
-        }
-    const int launches = steps / steps_per_launch, rest = steps - launches * steps_per_launch;
-    if (launches > 0) {
-        if (b->graph_steps != steps_per_launch) {
-            const int rc = capture(b, steps_per_launch);
+    if (g_timing) return SPONGE_AMD_ERROR_CONFLICTING_COMMAND;  // timers cannot join this capture
+    int done_steps = 0;
+    // the first step of a system builds its list; other schedules (an impulse
+    // interval, a fixed refresh interval, the surrogate) run system by system
+    bool all = true;
+    for (sponge_md* m : b->md) all = all && md_graphable(m);
+    if (!all) {
+        for (sponge_md* m : b->md) {
+            const int rc = sponge_md_run(m, 1, 0, 0);
             if (rc) return rc;
         }
-        // the launches follow each system's earlier work on its own stream
+        done_steps = 1;
+        for (sponge_md* m : b->md)
+            if (!md_graphable(m)) {
+                for (sponge_md* q : b->md) {
+                    const int rc = sponge_md_run(q, steps - done_steps, 0, 0);
+                    if (rc) return rc;
+                }
+                return 0;
+            }
+    }
+    while (done_steps < steps) {
+        if (!b->recorded) {
+            // the systems' earlier work completes before the recording reads
+            // their state
+            for (sponge_md* m : b->md) SA_CHECK(hipStreamSynchronize(md_stream(m)));
+            const int rc = batch_record(b);
+            if (rc) return rc;
+        }
+        const int left = steps - done_steps;
+        const int k = std::min(left, steps_per_launch);
+        const int launches = left / steps_per_launch;
+        // the batched work follows each system's earlier work on its stream
         for (size_t i = 0; i < b->md.size(); ++i) {
             SA_CHECK(hipEventRecord(b->done[i], md_stream(b->md[i])));
             SA_CHECK(hipStreamWaitEvent(b->stream, b->done[i], 0));
         }
-        for (int l = 0; l < launches; ++l) SA_CHECK(hipGraphLaunch(b->exec, b->stream));
+        int advanced = 0;
+        if (launches > 0) {
+            if (b->graph_steps != steps_per_launch) {
+                const int rc = batch_capture(b, steps_per_launch);
+                if (rc) return rc;
+            }
+            // one graph launch per steps_per_launch steps; overflow checks (a
+            // host synchronisation) at the launch boundaries
+            SA_CHECK(hipGraphLaunch(b->exec, b->stream));
+            advanced = steps_per_launch;
+        } else {
+            const int rc = batch_issue(b, k);
+            if (rc) return rc;
+            advanced = k;
+        }
         SA_CHECK(hipEventRecord(b->after, b->stream));
         for (sponge_md* m : b->md) {
             SA_CHECK(hipStreamWaitEvent(md_stream(m), b->after, 0));
             const long long first = md_step_count(m);
-            md_advance(m, launches * steps_per_launch);
-            // each system's neighbor-list overflow check over the replayed steps
+            md_advance(m, advanced);
+            // each system's neighbor-list overflow check over these steps
             const int rc = md_overflow_check(m, first, md_step_count(m) - 1);
             if (rc) return rc;
         }
+        done_steps += advanced;
         for (size_t i = 0; i < b->md.size(); ++i)
             if (md_generation(b->md[i]) != b->generation[i]) {
-                batch_free_graph(b);  // a re-initialised list: capture again
+                batch_free_record(b);  // a re-initialised list: record again
                 break;
             }
     }
-    for (sponge_md* m : b->md)
-        if (rest) {
-            const int rc = sponge_md_run(m, rest, 0, 0);
-            if (rc) return rc;
-        }
     return 0;
 }
 

@@ -186,6 +186,10 @@ hipStream_t pme_stream(const sponge_pme* p);
 bool pme_calculates_reciprocal(const sponge_pme* p);
 bool pme_updates_on(const sponge_pme* p, int step);  // the reciprocal part is evaluated this step
 void pme_set_stream(sponge_pme* p, hipStream_t s);
+// wavefronts (tile copies) per workgroup of the tile spread, and a smaller
+// count for launches that must share one workgroup size (the batched driver)
+int pme_spread_waves(const sponge_pme* p);
+void pme_set_spread_waves(sponge_pme* p, int waves);
 int rsur_atoms(const sponge_rsur* h);
 float rsur_cutoff(const sponge_rsur* h);
 

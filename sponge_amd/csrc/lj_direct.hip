@@ -26,6 +26,7 @@
 
 #include "internal.h"
 #include "pair_term.h"
+#include "batch_launch.h"
 #include "timing.h"
 
 using namespace sa;
@@ -247,7 +248,7 @@ __device__ __forceinline__ void lj_row(int p, int lane, int n, const unsigned sh
 }
 
 template <bool ORTHO, bool NEED_E, bool NEED_V, bool FAST, bool SOLV = false>
-__global__ void __launch_bounds__(1024) k_lj_cells(
+__device__ __forceinline__ void lj_cells_body(int sa_block, 
     const int* __restrict__ start, const int* __restrict__ nranges, int max_ranges,
     const int* __restrict__ g_rfirst, const int* __restrict__ g_roff,
     const int* __restrict__ list_count, const unsigned short* __restrict__ list, int cap,
@@ -263,7 +264,7 @@ __global__ void __launch_bounds__(1024) k_lj_cells(
     float2* cab = reinterpret_cast<float2*>(ctyp + (kGid ? 2 : 1) * stage);
     __shared__ int r_first[kMaxRanges];
     __shared__ int r_off[kMaxRanges + 1];
-    const int cell = blockIdx.x;
+    const int cell = sa_block;
     const int a0 = start[cell], a1 = start[cell + 1];
     if (a0 == a1) return;
     // Force-only steps in an orthorhombic box wider than twice (half the
@@ -332,18 +333,31 @@ __global__ void __launch_bounds__(1024) k_lj_cells(
     }
 }
 
+template <bool ORTHO, bool NEED_E, bool NEED_V, bool FAST, bool SOLV = false>
+__global__ void __launch_bounds__(1024) k_lj_cells(
+    const int* __restrict__ start, const int* __restrict__ nranges, int max_ranges,
+    const int* __restrict__ g_rfirst, const int* __restrict__ g_roff,
+    const int* __restrict__ list_count, const unsigned short* __restrict__ list, int cap,
+    const float4* __restrict__ pos, const int* __restrict__ ptype, const int* __restrict__ pgid,
+    const float2* __restrict__ AB_sq, int T, LT6 c, LT6 r, float rc2_thr, float beta, int stage,
+    float gl, int Nx, int Ny, int Nz, float reach,
+    float4* __restrict__ facc, float2* __restrict__ eacc, float* __restrict__ vacc,
+    int solvent_start) {
+    lj_cells_body<ORTHO, NEED_E, NEED_V, FAST, SOLV>(blockIdx.x, start, nranges, max_ranges, g_rfirst, g_roff, list_count, list, cap, pos, ptype, pgid, AB_sq, T, c, r, rc2_thr, beta, stage, gl, Nx, Ny, Nz, reach, facc, eacc, vacc, solvent_start);
+}
+
 // Scatter of the per-row accumulators to the caller's arrays: frc +=,
 // atom_energy += E_LJ + E_C, LJ energy and direct-PME energy overwritten
 // (the reference clears them before its kernel, Lennard_Jones_force.cpp:
 // 343-349), atom_virial +=.
 template <bool NEED_E, bool NEED_V>
-__global__ void k_lj_finalize(int N, const int* __restrict__ sidx, const float4* __restrict__ facc,
+__device__ __forceinline__ void lj_finalize_body(int sa_block, int N, const int* __restrict__ sidx, const float4* __restrict__ facc,
                               const float2* __restrict__ eacc, const float* __restrict__ vacc,
                               float* __restrict__ frc, float* __restrict__ atom_energy,
                               float* __restrict__ lj_energy, float* __restrict__ direct_energy,
                               float* __restrict__ atom_virial, int assign,
                               int* __restrict__ reset_flag) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    int p = sa_block * blockDim.x + threadIdx.x;
     if (p == 0 && reset_flag) *reset_flag = 0;
     if (p >= N) return;
     const int i = sidx ? sidx[p] : p;
@@ -368,6 +382,16 @@ __global__ void k_lj_finalize(int N, const int* __restrict__ sidx, const float4*
         float* o = atom_virial + (size_t)6 * i;
         o[0] += v[0]; o[1] += v[1]; o[2] += v[2]; o[3] += v[3]; o[4] += v[4]; o[5] += v[5];
     }
+}
+
+template <bool NEED_E, bool NEED_V>
+__global__ void k_lj_finalize(int N, const int* __restrict__ sidx, const float4* __restrict__ facc,
+                              const float2* __restrict__ eacc, const float* __restrict__ vacc,
+                              float* __restrict__ frc, float* __restrict__ atom_energy,
+                              float* __restrict__ lj_energy, float* __restrict__ direct_energy,
+                              float* __restrict__ atom_virial, int assign,
+                              int* __restrict__ reset_flag) {
+    lj_finalize_body<NEED_E, NEED_V>(blockIdx.x, N, sidx, facc, eacc, vacc, frc, atom_energy, lj_energy, direct_energy, atom_virial, assign, reset_flag);
 }
 
 // Identity-order pack for the reference-format list entry point.
@@ -417,7 +441,8 @@ int launch_rows(sponge_lj* lj, sponge_nl* nl, LT6 c, LT6 r, float beta, int need
                             ? lj->cutoff + nl->skin * nl->opt.skin_permit
                             : -1.f;
 #define SA_LJ_ROWS(E, V, F, SV)                                                                 \
-    hipLaunchKernelGGL((k_lj_cells<ORTHO, E, V, F, SV>), dim3(nl->G), dim3(1024), sh, s,          \
+    SA_LAUNCH(1024, (&lj_cells_body<ORTHO, E, V, F, SV>), (k_lj_cells<ORTHO, E, V, F, SV>),      \
+              dim3(nl->G), dim3(1024), sh, s,                                                   \
                        nl->d_cell_start, nl->d_nranges, nl->max_ranges, nl->d_rfirst, nl->d_roff, \
                        nl->d_list_count, nl->d_list, nl->cap, lj->d_pos, lj->d_ptype, lj->d_pgid, \
                        lj->d_AB_sq, lj->T, c, r, lj->rc2_thr, beta, lj->stage, nl->grid_length,   \
@@ -463,7 +488,8 @@ int finalize(sponge_lj* lj, int N, const int* sidx, int need_e, int need_v, floa
              int* reset_flag = nullptr) {
     dim3 grid(grid_for(N, 256)), block(256);
 #define SA_FIN(E, V)                                                                            \
-    hipLaunchKernelGGL((k_lj_finalize<E, V>), grid, block, 0, s, N, sidx, lj->d_facc,           \
+    SA_LAUNCH(256, (&lj_finalize_body<E, V>), (k_lj_finalize<E, V>), grid, block, 0, s, N, sidx, \
+              lj->d_facc,                                                                       \
                        lj->d_eacc, lj->d_vacc, frc, atom_energy, lj->d_LJ_energy_atom, direct, \
                        atom_virial, assign, reset_flag)
     if (need_e && need_v) SA_FIN(true, true);

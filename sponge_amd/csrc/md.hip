@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "internal.h"
+#include "batch_launch.h"
 #include "timing.h"
 
 using namespace sa;
@@ -178,13 +179,13 @@ __device__ __forceinline__ void leap(float3& x, float3& v, float3 f, float im, f
     x = f3add(x, f3scale(dt, v));
 }
 
-__global__ void __launch_bounds__(256) k_integrate(int n_water, const int4* __restrict__ w,
+__device__ __forceinline__ void integrate_body(int sa_block, int n_water, const int4* __restrict__ w,
                                                    int n_free, const int* __restrict__ free_atoms,
                                                    float* __restrict__ crd, float* __restrict__ vel,
                                                    const float* __restrict__ frc,
                                                    const float* __restrict__ inv_mass, float dt,
                                                    LT6 c, LT6 r, Triangle g, float inv_dt) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = sa_block * blockDim.x + threadIdx.x;
     if (k < n_water) {
         const int4 t = w[k];
         float3 xa = ld3(crd, t.x), xb = ld3(crd, t.y), xc = ld3(crd, t.z);
@@ -203,6 +204,15 @@ __global__ void __launch_bounds__(256) k_integrate(int n_water, const int4* __re
         st3(vel, i, v);
         st3(crd, i, x);
     }
+}
+
+__global__ void __launch_bounds__(256) k_integrate(int n_water, const int4* __restrict__ w,
+                                                   int n_free, const int* __restrict__ free_atoms,
+                                                   float* __restrict__ crd, float* __restrict__ vel,
+                                                   const float* __restrict__ frc,
+                                                   const float* __restrict__ inv_mass, float dt,
+                                                   LT6 c, LT6 r, Triangle g, float inv_dt) {
+    integrate_body(blockIdx.x, n_water, w, n_free, free_atoms, crd, vel, frc, inv_mass, dt, c, r, g, inv_dt);
 }
 
 __global__ void k_add_vec(int n, const float* __restrict__ a, float* __restrict__ frc) {
@@ -544,7 +554,7 @@ int md_step(sponge_md* m, long long step, int need_energy, StepKind* kind_out = 
     const Triangle g{m->geom[0], m->geom[1], m->geom[2], m->geom[3], m->geom[4], m->mO, m->mH1, m->mH2};
     const int threads = m->n_water + m->n_free;
     if (threads > 0) {
-        hipLaunchKernelGGL(k_integrate, dim3(grid_for(threads, 256)), dim3(256), 0, s, m->n_water,
+        SA_LAUNCH(256, (&integrate_body), (k_integrate), dim3(grid_for(threads, 256)), dim3(256), 0, s, m->n_water,
                            m->d_water, m->n_free, m->d_free, m->d_crd, m->d_vel, m->d_frc,
                            m->d_inv_mass, m->dt, m->cell, m->rcell, g, 1.0f / m->dt);
         SA_LAUNCH_CHECK();
@@ -866,6 +876,12 @@ int md_state_arrays(sponge_md* m, float** crd, float** charge) {
 }
 
 }  // namespace sa
+
+extern "C" int sponge_md_request_list_update(sponge_md* m) {
+    if (!m) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    if (!m->nl->built) return 0;  // the first step builds it anyway
+    return fill_async(m->nl->d_need, 1, 1, m->stream);
+}
 
 extern "C" int sponge_md_set_reciprocal_overlap(sponge_md* m, int mode) {
     if (!m || mode < -1 || mode > 1) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;

@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "internal.h"
+#include "batch_launch.h"
 #include "timing.h"
 
 using namespace sa;
@@ -44,14 +45,20 @@ __global__ void k_nl_set(int* need, int v) { *need = v; }
 
 // Check_Refresh (neighbor_list.cpp:307-321)
 template <bool ORTHO>
-__global__ void k_nl_check(int N, const float* __restrict__ crd, const float* __restrict__ old,
+__device__ __forceinline__ void nl_check_body(int sa_block, int N, const float* __restrict__ crd, const float* __restrict__ old,
                            LT6 c, LT6 r, float permit2, int* need) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = sa_block * blockDim.x + threadIdx.x;
     if (i >= N) return;
     float3 a = make_float3(crd[3 * i], crd[3 * i + 1], crd[3 * i + 2]);
     float3 b = make_float3(old[3 * i], old[3 * i + 1], old[3 * i + 2]);
     float3 d = disp_pbc<ORTHO>(a, b, c, r);
     if (norm2_exact(d) > permit2) *need = 1;
+}
+
+template <bool ORTHO>
+__global__ void k_nl_check(int N, const float* __restrict__ crd, const float* __restrict__ old,
+                           LT6 c, LT6 r, float permit2, int* need) {
+    nl_check_body<ORTHO>(blockIdx.x, N, crd, old, c, r, permit2, need);
 }
 
 // Cell of an atom as Put_Atom_In_Grids wraps it (neighbor_list.cpp:357-377).
@@ -72,12 +79,12 @@ __device__ __forceinline__ int cell_of(float x, float y, float z, const LT6& c, 
     return nx * Ny * Nz + ny * Nz + nz;
 }
 
-__global__ void k_nl_bin(const int* __restrict__ need, int N, const float* __restrict__ crd,
+__device__ __forceinline__ void nl_bin_body(int sa_block, const int* __restrict__ need, int N, const float* __restrict__ crd,
                          float* __restrict__ old, int need_copy, LT6 c, float gl, int Nx, int Ny,
                          int Nz, int* __restrict__ count, int* __restrict__ atom_cell,
                          int* __restrict__ atom_slot) {
     if (*need == 0) return;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = sa_block * blockDim.x + threadIdx.x;
     if (i >= N) return;
     float x = crd[3 * i], y = crd[3 * i + 1], z = crd[3 * i + 2];
     if (need_copy) {
@@ -90,10 +97,17 @@ __global__ void k_nl_bin(const int* __restrict__ need, int N, const float* __res
     atom_slot[i] = atomicAdd(&count[cell], 1);
 }
 
+__global__ void k_nl_bin(const int* __restrict__ need, int N, const float* __restrict__ crd,
+                         float* __restrict__ old, int need_copy, LT6 c, float gl, int Nx, int Ny,
+                         int Nz, int* __restrict__ count, int* __restrict__ atom_cell,
+                         int* __restrict__ atom_slot) {
+    nl_bin_body(blockIdx.x, need, N, crd, old, need_copy, c, gl, Nx, Ny, Nz, count, atom_cell, atom_slot);
+}
+
 // One-block exclusive scan of the cell counts -> cell_start[G+1]; flags cells
 // above max_in_grid (the reference's grid overflow, :382-386) without
 // dropping atoms.
-__global__ void __launch_bounds__(1024) k_nl_scan(const int* __restrict__ need, int G,
+__device__ __forceinline__ void nl_scan_body(int sa_block, const int* __restrict__ need, int G,
                                                   int* __restrict__ count, int* __restrict__ start,
                                                   int max_in_grid, int* __restrict__ flags,
                                                   unsigned long long* __restrict__ rebuilds) {
@@ -130,14 +144,21 @@ __global__ void __launch_bounds__(1024) k_nl_scan(const int* __restrict__ need, 
     }
 }
 
-__global__ void k_nl_scatter(const int* __restrict__ need, int N, const float* __restrict__ crd,
+__global__ void __launch_bounds__(1024) k_nl_scan(const int* __restrict__ need, int G,
+                                                  int* __restrict__ count, int* __restrict__ start,
+                                                  int max_in_grid, int* __restrict__ flags,
+                                                  unsigned long long* __restrict__ rebuilds) {
+    nl_scan_body(blockIdx.x, need, G, count, start, max_in_grid, flags, rebuilds);
+}
+
+__device__ __forceinline__ void nl_scatter_body(int sa_block, const int* __restrict__ need, int N, const float* __restrict__ crd,
                              const int* __restrict__ atom_local,
                              const int* __restrict__ atom_cell, const int* __restrict__ atom_slot,
                              const int* __restrict__ start, const int2* __restrict__ gexw,
                              int* __restrict__ sidx, int* __restrict__ rank,
                              float4* __restrict__ spos, int2* __restrict__ sexw) {
     if (*need == 0) return;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = sa_block * blockDim.x + threadIdx.x;
     if (i >= N) return;
     int p = start[atom_cell[i]] + atom_slot[i];
     int gi = atom_local ? atom_local[i] : i;
@@ -145,6 +166,15 @@ __global__ void k_nl_scatter(const int* __restrict__ need, int N, const float* _
     rank[i] = p;
     spos[p] = make_float4(crd[3 * i], crd[3 * i + 1], crd[3 * i + 2], __int_as_float(gi));
     sexw[p] = gexw ? gexw[gi] : make_int2(1, 0);
+}
+
+__global__ void k_nl_scatter(const int* __restrict__ need, int N, const float* __restrict__ crd,
+                             const int* __restrict__ atom_local,
+                             const int* __restrict__ atom_cell, const int* __restrict__ atom_slot,
+                             const int* __restrict__ start, const int2* __restrict__ gexw,
+                             int* __restrict__ sidx, int* __restrict__ rank,
+                             float4* __restrict__ spos, int2* __restrict__ sexw) {
+    nl_scatter_body(blockIdx.x, need, N, crd, atom_local, atom_cell, atom_slot, start, gexw, sidx, rank, spos, sexw);
 }
 
 // Combined exclusion window of every atom: the smallest and largest id among
@@ -225,7 +255,7 @@ constexpr int kSearchStage = 3584;  // covers the stencil streams of water at 12
 constexpr int kSearchRows = 4;
 
 template <bool ORTHO>
-__global__ void __launch_bounds__(512) k_nl_search_cells(
+__device__ __forceinline__ void nl_search_cells_body(int sa_block, 
     const int* __restrict__ need, const float4* __restrict__ spos, const int2* __restrict__ sexw,
     const int* __restrict__ start, const int2* __restrict__ ranges,
     const int* __restrict__ nranges, int max_ranges, LT6 c, LT6 r, float thr,
@@ -237,7 +267,7 @@ __global__ void __launch_bounds__(512) k_nl_search_cells(
     extern __shared__ float4 cand[];
     __shared__ int r_first[kMaxRanges];
     __shared__ int r_off[kMaxRanges + 1];
-    const int cell = blockIdx.x;
+    const int cell = sa_block;
     const int nr = cell_stream(cell, start, ranges, nranges, max_ranges, r_first, r_off);
     const int total = r_off[nr];
     // the stream tables are kept for the pair and export kernels
@@ -324,6 +354,18 @@ __global__ void __launch_bounds__(512) k_nl_search_cells(
             }
         }
     }
+}
+
+template <bool ORTHO>
+__global__ void __launch_bounds__(512) k_nl_search_cells(
+    const int* __restrict__ need, const float4* __restrict__ spos, const int2* __restrict__ sexw,
+    const int* __restrict__ start, const int2* __restrict__ ranges,
+    const int* __restrict__ nranges, int max_ranges, LT6 c, LT6 r, float thr,
+    const int* __restrict__ ex_start, const int* __restrict__ ex_list,
+    const int* __restrict__ ex_num, int cap, int stage, int* __restrict__ list_count,
+    unsigned short* __restrict__ list, int* __restrict__ g_rfirst, int* __restrict__ g_roff,
+    int* __restrict__ flags) {
+    nl_search_cells_body<ORTHO>(blockIdx.x, need, spos, sexw, start, ranges, nranges, max_ranges, c, r, thr, ex_start, ex_list, ex_num, cap, stage, list_count, list, g_rfirst, g_roff, flags);
 }
 
 
@@ -605,21 +647,19 @@ int sa::nl_update_impl(sponge_nl* nl, const int* atom_local, int local_atom_numb
         if (!need_cleared) hipLaunchKernelGGL(k_nl_set, dim3(1), dim3(1), 0, s, nl->d_need, 0);
         const float permit2 = nl->skin * nl->skin * nl->opt.skin_permit * nl->opt.skin_permit;
         if (nl->ortho)
-            hipLaunchKernelGGL(k_nl_check<true>, dim3(grid_for(N, 256)), dim3(256), 0, s, N, crd,
-                               nl->d_old_crd, c, r, permit2, nl->d_need);
+            SA_LAUNCH(256, (&nl_check_body<true>), (k_nl_check<true>), dim3(grid_for(N, 256)), dim3(256), 0, s, N, crd, nl->d_old_crd, c, r, permit2, nl->d_need);
         else
-            hipLaunchKernelGGL(k_nl_check<false>, dim3(grid_for(N, 256)), dim3(256), 0, s, N, crd,
-                               nl->d_old_crd, c, r, permit2, nl->d_need);
+            SA_LAUNCH(256, (&nl_check_body<false>), (k_nl_check<false>), dim3(grid_for(N, 256)), dim3(256), 0, s, N, crd, nl->d_old_crd, c, r, permit2, nl->d_need);
     } else {
         hipLaunchKernelGGL(k_nl_set, dim3(1), dim3(1), 0, s, nl->d_need,
                            ((step + 1) % nl->opt.refresh_interval == 0) ? 1 : 0);
     }
     SA_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_nl_bin, dim3(grid_for(N, 256)), dim3(256), 0, s, nl->d_need, N, crd,
+    SA_LAUNCH(256, (&nl_bin_body), (k_nl_bin), dim3(grid_for(N, 256)), dim3(256), 0, s, nl->d_need, N, crd,
                        nl->d_old_crd, auto_refresh ? 1 : 0, c, nl->grid_length, nl->Nx, nl->Ny,
                        nl->Nz, nl->d_cell_count, nl->d_atom_cell, nl->d_atom_slot);
     SA_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_nl_scan, dim3(1), dim3(1024), 0, s, nl->d_need, nl->G, nl->d_cell_count,
+    SA_LAUNCH(1024, (&nl_scan_body), (k_nl_scan), dim3(1), dim3(1024), 0, s, nl->d_need, nl->G, nl->d_cell_count,
                        nl->d_cell_start, nl->opt.max_atom_in_grid_numbers, nl->d_flags, nl->d_rebuilds);
     SA_LAUNCH_CHECK();
     // combined exclusion windows: the exclusion lists are topology, so the
@@ -636,7 +676,7 @@ int sa::nl_update_impl(sponge_nl* nl, const int* atom_local, int local_atom_numb
         nl->exw_src[1] = excluded_list;
         nl->exw_src[2] = excluded_numbers;
     }
-    hipLaunchKernelGGL(k_nl_scatter, dim3(grid_for(N, 256)), dim3(256), 0, s, nl->d_need, N, crd,
+    SA_LAUNCH(256, (&nl_scatter_body), (k_nl_scatter), dim3(grid_for(N, 256)), dim3(256), 0, s, nl->d_need, N, crd,
                        atom_local, nl->d_atom_cell, nl->d_atom_slot, nl->d_cell_start,
                        have_ex ? nl->d_gexw : nullptr, nl->d_sidx, nl->d_rank, nl->d_spos,
                        nl->d_sexw);
@@ -645,13 +685,13 @@ int sa::nl_update_impl(sponge_nl* nl, const int* atom_local, int local_atom_numb
         ScopedTimer timer("nl_search", s);
         const size_t sh = sizeof(float4) * nl->stage;
         if (nl->ortho)
-            hipLaunchKernelGGL(k_nl_search_cells<true>, dim3(nl->G), dim3(512), sh, s, nl->d_need,
+            SA_LAUNCH(512, (&nl_search_cells_body<true>), (k_nl_search_cells<true>), dim3(nl->G), dim3(512), sh, s, nl->d_need,
                                nl->d_spos, nl->d_sexw, nl->d_cell_start, nl->d_ranges,
                                nl->d_nranges, nl->max_ranges, c, r, nl->thr, excluded_list_start,
                                excluded_list, excluded_numbers, nl->cap, nl->stage, nl->d_list_count,
                                nl->d_list, nl->d_rfirst, nl->d_roff, nl->d_flags);
         else
-            hipLaunchKernelGGL(k_nl_search_cells<false>, dim3(nl->G), dim3(512), sh, s, nl->d_need,
+            SA_LAUNCH(512, (&nl_search_cells_body<false>), (k_nl_search_cells<false>), dim3(nl->G), dim3(512), sh, s, nl->d_need,
                                nl->d_spos, nl->d_sexw, nl->d_cell_start, nl->d_ranges,
                                nl->d_nranges, nl->max_ranges, c, r, nl->thr, excluded_list_start,
                                excluded_list, excluded_numbers, nl->cap, nl->stage, nl->d_list_count,
@@ -789,11 +829,11 @@ extern "C" int sponge_nl_clear(sponge_nl* nl) {
 // Gather of the current positions, charges and LJ types into the sorted
 // order of the last build (the role of Copy_Crd_And_Charge_To_New_Crd,
 // Lennard_Jones_force.cpp:17-27), clearing the sorted force accumulator.
-__global__ void k_nl_pack(int N, const int* __restrict__ sidx, const float4* __restrict__ spos,
+__device__ __forceinline__ void nl_pack_body(int sa_block, int N, const int* __restrict__ sidx, const float4* __restrict__ spos,
                           const float* __restrict__ crd, const float* __restrict__ charge,
                           const int* __restrict__ type, float4* __restrict__ pos,
                           int* __restrict__ ptype, int* __restrict__ pgid) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    int p = sa_block * blockDim.x + threadIdx.x;
     if (p >= N) return;
     int i = sidx[p];
     pos[p] = make_float4(crd[3 * i], crd[3 * i + 1], crd[3 * i + 2], charge[i]);
@@ -801,9 +841,16 @@ __global__ void k_nl_pack(int N, const int* __restrict__ sidx, const float4* __r
     pgid[p] = __float_as_int(spos[p].w);
 }
 
+__global__ void k_nl_pack(int N, const int* __restrict__ sidx, const float4* __restrict__ spos,
+                          const float* __restrict__ crd, const float* __restrict__ charge,
+                          const int* __restrict__ type, float4* __restrict__ pos,
+                          int* __restrict__ ptype, int* __restrict__ pgid) {
+    nl_pack_body(blockIdx.x, N, sidx, spos, crd, charge, type, pos, ptype, pgid);
+}
+
 int sa::nl_pack_for_pairs(sponge_nl* nl, const float* crd, const float* charge, const int* type,
                           float4* pos, int* ptype, int* pgid, hipStream_t s) {
-    hipLaunchKernelGGL(k_nl_pack, dim3(grid_for(nl->N, 256)), dim3(256), 0, s, nl->N, nl->d_sidx,
+    SA_LAUNCH(256, (&nl_pack_body), (k_nl_pack), dim3(grid_for(nl->N, 256)), dim3(256), 0, s, nl->N, nl->d_sidx,
                        nl->d_spos, crd, charge, type, pos, ptype, pgid);
     SA_LAUNCH_CHECK();
     return 0;

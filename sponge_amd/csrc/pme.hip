@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "internal.h"
+#include "batch_launch.h"
 #include "timing.h"
 
 using namespace sa;
@@ -189,13 +190,14 @@ __device__ __forceinline__ float2* fft_lines(float2* a, float2* b, int nl, int n
 constexpr int kMaxFftLen = 512;  // largest grid dimension (checked at init)
 
 // z pass, real -> half spectrum: block = (y block, x)
-__global__ void __launch_bounds__(256) k_fft_z_r2c(const float* __restrict__ Q,
+__device__ __forceinline__ void fft_z_r2c_body(int sa_block, const float* __restrict__ Q,
                                                    float2* __restrict__ FQ, int fx, int fy, int fz,
                                                    int fzc, int TL, sa_radix_plan pl,
                                                    const float2* __restrict__ tw) {
     extern __shared__ float2 sm[];
     __shared__ float2 stw[kMaxFftLen];
-    const int x = blockIdx.y, y0 = blockIdx.x * TL;
+    const int gy = (fy + TL - 1) / TL;  // y blocks per x line (1D grid: x-major)
+    const int x = sa_block / gy, y0 = (sa_block - x * gy) * TL;
     const int nl = min(TL, fy - y0), npair = (nl + 1) >> 1;
     float2* a = sm;
     float2* b = sm + npair * fz;
@@ -221,13 +223,20 @@ __global__ void __launch_bounds__(256) k_fft_z_r2c(const float* __restrict__ Q,
     }
 }
 
+__global__ void __launch_bounds__(256) k_fft_z_r2c(const float* __restrict__ Q,
+                                                   float2* __restrict__ FQ, int fx, int fy, int fz,
+                                                   int fzc, int TL, sa_radix_plan pl,
+                                                   const float2* __restrict__ tw) {
+    fft_z_r2c_body(blockIdx.x, Q, FQ, fx, fy, fz, fzc, TL, pl, tw);
+}
+
 // y pass over contiguous lines [kz][x][0..fy)
 template <int SIGN>
-__global__ void __launch_bounds__(256) k_fft_y(float2* __restrict__ FQ, int nlines, int fy, int TL,
+__device__ __forceinline__ void fft_y_body(int sa_block, float2* __restrict__ FQ, int nlines, int fy, int TL,
                                                sa_radix_plan pl, const float2* __restrict__ tw) {
     extern __shared__ float2 sm[];
     __shared__ float2 stw[kMaxFftLen];
-    const int l0 = blockIdx.x * TL;
+    const int l0 = sa_block * TL;
     const int nl = min(TL, nlines - l0);
     float2* a = sm;
     float2* b = sm + TL * fy;
@@ -239,6 +248,12 @@ __global__ void __launch_bounds__(256) k_fft_y(float2* __restrict__ FQ, int nlin
     for (int e = threadIdx.x; e < nl * fy; e += blockDim.x) base[e] = res[e];
 }
 
+template <int SIGN>
+__global__ void __launch_bounds__(256) k_fft_y(float2* __restrict__ FQ, int nlines, int fy, int TL,
+                                               sa_radix_plan pl, const float2* __restrict__ tw) {
+    fft_y_body<SIGN>(blockIdx.x, FQ, nlines, fy, TL, pl, tw);
+}
+
 // x pass with the reciprocal-space convolution: block = (y block, kz).
 // acc[0] += sum w(kz) BC |F|^2 (w = 1 on kz = 0 and kz = fz/2 for even fz,
 // 2 otherwise: the full-spectrum sum of the reference's real-space
@@ -248,14 +263,15 @@ __global__ void __launch_bounds__(256) k_fft_y(float2* __restrict__ FQ, int nlin
 // ACC: accumulate the energy (and, with need_virial, the virial) sums; off on
 // force-only evaluations.
 template <bool ACC>
-__global__ void __launch_bounds__(256) k_fft_x_conv(
+__device__ __forceinline__ void fft_x_conv_body(int sa_block, 
     float2* __restrict__ FQ, const float* __restrict__ BCt, const float* __restrict__ vBCt,
     long long nfft, int fx, int fy, int fz, int fzc, int TL, sa_radix_plan pl,
     const float2* __restrict__ tw, int need_virial, double* __restrict__ acc) {
     extern __shared__ float2 sm[];
     __shared__ double red[7][4];
     __shared__ float2 stw[kMaxFftLen];
-    const int kz = blockIdx.y, y0 = blockIdx.x * TL;
+    const int gy = (fy + TL - 1) / TL;  // y blocks per kz plane (1D grid: kz-major)
+    const int kz = sa_block / gy, y0 = (sa_block - kz * gy) * TL;
     const int nl = min(TL, fy - y0);
     float2* a = sm;
     float2* b = sm + TL * fx;
@@ -318,14 +334,23 @@ __global__ void __launch_bounds__(256) k_fft_x_conv(
     }
 }
 
+template <bool ACC>
+__global__ void __launch_bounds__(256) k_fft_x_conv(
+    float2* __restrict__ FQ, const float* __restrict__ BCt, const float* __restrict__ vBCt,
+    long long nfft, int fx, int fy, int fz, int fzc, int TL, sa_radix_plan pl,
+    const float2* __restrict__ tw, int need_virial, double* __restrict__ acc) {
+    fft_x_conv_body<ACC>(blockIdx.x, FQ, BCt, vBCt, nfft, fx, fy, fz, fzc, TL, pl, tw, need_virial, acc);
+}
+
 // z pass, half spectrum -> real (unnormalised C2R): block = (y block, x)
-__global__ void __launch_bounds__(256) k_fft_z_c2r(const float2* __restrict__ FQ,
+__device__ __forceinline__ void fft_z_c2r_body(int sa_block, const float2* __restrict__ FQ,
                                                    float* __restrict__ out, int fx, int fy, int fz,
                                                    int fzc, int TL, sa_radix_plan pl,
                                                    const float2* __restrict__ tw) {
     extern __shared__ float2 sm[];
     __shared__ float2 stw[kMaxFftLen];
-    const int x = blockIdx.y, y0 = blockIdx.x * TL;
+    const int gy = (fy + TL - 1) / TL;  // y blocks per x line (1D grid: x-major)
+    const int x = sa_block / gy, y0 = (sa_block - x * gy) * TL;
     const int nl = min(TL, fy - y0), npair = (nl + 1) >> 1;
     for (int k = threadIdx.x; k < fz; k += blockDim.x) stw[k] = tw[k];
     float2* h = sm;                       // [nl][fzc]
@@ -365,6 +390,13 @@ __global__ void __launch_bounds__(256) k_fft_z_c2r(const float2* __restrict__ FQ
         const float2 v = res[(l >> 1) * fz + z];
         dst[e] = (l & 1) ? v.y : v.x;
     }
+}
+
+__global__ void __launch_bounds__(256) k_fft_z_c2r(const float2* __restrict__ FQ,
+                                                   float* __restrict__ out, int fx, int fy, int fz,
+                                                   int fzc, int TL, sa_radix_plan pl,
+                                                   const float2* __restrict__ tw) {
+    fft_z_c2r_body(blockIdx.x, FQ, out, fx, fy, fz, fzc, TL, pl, tw);
 }
 
 // ---------------------------------------------------------------------------
@@ -443,7 +475,7 @@ __device__ __forceinline__ int brick_of(int x, int B, int nbk) {
 // each wavefront.
 constexpr int kBinLds = 4096;  // bricks tracked in LDS by the bin kernel
 
-__global__ void __launch_bounds__(256) k_pme_bin(int N, const float* __restrict__ crd,
+__device__ __forceinline__ void pme_bin_body(int sa_block, int N, const float* __restrict__ crd,
                                                  const float* __restrict__ charge, LT6 rc, int fx,
                                                  int fy, int fz, int bx, int by, int bz, int nbx,
                                                  int nby, int nbz, int nb, int bcap,
@@ -453,8 +485,8 @@ __global__ void __launch_bounds__(256) k_pme_bin(int N, const float* __restrict_
                                                  int* __restrict__ oi, double* __restrict__ acc) {
     __shared__ int lcount[kBinLds];
     __shared__ int lbase[kBinLds];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < 7 && blockIdx.x == 0) acc[i] = 0.0;
+    const int i = sa_block * blockDim.x + threadIdx.x;
+    if (i < 7 && sa_block == 0) acc[i] = 0.0;
     const bool live = i < N;
     int h = -1, u = 0;
     float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -499,6 +531,17 @@ __global__ void __launch_bounds__(256) k_pme_bin(int N, const float* __restrict_
         ou[o] = u;
         oi[o] = i;
     }
+}
+
+__global__ void __launch_bounds__(256) k_pme_bin(int N, const float* __restrict__ crd,
+                                                 const float* __restrict__ charge, LT6 rc, int fx,
+                                                 int fy, int fz, int bx, int by, int bz, int nbx,
+                                                 int nby, int nbz, int nb, int bcap,
+                                                 int* __restrict__ bcount, float4* __restrict__ bw,
+                                                 int* __restrict__ bu, int* __restrict__ bi,
+                                                 float4* __restrict__ ow, int* __restrict__ ou,
+                                                 int* __restrict__ oi, double* __restrict__ acc) {
+    pme_bin_body(blockIdx.x, N, crd, charge, rc, fx, fy, fz, bx, by, bz, nbx, nby, nbz, nb, bcap, bcount, bw, bu, bi, ow, ou, oi, acc);
 }
 
 // Brick of a workgroup of the spread, tile-reduce and gather launches (one
@@ -554,12 +597,12 @@ inline TileGeom tile_geom(int mx, int my, int mz) {
 // wavefront complete in order, so the plain read-add-write needs no atomic (an
 // LDS float atomic costs about three cycles per lane on CDNA4); the copies are
 // summed when the tile is stored.
-__global__ void __launch_bounds__(256) k_pme_spread_tiles(
+__device__ __forceinline__ void pme_spread_tiles_body(int sa_block, 
     int bx, int by, int bz, int nbx, int nby, int nbz, int nb, int bcap, TileGeom tg,
     const int* __restrict__ bcount, const float4* __restrict__ bw, const int* __restrict__ bu,
     const float4* __restrict__ ow, const int* __restrict__ ou, float* __restrict__ tiles) {
     extern __shared__ float acc[];
-    const int b = brick_for_block(blockIdx.x, gridDim.x);
+    const int b = brick_for_block(sa_block, nb);
     const int cz = b % nbz, cy = (b / nbz) % nby, cx = b / (nbz * nby);
     const int X0 = cx * bx, Y0 = cy * by, Z0 = cz * bz;
     const int nwave = blockDim.x / kWave;
@@ -642,19 +685,26 @@ __global__ void __launch_bounds__(256) k_pme_spread_tiles(
     }
 }
 
+__global__ void __launch_bounds__(256) k_pme_spread_tiles(
+    int bx, int by, int bz, int nbx, int nby, int nbz, int nb, int bcap, TileGeom tg,
+    const int* __restrict__ bcount, const float4* __restrict__ bw, const int* __restrict__ bu,
+    const float4* __restrict__ ow, const int* __restrict__ ou, float* __restrict__ tiles) {
+    pme_spread_tiles_body(blockIdx.x, bx, by, bz, nbx, nby, nbz, nb, bcap, tg, bcount, bw, bu, ow, ou, tiles);
+}
+
 // Grid point (x, y, z) of brick c at local (lx, ly, lz) sums, per axis, its
 // own tile at l + 3 and, for the top 3 points of the brick, the tile of the
 // brick above at l - extent + 3 (the same tile when the axis has one brick).
 // It also hands the bin counts to the gather (gcount) and clears them for the
 // next evaluation's binning (the spread, their other reader, has finished).
-__global__ void __launch_bounds__(256) k_pme_tiles_reduce(int fx, int fy, int fz, int bx, int by,
+__device__ __forceinline__ void pme_tiles_reduce_body(int sa_block, int fx, int fy, int fz, int bx, int by,
                                                           int bz, int nbx, int nby, int nbz,
                                                           int nb, TileGeom tg,
                                                           const float* __restrict__ tiles,
                                                           float* __restrict__ Q,
                                                           int* __restrict__ bcount,
                                                           int* __restrict__ gcount) {
-    const int b = brick_for_block(blockIdx.x, gridDim.x);
+    const int b = brick_for_block(sa_block, nb);
     if (threadIdx.x == 0) {
         gcount[b] = bcount[b];
         bcount[b] = 0;
@@ -705,6 +755,16 @@ __global__ void __launch_bounds__(256) k_pme_tiles_reduce(int fx, int fy, int fz
     }
 }
 
+__global__ void __launch_bounds__(256) k_pme_tiles_reduce(int fx, int fy, int fz, int bx, int by,
+                                                          int bz, int nbx, int nby, int nbz,
+                                                          int nb, TileGeom tg,
+                                                          const float* __restrict__ tiles,
+                                                          float* __restrict__ Q,
+                                                          int* __restrict__ bcount,
+                                                          int* __restrict__ gcount) {
+    pme_tiles_reduce_body(blockIdx.x, fx, fy, fz, bx, by, bz, nbx, nby, nbz, nb, tg, tiles, Q, bcount, gcount);
+}
+
 // B-spline weight polynomials M_k and their derivatives dM_k as compile-time
 // constants (M_k the same as cMa..cMd) for the thread-per-atom gather
 struct BspK {
@@ -733,14 +793,14 @@ struct BspK {
 constexpr int kGatherThreads = 256;
 constexpr int kGatherLoads = 8;
 
-__global__ void __launch_bounds__(kGatherThreads) k_pme_gather_tiles(
+__device__ __forceinline__ void pme_gather_tiles_body(int sa_block, 
     int fx, int fy, int fz, int bx, int by, int bz, int nbx, int nby, int nbz, int nb, int bcap,
     TileGeom tg, const int* __restrict__ gcount, const float4* __restrict__ bw,
     const int* __restrict__ bu, const int* __restrict__ bi, const float4* __restrict__ ow,
     const int* __restrict__ ou, const int* __restrict__ oi, const float* __restrict__ phi, LT6 rc,
     float update_interval, float* __restrict__ frc, const float* __restrict__ extra) {
     extern __shared__ float tile[];
-    const int b = brick_for_block(blockIdx.x, gridDim.x);
+    const int b = brick_for_block(sa_block, nb);
     const int cz = b % nbz, cy = (b / nbz) % nby, cx = b / (nbz * nby);
     const int X0 = cx * bx, Y0 = cy * by, Z0 = cz * bz;
     const int TX = (cx == nbx - 1 ? fx - X0 : bx) + 3;
@@ -840,6 +900,15 @@ __global__ void __launch_bounds__(kGatherThreads) k_pme_gather_tiles(
     }
 }
 
+__global__ void __launch_bounds__(kGatherThreads) k_pme_gather_tiles(
+    int fx, int fy, int fz, int bx, int by, int bz, int nbx, int nby, int nbz, int nb, int bcap,
+    TileGeom tg, const int* __restrict__ gcount, const float4* __restrict__ bw,
+    const int* __restrict__ bu, const int* __restrict__ bi, const float4* __restrict__ ow,
+    const int* __restrict__ ou, const int* __restrict__ oi, const float* __restrict__ phi, LT6 rc,
+    float update_interval, float* __restrict__ frc, const float* __restrict__ extra) {
+    pme_gather_tiles_body(blockIdx.x, fx, fy, fz, bx, by, bz, nbx, nby, nbz, nb, bcap, tg, gcount, bw, bu, bi, ow, ou, oi, phi, rc, update_interval, frc, extra);
+}
+
 __global__ void k_pme_charge_sums(int N, const float* __restrict__ q, double* __restrict__ acc) {
     double s2 = 0, s1 = 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
@@ -899,14 +968,14 @@ __device__ __forceinline__ float excl_bracket(float x) {
 // MPI_PME_Excluded_Force_With_Atom_Energy_Correction with factor 1
 // (PM_force.cpp:1744-1817), one thread per local atom.
 template <bool ORTHO>
-__global__ void k_pme_excluded(int N, const float* __restrict__ crd,
+__device__ __forceinline__ void pme_excluded_body(int sa_block, int N, const float* __restrict__ crd,
                                const float* __restrict__ charge, LT6 c, LT6 r, float beta,
                                const int* __restrict__ ex_start, const int* __restrict__ ex_list,
                                const int* __restrict__ ex_num, float* __restrict__ frc,
                                int need_e, float* __restrict__ atom_ene,
                                float* __restrict__ corr_ene, int need_v,
                                float* __restrict__ atom_virial) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = sa_block * blockDim.x + threadIdx.x;
     if (i >= N) return;
     const int n = ex_num[i];
     if (n <= 0) return;
@@ -961,6 +1030,17 @@ __global__ void k_pme_excluded(int N, const float* __restrict__ crd,
         unsafeAtomicAdd(&o[4], v4);
         unsafeAtomicAdd(&o[5], v5);
     }
+}
+
+template <bool ORTHO>
+__global__ void k_pme_excluded(int N, const float* __restrict__ crd,
+                               const float* __restrict__ charge, LT6 c, LT6 r, float beta,
+                               const int* __restrict__ ex_start, const int* __restrict__ ex_list,
+                               const int* __restrict__ ex_num, float* __restrict__ frc,
+                               int need_e, float* __restrict__ atom_ene,
+                               float* __restrict__ corr_ene, int need_v,
+                               float* __restrict__ atom_virial) {
+    pme_excluded_body<ORTHO>(blockIdx.x, N, crd, charge, c, r, beta, ex_start, ex_list, ex_num, frc, need_e, atom_ene, corr_ene, need_v, atom_virial);
 }
 
 __global__ void k_sum2_f32(int n, const float* __restrict__ a, const float* __restrict__ b,
@@ -1328,42 +1408,47 @@ static int pme_fft_convolve(sponge_pme* p, int need_virial, int need_energy) {
         const int TL = p->tl_z;
         const size_t sh = (size_t)2 * ((TL + 1) / 2) * fz * sizeof(float2);
         ScopedTimer timer("pme_fft_z_r2c", s);
-        hipLaunchKernelGGL(k_fft_z_r2c, dim3(grid_for(fy, TL), fx), dim3(256), sh, s, p->d_Q,
-                           p->d_FQ, fx, fy, fz, fzc, TL, p->plan[2], p->d_tw[2]);
+        SA_LAUNCH(256, (&fft_z_r2c_body), (k_fft_z_r2c), dim3(grid_for(fy, TL) * fx), dim3(256),
+                  sh, s, p->d_Q, p->d_FQ, fx, fy, fz, fzc, TL, p->plan[2], p->d_tw[2]);
         SA_LAUNCH_CHECK();
     }
     {
         const int TL = p->tl_y, nlines = fzc * fx;
         const size_t sh = (size_t)2 * TL * fy * sizeof(float2);
         ScopedTimer timer("pme_fft_y", s);
-        hipLaunchKernelGGL(k_fft_y<-1>, dim3(grid_for(nlines, TL)), dim3(256), sh, s, p->d_FQ,
-                           nlines, fy, TL, p->plan[1], p->d_tw[1]);
+        SA_LAUNCH(256, (&fft_y_body<-1>), (k_fft_y<-1>), dim3(grid_for(nlines, TL)), dim3(256), sh,
+                  s, p->d_FQ, nlines, fy, TL, p->plan[1], p->d_tw[1]);
         SA_LAUNCH_CHECK();
     }
     {
         const int TL = p->tl_x;
         const size_t sh = (size_t)2 * TL * fx * sizeof(float2);
         ScopedTimer timer("pme_fft_x_conv", s);
-        hipLaunchKernelGGL((need_energy || need_virial) ? k_fft_x_conv<true> : k_fft_x_conv<false>,
-                           dim3(grid_for(fy, TL), fzc), dim3(256), sh, s, p->d_FQ, p->d_BCt,
-                           p->d_vBCt, p->nfft, fx, fy, fz, fzc, TL, p->plan[0], p->d_tw[0],
-                           need_virial, p->d_acc);
+        const dim3 grid(grid_for(fy, TL) * fzc);
+        if (need_energy || need_virial)
+            SA_LAUNCH(256, (&fft_x_conv_body<true>), (k_fft_x_conv<true>), grid, dim3(256), sh, s,
+                      p->d_FQ, p->d_BCt, p->d_vBCt, p->nfft, fx, fy, fz, fzc, TL, p->plan[0],
+                      p->d_tw[0], need_virial, p->d_acc);
+        else
+            SA_LAUNCH(256, (&fft_x_conv_body<false>), (k_fft_x_conv<false>), grid, dim3(256), sh, s,
+                      p->d_FQ, p->d_BCt, p->d_vBCt, p->nfft, fx, fy, fz, fzc, TL, p->plan[0],
+                      p->d_tw[0], need_virial, p->d_acc);
         SA_LAUNCH_CHECK();
     }
     {
         const int TL = p->tl_y, nlines = fzc * fx;
         const size_t sh = (size_t)2 * TL * fy * sizeof(float2);
         ScopedTimer timer("pme_ifft_y", s);
-        hipLaunchKernelGGL(k_fft_y<+1>, dim3(grid_for(nlines, TL)), dim3(256), sh, s, p->d_FQ,
-                           nlines, fy, TL, p->plan[1], p->d_tw[1]);
+        SA_LAUNCH(256, (&fft_y_body<+1>), (k_fft_y<+1>), dim3(grid_for(nlines, TL)), dim3(256), sh,
+                  s, p->d_FQ, nlines, fy, TL, p->plan[1], p->d_tw[1]);
         SA_LAUNCH_CHECK();
     }
     {
         const int TL = p->tl_z;
         const size_t sh = ((size_t)TL * fzc + (size_t)2 * ((TL + 1) / 2) * fz) * sizeof(float2);
         ScopedTimer timer("pme_fft_z_c2r", s);
-        hipLaunchKernelGGL(k_fft_z_c2r, dim3(grid_for(fy, TL), fx), dim3(256), sh, s, p->d_FQ,
-                           p->d_FBCFQ, fx, fy, fz, fzc, TL, p->plan[2], p->d_tw[2]);
+        SA_LAUNCH(256, (&fft_z_c2r_body), (k_fft_z_c2r), dim3(grid_for(fy, TL) * fx), dim3(256),
+                  sh, s, p->d_FQ, p->d_FBCFQ, fx, fy, fz, fzc, TL, p->plan[2], p->d_tw[2]);
         SA_LAUNCH_CHECK();
     }
     return 0;
@@ -1373,6 +1458,12 @@ static int pme_fft_convolve(sponge_pme* p, int need_virial, int need_energy) {
 // spread, FFT convolution: it reads only the coordinates and charges) on a
 // stream of its own beside other work and join before the second (gather,
 // energy), which adds to the force array.  Both use the module's stream.
+int sa::pme_spread_waves(const sponge_pme* p) { return p->spread_waves; }
+
+void sa::pme_set_spread_waves(sponge_pme* p, int waves) {
+    if (waves >= 1 && waves <= p->spread_waves) p->spread_waves = waves;
+}
+
 int sa::pme_recip_begin(sponge_pme* p, const float* crd, const float* cell, const float* rcell,
                         const float* charge, int need_virial, int need_energy, int step) {
     const LT6 c = lt_from(cell), r = lt_from(rcell);
@@ -1382,18 +1473,17 @@ int sa::pme_recip_begin(sponge_pme* p, const float* crd, const float* cell, cons
     const int N = p->N;
     {
         ScopedTimer timer("pme_spread", s);
-        hipLaunchKernelGGL(k_pme_bin, dim3(grid_for(N, 256)), dim3(256), 0, s, N, crd, charge, r,
-                           p->fx, p->fy, p->fz, p->bx, p->by, p->bz, p->nbx, p->nby, p->nbz, p->nb,
+        SA_LAUNCH(256, (&pme_bin_body), (k_pme_bin), dim3(grid_for(N, 256)), dim3(256), 0, s, N, crd, charge, r, p->fx, p->fy, p->fz, p->bx, p->by, p->bz, p->nbx, p->nby, p->nbz, p->nb,
                            p->bcap, p->d_bcount, p->d_bw, p->d_bu, p->d_bi, p->d_ow, p->d_ou,
                            p->d_oi, p->d_acc);
         SA_LAUNCH_CHECK();
         const TileGeom tg = pme_tile_geom(p);
         const size_t sh = sizeof(float) * (size_t)p->spread_waves * tg.lds_floats();
-        hipLaunchKernelGGL(k_pme_spread_tiles, dim3(p->nb), dim3(kWave * p->spread_waves), sh, s,
+        SA_LAUNCH(256, (&pme_spread_tiles_body), (k_pme_spread_tiles), dim3(p->nb), dim3(kWave * p->spread_waves), sh, s,
                            p->bx, p->by, p->bz, p->nbx, p->nby, p->nbz, p->nb, p->bcap, tg,
                            p->d_bcount, p->d_bw, p->d_bu, p->d_ow, p->d_ou, p->d_tiles);
         SA_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_pme_tiles_reduce, dim3(p->nb), dim3(256), 0, s, p->fx, p->fy, p->fz,
+        SA_LAUNCH(256, (&pme_tiles_reduce_body), (k_pme_tiles_reduce), dim3(p->nb), dim3(256), 0, s, p->fx, p->fy, p->fz,
                            p->bx, p->by, p->bz, p->nbx, p->nby, p->nbz, p->nb, tg, p->d_tiles,
                            p->d_Q, p->d_bcount, p->d_gcount);
         SA_LAUNCH_CHECK();
@@ -1416,7 +1506,7 @@ int sa::pme_recip_end(sponge_pme* p, const float* rcell, const float* charge, fl
         }
         ScopedTimer timer("pme_gather", s);
         const TileGeom tg = pme_tile_geom(p);
-        hipLaunchKernelGGL(k_pme_gather_tiles, dim3(p->nb), dim3(kGatherThreads),
+        SA_LAUNCH(256, (&pme_gather_tiles_body), (k_pme_gather_tiles), dim3(p->nb), dim3(kGatherThreads),
                            sizeof(float) * tg.lds_floats(), s, p->fx, p->fy, p->fz, p->bx, p->by,
                            p->bz, p->nbx, p->nby, p->nbz, p->nb, p->bcap, tg, p->d_gcount, p->d_bw,
                            p->d_bu, p->d_bi, p->d_ow, p->d_ou, p->d_oi, p->d_FBCFQ, r,
@@ -1480,13 +1570,11 @@ extern "C" int sponge_pme_excluded_force(sponge_pme* p, int local_atom_numbers,
     const LT6 c = lt_from(cell), r = lt_from(rcell);
     dim3 grid(grid_for(local_atom_numbers, 256)), block(256);
     if (lt_is_orthorhombic(c))
-        hipLaunchKernelGGL(k_pme_excluded<true>, grid, block, 0, s, local_atom_numbers, crd, charge,
-                           c, r, p->beta, excluded_list_start, excluded_list, excluded_numbers, frc,
+        SA_LAUNCH(256, (&pme_excluded_body<true>), (k_pme_excluded<true>), grid, block, 0, s, local_atom_numbers, crd, charge, c, r, p->beta, excluded_list_start, excluded_list, excluded_numbers, frc,
                            need_energy, atom_energy, p->d_correction_atom_energy, need_virial,
                            atom_virial);
     else
-        hipLaunchKernelGGL(k_pme_excluded<false>, grid, block, 0, s, local_atom_numbers, crd,
-                           charge, c, r, p->beta, excluded_list_start, excluded_list,
+        SA_LAUNCH(256, (&pme_excluded_body<false>), (k_pme_excluded<false>), grid, block, 0, s, local_atom_numbers, crd, charge, c, r, p->beta, excluded_list_start, excluded_list,
                            excluded_numbers, frc, need_energy, atom_energy,
                            p->d_correction_atom_energy, need_virial, atom_virial);
     SA_LAUNCH_CHECK();
