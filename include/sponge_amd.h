@@ -564,6 +564,17 @@ int sponge_md_set_state(sponge_md* md, const float* crd, const float* vel);
 /* The module handles the driver owns (for timing and list export). */
 int sponge_md_handles(sponge_md* md, sponge_nl** nl, sponge_lj** lj, sponge_pme** pme);
 
+/* Inner rows of the driver's pair kernel (no counterpart in the reference):
+ * the list's entries within cutoff + delta (Angstrom) at the last prune,
+ * pruned again once an atom moved more than delta / 2 since then or the list
+ * was rebuilt, so the pair kernel skips most pairs beyond the cutoff.  The
+ * pair set of every force is unchanged (every pair within the cutoff is in
+ * the inner rows); the list itself (sponge_nl_export_half_list) is the
+ * reference's.  delta = 0 reads the full rows.  Default
+ * SPONGE_AMD_MD_LIST_PRUNE_DELTA; at most the skin. */
+#define SPONGE_AMD_MD_LIST_PRUNE_DELTA 0.0f
+int sponge_md_set_list_prune(sponge_md* md, float delta);
+
 /* The driver's list is rebuilt from the coordinates of the next step, alone or
  * batched (NEIGHBOR_LIST::Update with update = FORCED, neighbor_list.cpp:
  * 983-998, for the list the driver holds). */

@@ -148,6 +148,7 @@ SIGNATURES = {
     "sponge_md_get_state": (_I, [_P, _FP, _FP, _FP]),
     "sponge_md_set_state": (_I, [_P, _FP, _FP]),
     "sponge_md_request_list_update": (_I, [_P]),
+    "sponge_md_set_list_prune": (_I, [_P, _F]),
     "sponge_md_handles": (_I, [_P, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),
     "sponge_md_clear": (_I, [_P]),
 }
@@ -756,6 +757,10 @@ class MDSystem:
         o = NLOptions()
         _check(self.lib.sponge_nl_get_options(nl, C.byref(o)), "sponge_nl_get_options")
         return {k: getattr(o, k) for k, _ in NLOptions._fields_}
+
+    def set_list_prune(self, delta):
+        """Inner rows within cutoff + delta for the pair kernel (0: off)."""
+        _check(self.lib.sponge_md_set_list_prune(self.h, float(delta)), "sponge_md_set_list_prune")
 
     def request_list_update(self):
         """The next step (alone or batched) rebuilds the list."""

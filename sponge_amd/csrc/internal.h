@@ -53,6 +53,14 @@ struct sponge_nl {
     int* d_exp_count = nullptr;
     sa_atom_group* d_atom_group = nullptr;
     int built = 0;
+    // inner (pruned) rows used by the MD driver's pair kernel: the entries of
+    // the rows within cutoff + prune_delta at the last prune, pruned again once
+    // an atom moved more than prune_delta / 2 since then or the list was rebuilt
+    float prune_delta = 0.f;          // 0: the pair kernel reads the full rows
+    int* d_need_prune = nullptr;      // prune flag of this step
+    float* d_prune_crd = nullptr;     // VECTOR[N] at the last prune
+    int* d_ilist_count = nullptr;     // [N] per sorted atom
+    unsigned short* d_ilist = nullptr;  // [N][cap]
 };
 
 struct sponge_lj {
@@ -203,6 +211,10 @@ int md_overflow_check(sponge_md* m, long long first, long long last);
 long long md_step_count(const sponge_md* m);
 int md_generation(const sponge_md* m);  // changes when the list is re-initialised
 int md_state_arrays(sponge_md* m, float** crd, float** charge);
+
+// inner-row pruning of the list (delta > 0 on, 0 off): allocates the inner
+// rows on first use and marks them for a prune on the next step
+int nl_set_prune(sponge_nl* nl, float delta);
 
 // the list's sticky overflow flags, read (synchronising) and cleared
 int nl_take_flags(sponge_nl* nl, int* flags);

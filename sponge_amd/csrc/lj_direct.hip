@@ -346,6 +346,84 @@ __global__ void __launch_bounds__(1024) k_lj_cells(
     lj_cells_body<ORTHO, NEED_E, NEED_V, FAST, SOLV>(blockIdx.x, start, nranges, max_ranges, g_rfirst, g_roff, list_count, list, cap, pos, ptype, pgid, AB_sq, T, c, r, rc2_thr, beta, stage, gl, Nx, Ny, Nz, reach, facc, eacc, vacc, solvent_start);
 }
 
+// Prune of the full rows into the inner rows that the MD driver's pair
+// kernel reads: one workgroup per cell stages the positions of the cell's
+// candidate stream (this step's coordinates, packed by k_nl_pack); one
+// wavefront per row keeps, in row order, the entries within cutoff + delta
+// (keep2 = (cutoff + delta)^2) and records the row atom's position of this
+// prune.  It runs only on steps whose device flags ask for it: the list was
+// rebuilt, or an atom moved more than delta / 2 since the last prune.
+template <bool ORTHO>
+__device__ __forceinline__ void lj_prune_cells_body(
+    int sa_block, const int* __restrict__ need, const int* __restrict__ need_prune,
+    const int* __restrict__ start, const int* __restrict__ nranges, int max_ranges,
+    const int* __restrict__ g_rfirst, const int* __restrict__ g_roff,
+    const int* __restrict__ list_count, const unsigned short* __restrict__ list, int cap,
+    const float4* __restrict__ pos, const int* __restrict__ pgid, LT6 c, LT6 r, float keep2,
+    int stage, int* __restrict__ ilist_count, unsigned short* __restrict__ ilist,
+    float* __restrict__ prune_crd) {
+    if (*need == 0 && *need_prune == 0) return;
+    extern __shared__ float4 cpos[];
+    __shared__ int r_first[kMaxRanges];
+    __shared__ int r_off[kMaxRanges + 1];
+    const int cell = sa_block;
+    const int a0 = start[cell], a1 = start[cell + 1];
+    if (a0 == a1) return;
+    const int nr = nranges[cell];
+    for (int k = threadIdx.x; k < nr; k += blockDim.x) r_first[k] = g_rfirst[(size_t)cell * max_ranges + k];
+    for (int k = threadIdx.x; k <= nr; k += blockDim.x) r_off[k] = g_roff[(size_t)cell * (max_ranges + 1) + k];
+    __syncthreads();
+    const int total = r_off[nr];
+    const bool staged = total <= stage;
+    if (staged) stage_stream(nr, r_first, r_off, total, [&](int t, int q) { cpos[t] = pos[q]; });
+    __syncthreads();
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave, nwave = blockDim.x / kWave;
+    for (int p = a0 + wave; p < a1; p += nwave) {
+        const float4 pi4 = pos[p];
+        const float3 pi = make_float3(pi4.x, pi4.y, pi4.z);
+        const int n = list_count[p];
+        const unsigned short* row = list + (size_t)p * cap;
+        unsigned short* out = ilist + (size_t)p * cap;
+        int kept = 0;
+        for (int k0 = 0; k0 < n; k0 += kWave) {
+            const int k = k0 + lane;
+            bool keep = false;
+            unsigned short t = 0;
+            if (k < n) {
+                t = row[k];
+                const float4 pj = staged ? cpos[t] : pos[stream_to_sorted(t, nr, r_first, r_off)];
+                const float3 d = disp_pbc<ORTHO>(make_float3(pj.x, pj.y, pj.z), pi, c, r);
+                keep = norm2_exact(d) < keep2;
+            }
+            const unsigned long long m = __ballot(keep);
+            if (keep) out[kept + lanes_below(m)] = t;
+            kept += __popcll(m);
+        }
+        if (lane == 0) {
+            ilist_count[p] = kept;
+            const int g = pgid[p];
+            prune_crd[3 * g] = pi4.x;
+            prune_crd[3 * g + 1] = pi4.y;
+            prune_crd[3 * g + 2] = pi4.z;
+        }
+    }
+}
+
+template <bool ORTHO>
+__global__ void __launch_bounds__(1024) k_lj_prune_cells(
+    const int* __restrict__ need, const int* __restrict__ need_prune,
+    const int* __restrict__ start, const int* __restrict__ nranges, int max_ranges,
+    const int* __restrict__ g_rfirst, const int* __restrict__ g_roff,
+    const int* __restrict__ list_count, const unsigned short* __restrict__ list, int cap,
+    const float4* __restrict__ pos, const int* __restrict__ pgid, LT6 c, LT6 r, float keep2,
+    int stage, int* __restrict__ ilist_count, unsigned short* __restrict__ ilist,
+    float* __restrict__ prune_crd) {
+    lj_prune_cells_body<ORTHO>(blockIdx.x, need, need_prune, start, nranges, max_ranges, g_rfirst,
+                               g_roff, list_count, list, cap, pos, pgid, c, r, keep2, stage,
+                               ilist_count, ilist, prune_crd);
+}
+
 // Scatter of the per-row accumulators to the caller's arrays: frc +=,
 // atom_energy += E_LJ + E_C, LJ energy and direct-PME energy overwritten
 // (the reference clears them before its kernel, Lennard_Jones_force.cpp:
@@ -356,9 +434,10 @@ __device__ __forceinline__ void lj_finalize_body(int sa_block, int N, const int*
                               float* __restrict__ frc, float* __restrict__ atom_energy,
                               float* __restrict__ lj_energy, float* __restrict__ direct_energy,
                               float* __restrict__ atom_virial, int assign,
-                              int* __restrict__ reset_flag) {
+                              int* __restrict__ reset_flag, int* __restrict__ reset_flag2) {
     int p = sa_block * blockDim.x + threadIdx.x;
     if (p == 0 && reset_flag) *reset_flag = 0;
+    if (p == 0 && reset_flag2) *reset_flag2 = 0;
     if (p >= N) return;
     const int i = sidx ? sidx[p] : p;
     const float4 f = facc[p];
@@ -390,8 +469,10 @@ __global__ void k_lj_finalize(int N, const int* __restrict__ sidx, const float4*
                               float* __restrict__ frc, float* __restrict__ atom_energy,
                               float* __restrict__ lj_energy, float* __restrict__ direct_energy,
                               float* __restrict__ atom_virial, int assign,
-                              int* __restrict__ reset_flag) {
-    lj_finalize_body<NEED_E, NEED_V>(blockIdx.x, N, sidx, facc, eacc, vacc, frc, atom_energy, lj_energy, direct_energy, atom_virial, assign, reset_flag);
+                              int* __restrict__ reset_flag, int* __restrict__ reset_flag2) {
+    lj_finalize_body<NEED_E, NEED_V>(blockIdx.x, N, sidx, facc, eacc, vacc, frc, atom_energy,
+                                     lj_energy, direct_energy, atom_virial, assign, reset_flag,
+                                     reset_flag2);
 }
 
 // Identity-order pack for the reference-format list entry point.
@@ -428,7 +509,10 @@ __global__ void k_sum_f32(int n, const float* __restrict__ a, double* out) {
 // (the step driver); the public entry point makes no such assumption.
 template <bool ORTHO>
 int launch_rows(sponge_lj* lj, sponge_nl* nl, LT6 c, LT6 r, float beta, int need_e, int need_v,
-                hipStream_t s, bool allow_shift = false, int solvent_start = -1) {
+                hipStream_t s, bool allow_shift = false, int solvent_start = -1,
+                bool inner = false) {
+    const int* rows_count = inner ? nl->d_ilist_count : nl->d_list_count;
+    const unsigned short* rows = inner ? nl->d_ilist : nl->d_list;
     ScopedTimer timer("lj_pairs", s);
     const bool solv = solvent_start >= 0 && solvent_start < lj->N;
     const size_t sh =
@@ -444,7 +528,7 @@ int launch_rows(sponge_lj* lj, sponge_nl* nl, LT6 c, LT6 r, float beta, int need
     SA_LAUNCH(1024, (&lj_cells_body<ORTHO, E, V, F, SV>), (k_lj_cells<ORTHO, E, V, F, SV>),      \
               dim3(nl->G), dim3(1024), sh, s,                                                   \
                        nl->d_cell_start, nl->d_nranges, nl->max_ranges, nl->d_rfirst, nl->d_roff, \
-                       nl->d_list_count, nl->d_list, nl->cap, lj->d_pos, lj->d_ptype, lj->d_pgid, \
+                       rows_count, rows, nl->cap, lj->d_pos, lj->d_ptype, lj->d_pgid,             \
                        lj->d_AB_sq, lj->T, c, r, lj->rc2_thr, beta, lj->stage, nl->grid_length,   \
                        nl->Nx, nl->Ny, nl->Nz, reach, lj->d_facc, lj->d_eacc, lj->d_vacc,         \
                        solvent_start)
@@ -485,13 +569,13 @@ int launch_pairs(sponge_lj* lj, int nrows, const int* cnt, const int* list, int 
 
 int finalize(sponge_lj* lj, int N, const int* sidx, int need_e, int need_v, float* frc,
              float* atom_energy, float* atom_virial, float* direct, hipStream_t s, int assign = 0,
-             int* reset_flag = nullptr) {
+             int* reset_flag = nullptr, int* reset_flag2 = nullptr) {
     dim3 grid(grid_for(N, 256)), block(256);
 #define SA_FIN(E, V)                                                                            \
     SA_LAUNCH(256, (&lj_finalize_body<E, V>), (k_lj_finalize<E, V>), grid, block, 0, s, N, sidx, \
               lj->d_facc,                                                                       \
                        lj->d_eacc, lj->d_vacc, frc, atom_energy, lj->d_LJ_energy_atom, direct, \
-                       atom_virial, assign, reset_flag)
+                       atom_virial, assign, reset_flag, reset_flag2)
     if (need_e && need_v) SA_FIN(true, true);
     else if (need_e) SA_FIN(true, false);
     else if (need_v) SA_FIN(false, true);
@@ -621,13 +705,35 @@ int sa::lj_direct_force_md(sponge_lj* lj, sponge_nl* nl, const float* crd, const
     const LT6 c = lt_from(cell), r = lt_from(rcell);
     int rc = nl_pack_for_pairs(nl, crd, charge, lj->d_type, lj->d_pos, lj->d_ptype, lj->d_pgid, s);
     if (rc) return rc;
-    if (lt_is_orthorhombic(c))
-        rc = launch_rows<true>(lj, nl, c, r, pme_beta, need_atom_energy, 0, s, true);
+    // the inner rows (the driver's automatic refresh runs the displacement
+    // test that keeps them valid)
+    const bool inner = nl->prune_delta > 0.f && nl->opt.refresh_interval <= 0;
+    const bool ortho = lt_is_orthorhombic(c);
+    if (inner) {
+        ScopedTimer timer("lj_prune", s);
+        const float keep = lj->cutoff + nl->prune_delta;
+        const size_t sh = sizeof(float4) * lj->stage;
+        if (ortho)
+            SA_LAUNCH(1024, (&lj_prune_cells_body<true>), (k_lj_prune_cells<true>), dim3(nl->G),
+                      dim3(1024), sh, s, nl->d_need, nl->d_need_prune, nl->d_cell_start,
+                      nl->d_nranges, nl->max_ranges, nl->d_rfirst, nl->d_roff, nl->d_list_count,
+                      nl->d_list, nl->cap, lj->d_pos, lj->d_pgid, c, r, keep * keep, lj->stage,
+                      nl->d_ilist_count, nl->d_ilist, nl->d_prune_crd);
+        else
+            SA_LAUNCH(1024, (&lj_prune_cells_body<false>), (k_lj_prune_cells<false>), dim3(nl->G),
+                      dim3(1024), sh, s, nl->d_need, nl->d_need_prune, nl->d_cell_start,
+                      nl->d_nranges, nl->max_ranges, nl->d_rfirst, nl->d_roff, nl->d_list_count,
+                      nl->d_list, nl->cap, lj->d_pos, lj->d_pgid, c, r, keep * keep, lj->stage,
+                      nl->d_ilist_count, nl->d_ilist, nl->d_prune_crd);
+        SA_LAUNCH_CHECK();
+    }
+    if (ortho)
+        rc = launch_rows<true>(lj, nl, c, r, pme_beta, need_atom_energy, 0, s, true, -1, inner);
     else
-        rc = launch_rows<false>(lj, nl, c, r, pme_beta, need_atom_energy, 0, s, true);
+        rc = launch_rows<false>(lj, nl, c, r, pme_beta, need_atom_energy, 0, s, true, -1, inner);
     if (rc) return rc;
     return finalize(lj, lj->N, nl->d_sidx, need_atom_energy, 0, frc, atom_energy, nullptr,
-                    atom_direct_pme_energy, s, 1, reset_flag);
+                    atom_direct_pme_energy, s, 1, reset_flag, inner ? nl->d_need_prune : nullptr);
 }
 
 extern "C" int sponge_lj_pme_direct_force(sponge_lj* lj, sponge_nl* nl, int atom_numbers,
@@ -861,7 +967,7 @@ extern "C" int sponge_lj_pme_direct_force_list_noalloc(
 #define SA_NA_FIN(E, V)                                                                          \
     hipLaunchKernelGGL((k_lj_finalize<E, V>), fgrid, block, 0, s, N, nullptr, facc, eacc, vacc, \
                        frc, atom_energy, atom_LJ_energy, atom_direct_pme_energy, atom_virial, 0,  \
-                       nullptr)
+                       nullptr, nullptr)
     if (need_atom_energy && need_virial) SA_NA_FIN(true, true);
     else if (need_atom_energy) SA_NA_FIN(true, false);
     else if (need_virial) SA_NA_FIN(false, true);

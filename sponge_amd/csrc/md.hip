@@ -83,6 +83,7 @@ struct sponge_md {
     int overlap_mode = -1;  // 1 on, 0 off, -1 from the atom count
     float* d_exfrc = nullptr;  // [N][3] excluded-pair correction computed beside the pair work
     int nl_generation = 0;  // bumped when an overflow re-initialises the list
+    float prune_delta = SPONGE_AMD_MD_LIST_PRUNE_DELTA;  // inner-row buffer of the pair kernel
     int overflow_flags = 0;  // every overflow flag seen by the driver's checks
 };
 
@@ -603,6 +604,7 @@ int md_check_overflow(sponge_md* m, long long first, long long last) {
     if ((flags & SPONGE_AMD_NL_STREAM_OVERFLOW) || m->nl->opt.throw_error_when_overflow)
         return SPONGE_AMD_ERROR_OVERFLOW;
     rc = nl_reinitialize(&m->nl, flags);
+    if (!rc) rc = nl_set_prune(m->nl, m->prune_delta);
     if (rc) return rc;
     capture_destroy(m->capture);
     m->capture_len = 0;
@@ -743,6 +745,7 @@ extern "C" int sponge_md_initial(sponge_md** out, int atom_numbers, const float*
     if (he == hipSuccess) he = hipStreamSynchronize(s);
     int rc = he != hipSuccess ? (int)he : 0;
     if (!rc) rc = sponge_nl_initial(&m->nl, N, cutoff, skin, &m->cell.a11, &m->rcell.a11, nl_opt, stream);
+    if (!rc) rc = nl_set_prune(m->nl, m->prune_delta);
     if (!rc) rc = sponge_lj_initial(&m->lj, N, atom_type_numbers, LJ_A, LJ_B, atom_LJ_type, cutoff, stream);
     if (!rc) rc = sponge_pme_initial(&m->pme, N, &m->cell.a11, &m->rcell.a11, cutoff, pme_opt, stream);
     if (rc) {
@@ -876,6 +879,16 @@ int md_state_arrays(sponge_md* m, float** crd, float** charge) {
 }
 
 }  // namespace sa
+
+extern "C" int sponge_md_set_list_prune(sponge_md* m, float delta) {
+    if (!m || !(delta >= 0.f) || delta > m->skin) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    const int rc = nl_set_prune(m->nl, delta);
+    if (rc) return rc;
+    m->prune_delta = delta;
+    capture_destroy(m->capture);  // the captured step reads the other rows
+    m->capture_len = 0;
+    return 0;
+}
 
 extern "C" int sponge_md_request_list_update(sponge_md* m) {
     if (!m) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;

@@ -45,20 +45,31 @@ __global__ void k_nl_set(int* need, int v) { *need = v; }
 
 // Check_Refresh (neighbor_list.cpp:307-321)
 template <bool ORTHO>
-__device__ __forceinline__ void nl_check_body(int sa_block, int N, const float* __restrict__ crd, const float* __restrict__ old,
-                           LT6 c, LT6 r, float permit2, int* need) {
+__device__ __forceinline__ void nl_check_body(int sa_block, int N, const float* __restrict__ crd,
+                                              const float* __restrict__ old, LT6 c, LT6 r,
+                                              float permit2, int* need,
+                                              const float* __restrict__ pruned, float pmove2,
+                                              int* need_prune) {
     int i = sa_block * blockDim.x + threadIdx.x;
     if (i >= N) return;
     float3 a = make_float3(crd[3 * i], crd[3 * i + 1], crd[3 * i + 2]);
     float3 b = make_float3(old[3 * i], old[3 * i + 1], old[3 * i + 2]);
     float3 d = disp_pbc<ORTHO>(a, b, c, r);
     if (norm2_exact(d) > permit2) *need = 1;
+    if (pruned) {
+        // the inner rows hold every pair within cutoff + delta at the prune:
+        // valid while no atom moved more than delta / 2 since then
+        const float3 e = disp_pbc<ORTHO>(a, make_float3(pruned[3 * i], pruned[3 * i + 1],
+                                                        pruned[3 * i + 2]), c, r);
+        if (norm2_exact(e) > pmove2) *need_prune = 1;
+    }
 }
 
 template <bool ORTHO>
 __global__ void k_nl_check(int N, const float* __restrict__ crd, const float* __restrict__ old,
-                           LT6 c, LT6 r, float permit2, int* need) {
-    nl_check_body<ORTHO>(blockIdx.x, N, crd, old, c, r, permit2, need);
+                           LT6 c, LT6 r, float permit2, int* need, const float* __restrict__ pruned,
+                           float pmove2, int* need_prune) {
+    nl_check_body<ORTHO>(blockIdx.x, N, crd, old, c, r, permit2, need, pruned, pmove2, need_prune);
 }
 
 // Cell of an atom as Put_Atom_In_Grids wraps it (neighbor_list.cpp:357-377).
@@ -505,6 +516,10 @@ void nl_free(sponge_nl* nl) {
     dfree(nl->d_exp_serial);
     dfree(nl->d_exp_count);
     dfree(nl->d_atom_group);
+    dfree(nl->d_need_prune);
+    dfree(nl->d_prune_crd);
+    dfree(nl->d_ilist_count);
+    dfree(nl->d_ilist);
 }
 
 }  // namespace
@@ -646,10 +661,19 @@ int sa::nl_update_impl(sponge_nl* nl, const int* atom_local, int local_atom_numb
     } else if (auto_refresh) {
         if (!need_cleared) hipLaunchKernelGGL(k_nl_set, dim3(1), dim3(1), 0, s, nl->d_need, 0);
         const float permit2 = nl->skin * nl->skin * nl->opt.skin_permit * nl->opt.skin_permit;
+        const bool prune = nl->prune_delta > 0.f;
+        const float half = 0.5f * nl->prune_delta - 1e-3f;  // margin for the rounding of both tests
+        const float pmove2 = half * half;
+        const float* pruned = prune ? nl->d_prune_crd : nullptr;
+        int* need_prune = prune ? nl->d_need_prune : nullptr;
         if (nl->ortho)
-            SA_LAUNCH(256, (&nl_check_body<true>), (k_nl_check<true>), dim3(grid_for(N, 256)), dim3(256), 0, s, N, crd, nl->d_old_crd, c, r, permit2, nl->d_need);
+            SA_LAUNCH(256, (&nl_check_body<true>), (k_nl_check<true>), dim3(grid_for(N, 256)),
+                      dim3(256), 0, s, N, crd, nl->d_old_crd, c, r, permit2, nl->d_need, pruned,
+                      pmove2, need_prune);
         else
-            SA_LAUNCH(256, (&nl_check_body<false>), (k_nl_check<false>), dim3(grid_for(N, 256)), dim3(256), 0, s, N, crd, nl->d_old_crd, c, r, permit2, nl->d_need);
+            SA_LAUNCH(256, (&nl_check_body<false>), (k_nl_check<false>), dim3(grid_for(N, 256)),
+                      dim3(256), 0, s, N, crd, nl->d_old_crd, c, r, permit2, nl->d_need, pruned,
+                      pmove2, need_prune);
     } else {
         hipLaunchKernelGGL(k_nl_set, dim3(1), dim3(1), 0, s, nl->d_need,
                            ((step + 1) % nl->opt.refresh_interval == 0) ? 1 : 0);
@@ -699,6 +723,23 @@ int sa::nl_update_impl(sponge_nl* nl, const int* atom_local, int local_atom_numb
         SA_LAUNCH_CHECK();
     }
     nl->built = 1;
+    return 0;
+}
+
+int sa::nl_set_prune(sponge_nl* nl, float delta) {
+    if (!(delta >= 0.f)) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    if (delta > 0.f && !nl->d_ilist) {
+        int e = 0;
+        e |= dmalloc(&nl->d_need_prune, 1);
+        e |= dmalloc(&nl->d_prune_crd, (size_t)3 * nl->N);
+        e |= dmalloc(&nl->d_ilist_count, nl->N);
+        e |= dmalloc(&nl->d_ilist, (size_t)nl->N * nl->cap);
+        if (e) return SPONGE_AMD_ERROR_MALLOC_FAILED;
+        const int rc = fill_async(nl->d_ilist_count, 0, nl->N, nl->stream);
+        if (rc) return rc;
+    }
+    nl->prune_delta = delta;
+    if (delta > 0.f) return fill_async(nl->d_need_prune, 1, 1, nl->stream);
     return 0;
 }
 

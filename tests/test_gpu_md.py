@@ -151,3 +151,30 @@ def test_kernel_timers_inside_graph(dev):
         abi.set_kernel_timing(False)
         abi.set_kernel_timing_filter(None)
         abi.reset_kernel_timing()
+
+
+@pytest.mark.parametrize("delta", [0.0, 0.5, 1.0])
+def test_pruned_rows_keep_every_pair(dev, delta):
+    """The pair kernel of the step driver reads inner rows pruned to cutoff +
+    delta (pruned again once an atom moved delta / 2): after 200 steps the
+    force of a step is the oracle's force of its coordinates (no pair within
+    the cutoff missing), for several buffers and for the full rows."""
+    from oracle import oracle as O
+    from sponge_amd import abi
+    from sponge_amd import system as S
+    s = S.config(2)
+    md = abi.MDSystem(s)
+    md.set_list_prune(delta)
+    md.run(200, use_graph=True)
+    crd, _, _ = md.state()
+    md.run(1)
+    _, _, frc = md.state()
+    beta = O.get_beta(s.cutoff, s.tolerance)
+    c, ser, _ = O.nl_build(s, crd=crd)
+    ref = O.lj_direct(s, c, ser, beta, crd=crd, need_energy=False,
+                      need_virial=False)["frc"].astype(np.float64)
+    ref += O.pme_excluded(s, beta, crd=crd, need_energy=False, need_virial=False)["frc"]
+    ref += O.PME(s, s.fft, beta).reciprocal(crd=crd, need_energy=False, need_virial=False)["frc"]
+    err = float(np.sqrt(((frc - ref) ** 2).sum() / (ref ** 2).sum()))
+    print(f"case: prune delta {delta}: force rel RMS {err:.2e}")
+    assert err < 1e-5
