@@ -164,17 +164,20 @@ def cpu_baseline(s, gpu_rebuild_interval, budget_s=20.0):
                 ms_split=full["ms_split"], cpu_model=cpu_model(), one_thread=one)
 
 
-def learned_form(md, s, torch, abi, n_train=40, n_heldout=16, stride=5, epochs=30, lr=2e-3,
-                 period=4, check_interval=8, bound=0.5, timed_steps=400, drift_steps=1000):
+def learned_form(md, s, torch, abi, n_train=200, n_heldout=16, stride=25, epochs=30, lr=2e-3,
+                 period=4, check_interval=8, bound=0.01, timed_steps=400, drift_steps=10000):
     """The learned multiple-time-step form on the same system: training
     frames from the plain engine (stride steps apart) and consecutive
     held-out frames, the surrogate trained on the GPU, its reciprocal-force
     error 1-3 steps after an exact PME step (the schedule's lags), the graph-
-    replayed ms/step of both schedules and the NVE energy drift of both from
-    the same state."""
+    replayed ms/step of the schedules, and the NVE energy drift over
+    drift_steps of: the plain step, the reference's own impulse schedule (PME
+    update_interval = period), the surrogate in the impulse schedule (trained
+    and untrained) and the anchored schedule, all from the same state."""
     beta = abi.load().sponge_amd_get_beta(float(s.cutoff), float(s.tolerance))
     rs = abi.ReciprocalSurrogate(s.n_atoms, beta, feature_cutoff=s.cutoff,
                                  max_frames=n_train + n_heldout)
+    model0 = rs.get_model()
     t0 = time.perf_counter()
     md.collect_frames(rs, n_train, stride)
     md.collect_frames(rs, n_heldout, 1)
@@ -183,6 +186,7 @@ def learned_form(md, s, torch, abi, n_train=40, n_heldout=16, stride=5, epochs=3
     t0 = time.perf_counter()
     losses = rs.train(n_train, epochs, lr)
     t_train = time.perf_counter() - t0
+    trained = rs.get_model()
     ev = {lag: rs.evaluate(n_train + 3, n_train + n_heldout, lag) for lag in (1, 2, 3)}
     crd0, vel0, _ = md.state()
 
@@ -196,10 +200,9 @@ def learned_form(md, s, torch, abi, n_train=40, n_heldout=16, stride=5, epochs=3
     md.set_surrogate(None)
     md.run(8, use_graph=True)
     plain_ms = timed(timed_steps)
-    md.set_surrogate(rs, period, check_interval, bound)
+    md.set_surrogate(rs, period, check_interval, 1e9)
     md.run(2 * period * check_interval, use_graph=True)
     learned_ms = timed(timed_steps)
-    stats = md.surrogate_stats()
     # kernel breakdown of the learned schedule (eager, timers on)
     abi.set_kernel_timing_filter(["rsur_features", "rsur_mlp"])
     abi.reset_kernel_timing()
@@ -210,23 +213,47 @@ def learned_form(md, s, torch, abi, n_train=40, n_heldout=16, stride=5, epochs=3
     kt = {k: abi.kernel_timing(k) for k in ("rsur_features", "rsur_mlp")}
     abi.set_kernel_timing_filter(None)
 
-    def drift(use):
-        md.set_state(crd0, vel0)
-        md.set_surrogate(rs if use else None, period, check_interval, bound)
+    every = drift_steps // 10
+
+    def drift(system, setup):
+        system.set_state(crd0, vel0)
+        setup(system)
         e = []
-        every = drift_steps // 10
         for k in range(11):
-            md.run(1, energy_interval=1)
-            en = md.energies()
+            system.run(1, energy_interval=1)
+            en = system.energies()
             e.append(en["potential"] + en["kinetic"])
             if k < 10:
-                md.run(every - 1, use_graph=True)
+                system.run(every - 1, use_graph=True)
         x = np.arange(11) * every
         slope = float(np.polyfit(x, np.asarray(e), 1)[0])
-        return slope * 1000.0, e
+        return slope * 1000.0
 
-    d_plain, e_plain = drift(False)
-    d_learned, e_learned = drift(True)
+    def learned(mode, model):
+        def f(system):
+            rs.set_model(model)
+            system.set_surrogate_mode(mode)
+            system.set_surrogate(rs, period, check_interval, 1e9)
+        return f
+
+    d = {"plain": drift(md, lambda m: m.set_surrogate(None)),
+         "learned_impulse": drift(md, learned(1, trained)),
+         "untrained_impulse": drift(md, learned(1, model0)),
+         "learned_anchored": drift(md, learned(0, trained))}
+    stats = md.surrogate_stats()
+    md.set_surrogate(None)
+    md.set_surrogate_mode(1)
+    rs.set_model(trained)
+    # the reference's own impulse schedule: PME every period-th step, kicked
+    ref = abi.MDSystem(s, pme_opts={"update_interval": period})
+    d["pme_update_interval"] = drift(ref, lambda m: None)
+    ref.clear()
+    # the fallback test at the stated bound: a sampled step whose anchored
+    # error exceeds it takes PME's force and switches the surrogate off
+    md.set_state(crd0, vel0)
+    md.set_surrogate(rs, period, check_interval, bound)
+    md.run(4 * period * check_interval, use_graph=True)
+    fb = md.surrogate_stats()
     md.set_surrogate(None)
     rmse = {f"{lag}_step": ev[lag]["model_anchored"] for lag in (1, 2, 3)}
     return dict(
@@ -240,15 +267,17 @@ def learned_form(md, s, torch, abi, n_train=40, n_heldout=16, stride=5, epochs=3
                                       network_anchored=ev[lag]["model_anchored"])
                   for lag in (1, 2, 3)},
         network_alone_rel_rmse=ev[1]["model"], baseline_alone_rel_rmse=ev[1]["baseline"],
-        energy_drift_kcal_mol_per_1000_steps=dict(plain=d_plain, learned=d_learned),
-        checks=stats["checks"], fallbacks=stats["fallbacks"],
+        energy_drift_kcal_mol_per_1000_steps=d, drift_steps=drift_steps,
+        checks=stats["checks"], fallback_bound=bound,
+        fallback_run=dict(checks=fb["checks"], fallbacks=fb["fallbacks"],
+                          switched_off=fb["switched_off"], last_error=fb["last_error"]),
         kernel_avg_us={k: (v[0] * 1e3 / v[1] if v[1] else None) for k, v in kt.items()},
         training=dict(frames=n_train, heldout=n_heldout, stride_steps=stride, epochs=epochs,
                       lr=lr, loss_first_epoch=losses[0], loss_last_epoch=losses[-1],
                       collect_s=t_collect, train_s=t_train),
-        schedule=f"PME every {period}th step (and on energy steps), the surrogate anchored on the "
-                 f"last PME step in between; every {check_interval}th period one surrogate step "
-                 f"is checked against PME, fallback above {bound} relative RMS error")
+        schedule=f"impulse: PME every {period}th step (and on energy steps) with the kick "
+                 f"{period} x (F_PME - F_surrogate), the surrogate's force on the other steps; "
+                 f"every {check_interval}th period one surrogate step is checked against PME")
 
 
 def batched_form(torch, abi, S, steps=500, steps_per_launch=10, n_boxes=48):

@@ -614,6 +614,17 @@ int sponge_md_set_reciprocal_overlap(sponge_md* md, int mode);
 int sponge_md_set_surrogate(sponge_md* md, sponge_rsur* rsur, int period, int check_interval,
                             float bound);
 
+/* Force of the surrogate steps (no counterpart in the reference):
+ * mode 1 (default), impulse: the surrogate's force on every step and, on each
+ * period-th (PME) step, the kick period x (F_PME - F_surrogate) -- the
+ * reference's update_interval splitting (PM_force.cpp:1082, 1135-1140) with the
+ * surrogate as the part evaluated every step, which is time-reversible and
+ * conserves energy as far as the surrogate force is a gradient; mode 0,
+ * anchored: F_surrogate(t) + F_PME(t0) - F_surrogate(t0) on every step, the
+ * smaller instantaneous error but no conserved energy.  The checked steps
+ * measure the anchored force's error in both modes. */
+int sponge_md_set_surrogate_mode(sponge_md* md, int mode);
+
 /* out[4] = {surrogate steps, checked steps, fallbacks, switched off (0/1)};
  * *last_error = the relative RMS error of the last checked step (either may
  * be NULL). */

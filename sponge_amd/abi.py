@@ -149,6 +149,7 @@ SIGNATURES = {
     "sponge_md_set_state": (_I, [_P, _FP, _FP]),
     "sponge_md_request_list_update": (_I, [_P]),
     "sponge_md_set_list_prune": (_I, [_P, _F]),
+    "sponge_md_set_surrogate_mode": (_I, [_P, _I]),
     "sponge_md_handles": (_I, [_P, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),
     "sponge_md_clear": (_I, [_P]),
 }
@@ -737,6 +738,11 @@ class MDSystem:
         _check(self.lib.sponge_md_set_surrogate(self.h, rsur.h if rsur is not None else None,
                                                 int(period), int(check_interval), float(bound)),
                "sponge_md_set_surrogate")
+
+    def set_surrogate_mode(self, mode):
+        """1: impulse schedule (default), 0: anchored."""
+        _check(self.lib.sponge_md_set_surrogate_mode(self.h, int(mode)),
+               "sponge_md_set_surrogate_mode")
 
     def surrogate_stats(self):
         out = (C.c_longlong * 4)()

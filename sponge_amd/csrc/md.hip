@@ -72,6 +72,11 @@ struct sponge_md {
     float* d_sur = nullptr;        // [3N] surrogate force of a checked step
     float* d_anchor = nullptr;     // [3N] PME minus surrogate force of the last PME step
     bool anchor_valid = false;
+    // 1: impulse schedule (the surrogate's force every step, the slow part
+    // period x (F_PME - F_surrogate) as a kick on the PME steps: the
+    // reference's update_interval splitting with the surrogate as the fast
+    // part); 0: anchored (F_surrogate(t) + F_PME(t0) - F_surrogate(t0))
+    int sur_mode = 1;
     double* d_sacc = nullptr;      // [0..2] sum of rec, [3] error^2, [4] reference^2, [5] last error
     unsigned long long* d_scount = nullptr;  // [0] checks, [1] fallbacks
     int use_graph = 0;
@@ -224,6 +229,7 @@ __global__ void k_add_vec(int n, const float* __restrict__ a, float* __restrict_
 // check of a surrogate step against PME: acc[3] += |sur - (rec - mean)|^2,
 // acc[4] += |rec - mean|^2 (mean from acc[0..2], the sums of rec)
 __global__ void __launch_bounds__(256) k_sur_check(int N, const float* __restrict__ sur,
+                                                   const float* __restrict__ anchor,
                                                    const float* __restrict__ rec,
                                                    double* __restrict__ acc) {
     __shared__ double s[2][256];
@@ -231,7 +237,13 @@ __global__ void __launch_bounds__(256) k_sur_check(int N, const float* __restric
     double e = 0, r = 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
         const double px = rec[3 * i] - mx, py = rec[3 * i + 1] - my, pz = rec[3 * i + 2] - mz;
-        const double dx = sur[3 * i] - px, dy = sur[3 * i + 1] - py, dz = sur[3 * i + 2] - pz;
+        double sx = sur[3 * i], sy = sur[3 * i + 1], sz = sur[3 * i + 2];
+        if (anchor) {
+            sx += anchor[3 * i];
+            sy += anchor[3 * i + 1];
+            sz += anchor[3 * i + 2];
+        }
+        const double dx = sx - px, dy = sy - py, dz = sz - pz;
         e += dx * dx + dy * dy + dz * dz;
         r += px * px + py * py + pz * pz;
     }
@@ -267,13 +279,16 @@ __global__ void k_sur_select(int N, const float* __restrict__ sur, const float* 
     }
 }
 
-// anchor = PME - surrogate of this step; the force takes PME's
+// anchor = PME - surrogate of this step; the force takes PME's (anchored
+// schedule, kick = 1) or the surrogate's plus period x the anchor (impulse
+// schedule, kick = period)
 __global__ void k_anchor(int n, const float* __restrict__ rec, const float* __restrict__ sur,
-                         float* __restrict__ anchor, float* __restrict__ frc) {
+                         float* __restrict__ anchor, float* __restrict__ frc, float kick) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    anchor[i] = rec[i] - sur[i];
-    frc[i] += rec[i];
+    const float a = rec[i] - sur[i];
+    if (anchor) anchor[i] = a;
+    frc[i] += sur[i] + kick * a;
 }
 
 __global__ void k_kinetic(int N, const float* __restrict__ vel, const float* __restrict__ mass,
@@ -500,8 +515,12 @@ int md_step(sponge_md* m, long long step, int need_energy, StepKind* kind_out = 
                                       rcell, m->d_excl_start, m->d_excl_list, m->d_excl_num);
         if (!rc) rc = rsur_predict_sorted(m->rsur, m->d_sur, 1, nullptr);
         if (rc) return rc;
+        // the impulse schedule kicks only on the period's PME step: an energy
+        // step between two of them takes the surrogate's force
+        const bool on_period = step % m->sur_period == 0;
+        const float kick = m->sur_mode == 0 ? 1.0f : (on_period ? (float)m->sur_period : 0.0f);
         hipLaunchKernelGGL(k_anchor, dim3(grid_for(N3, 256)), dim3(256), 0, s, N3, m->d_rec, m->d_sur,
-                           m->d_anchor, m->d_frc);
+                           on_period || m->sur_mode == 0 ? m->d_anchor : nullptr, m->d_frc, kick);
         SA_LAUNCH_CHECK();
         m->anchor_valid = true;
     } else if (kind == StepKind::Pme && !m->collect) {
@@ -530,11 +549,14 @@ int md_step(sponge_md* m, long long step, int need_energy, StepKind* kind_out = 
         rc = rsur_features_sorted(m->rsur, m->nl, m->lj->d_pos, m->d_crd, m->d_charge, cell, rcell,
                                   m->d_excl_start, m->d_excl_list, m->d_excl_num);
         if (rc) return rc;
+        // the anchored schedule adds F_PME(t0) - F_surrogate(t0); the impulse
+        // schedule applies the surrogate alone (the slow part was the kick)
+        const float* anchor = m->sur_mode == 0 ? m->d_anchor : nullptr;
         if (kind == StepKind::Surrogate) {
-            rc = rsur_predict_sorted(m->rsur, m->d_frc, 0, m->d_anchor);
+            rc = rsur_predict_sorted(m->rsur, m->d_frc, 0, anchor);
             if (rc) return rc;
         } else {
-            rc = rsur_predict_sorted(m->rsur, m->d_sur, 1, m->d_anchor);
+            rc = rsur_predict_sorted(m->rsur, m->d_sur, 1, anchor);
             if (!rc) rc = fill_async(m->d_rec, 0.f, N3, s);
             if (!rc) rc = fill_async(m->d_sacc, 0.0, 5, s);
             if (!rc)
@@ -543,7 +565,9 @@ int md_step(sponge_md* m, long long step, int need_energy, StepKind* kind_out = 
             if (rc) return rc;
             const int blocks = std::min(grid_for(N, 256), 256);
             hipLaunchKernelGGL(k_sum_vec3, dim3(blocks), dim3(256), 0, s, N, m->d_rec, m->d_sacc);
-            hipLaunchKernelGGL(k_sur_check, dim3(blocks), dim3(256), 0, s, N, m->d_sur, m->d_rec,
+            // the error is that of the anchored force in both schedules
+            hipLaunchKernelGGL(k_sur_check, dim3(blocks), dim3(256), 0, s, N, m->d_sur,
+                               anchor ? nullptr : m->d_anchor, m->d_rec,
                                m->d_sacc);
             hipLaunchKernelGGL(k_sur_select, dim3(grid_for(N3, 256)), dim3(256), 0, s, N, m->d_sur,
                                m->d_rec, m->d_sacc, m->sur_bound, m->d_frc, m->d_scount);
@@ -939,6 +963,15 @@ extern "C" int sponge_md_set_surrogate(sponge_md* m, sponge_rsur* rsur, int peri
     m->sur_period = period;
     m->sur_check = check_interval;
     m->sur_bound = bound;
+    return 0;
+}
+
+extern "C" int sponge_md_set_surrogate_mode(sponge_md* m, int mode) {
+    if (!m || (mode != 0 && mode != 1)) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    m->sur_mode = mode;
+    capture_destroy(m->capture);
+    m->capture_len = 0;
+    m->anchor_valid = false;  // the next step is a PME step of the new schedule
     return 0;
 }
 

@@ -257,3 +257,35 @@ def test_md_learned_schedule(dev, trained):
     e = md.energies()
     assert np.isfinite(e["potential"]) and np.isfinite(e["kinetic"])
     md.set_surrogate(None)
+
+
+def test_impulse_schedule_conserves_energy(dev, trained):
+    """The surrogate in the impulse schedule (its force every step, the kick
+    period x (F_PME - F_surrogate) on the PME steps) keeps the total energy
+    of the equilibrated box as the plain step does; the anchored schedule
+    (not a conservative force) heats it far more."""
+    md, rs, _, _, _ = trained
+    crd0, vel0, _ = md.state()
+
+    def drift(mode):
+        md.set_state(crd0, vel0)
+        if mode is None:
+            md.set_surrogate(None)
+        else:
+            md.set_surrogate_mode(mode)
+            md.set_surrogate(rs, period=4, check_interval=8, bound=1e9)
+        e = []
+        for k in range(9):
+            md.run(1, energy_interval=1)
+            en = md.energies()
+            e.append(en["potential"] + en["kinetic"])
+            md.run(63, use_graph=True)
+        return float(np.polyfit(np.arange(9) * 64, np.asarray(e), 1)[0]) * 1000.0
+
+    d_plain, d_impulse, d_anchored = drift(None), drift(1), drift(0)
+    md.set_surrogate(None)
+    md.set_surrogate_mode(1)
+    print(f"drift kcal/mol per 1000 steps: plain {d_plain:.2f}, impulse {d_impulse:.2f}, "
+          f"anchored {d_anchored:.2f}")
+    assert abs(d_impulse) < max(5.0 * abs(d_plain), 5.0)
+    assert abs(d_anchored) > 10.0 * abs(d_impulse)
