@@ -156,6 +156,12 @@ int sponge_nl_export_half_list(sponge_nl* nl, int* counts, int* serial, int max_
  * NEIGHBOR_LIST::d_nl directly.  The storage is owned by the list. */
 int sponge_nl_atom_group(sponge_nl* nl, void** d_nl);
 
+/* The address of that ATOM_GROUP array without filling it (allocated on
+ * first use, N x 16 bytes, stable until sponge_nl_clear): the value of
+ * NEIGHBOR_LIST::d_nl (neighbor_list.h:42) that callers pass to the force
+ * modules as the list's name, as SPONGE/main.cpp:1452-1470 does. */
+int sponge_nl_atom_group_address(sponge_nl* nl, void** d_nl);
+
 /* Statistics of the current list: entries and the cell grid dims. */
 int sponge_nl_info(sponge_nl* nl, long long* list_entries, int* grid_dims);
 

@@ -72,6 +72,7 @@ SIGNATURES = {
     "sponge_nl_get_options": (_I, [_P, C.POINTER(NLOptions)]),
     "sponge_nl_export_half_list": (_I, [_P, _P, _P, _I]),
     "sponge_nl_atom_group": (_I, [_P, C.POINTER(_P)]),
+    "sponge_nl_atom_group_address": (_I, [_P, C.POINTER(_P)]),
     "sponge_nl_info": (_I, [_P, C.POINTER(C.c_longlong), C.POINTER(_I)]),
     "sponge_nl_rebuild_count": (_I, [_P, C.POINTER(C.c_longlong)]),
     "sponge_nl_clear": (_I, [_P]),

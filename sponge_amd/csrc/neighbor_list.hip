@@ -808,10 +808,10 @@ extern "C" int sponge_nl_atom_group(sponge_nl* nl, void** d_nl) {
     if (!nl || !d_nl) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
     if (!nl->built) return SPONGE_AMD_ERROR_MISSING_COMMAND;
     const int M = nl->opt.max_neighbor_numbers;
+    if (!nl->d_atom_group && dmalloc(&nl->d_atom_group, nl->N)) return SPONGE_AMD_ERROR_MALLOC_FAILED;
     if (!nl->d_exp_serial) {
         int e = dmalloc(&nl->d_exp_serial, (size_t)nl->N * M);
         e |= dmalloc(&nl->d_exp_count, nl->N);
-        e |= dmalloc(&nl->d_atom_group, nl->N);
         if (e) return SPONGE_AMD_ERROR_MALLOC_FAILED;
     }
     hipLaunchKernelGGL(k_nl_export, dim3(nl->G), dim3(256), 0, nl->stream, nl->d_sidx, nl->d_spos,
@@ -819,6 +819,13 @@ extern "C" int sponge_nl_atom_group(sponge_nl* nl, void** d_nl) {
                        nl->d_list_count, nl->d_list, nl->cap, nl->d_exp_count, nl->d_exp_serial, M,
                        nl->d_atom_group, nl->d_flags);
     SA_LAUNCH_CHECK();
+    *d_nl = nl->d_atom_group;
+    return 0;
+}
+
+extern "C" int sponge_nl_atom_group_address(sponge_nl* nl, void** d_nl) {
+    if (!nl || !d_nl) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    if (!nl->d_atom_group && dmalloc(&nl->d_atom_group, nl->N)) return SPONGE_AMD_ERROR_MALLOC_FAILED;
     *d_nl = nl->d_atom_group;
     return 0;
 }

@@ -6,6 +6,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <map>
 #include <sstream>
 #include <string>
 
@@ -28,6 +29,22 @@ void check(int rc, const char* where) {
     else
         snprintf(reason, sizeof(reason), "Reason:\n\tstatus %d", rc);
     g_handler(rc, where, reason);
+}
+
+// d_nl -> the NEIGHBOR_LIST that made it
+std::map<const void*, NEIGHBOR_LIST*>& list_registry() {
+    static std::map<const void*, NEIGHBOR_LIST*> r;
+    return r;
+}
+
+sponge_nl* list_of(const ATOM_GROUP* d_nl, const char* where) {
+    NEIGHBOR_LIST* nl = NEIGHBOR_LIST::From_Device_List(d_nl);
+    if (!nl || !nl->handle) {
+        g_handler(SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND, where,
+                  "Reason:\n\tthe neighbor list is not a d_nl of this library's NEIGHBOR_LIST");
+        return nullptr;
+    }
+    return nl->handle;
 }
 
 std::string trim(const std::string& s) {
@@ -206,6 +223,18 @@ void NEIGHBOR_LIST::Initial_With_Options(const sponge_nl_options& o, int atom_nu
                             stream),
           "NEIGHBOR_LIST::Initial");
     is_initialized = handle != nullptr;
+    if (is_initialized) {
+        void* a = nullptr;
+        check(sponge_nl_atom_group_address(handle, &a), "NEIGHBOR_LIST::Initial");
+        d_nl = static_cast<ATOM_GROUP*>(a);
+        if (d_nl) list_registry()[d_nl] = this;
+    }
+}
+
+NEIGHBOR_LIST* NEIGHBOR_LIST::From_Device_List(const ATOM_GROUP* d_nl) {
+    auto& r = list_registry();
+    auto it = r.find(d_nl);
+    return it == r.end() ? nullptr : it->second;
 }
 
 void NEIGHBOR_LIST::Update(int* atom_local, int local_atom_numbers, int ghost_numbers, float* crd,
@@ -252,13 +281,15 @@ void NEIGHBOR_LIST::Check_Overflow(CONTROLLER* controller, int steps) {
     Initial_With_Options(o, atom_numbers, cutoff, skin, b, stream);
 }
 
-void* NEIGHBOR_LIST::Device_List() {
+ATOM_GROUP* NEIGHBOR_LIST::Device_List() {
     void* d = nullptr;
     if (is_initialized) check(sponge_nl_atom_group(handle, &d), "NEIGHBOR_LIST::Device_List");
-    return d;
+    return static_cast<ATOM_GROUP*>(d);
 }
 
 void NEIGHBOR_LIST::Clear() {
+    if (d_nl) list_registry().erase(d_nl);
+    d_nl = nullptr;
     if (handle) sponge_nl_clear(handle);
     handle = nullptr;
     is_initialized = false;
@@ -290,15 +321,19 @@ void LENNARD_JONES_INFORMATION::Initial(CONTROLLER* controller, float cutoff_, i
 
 void LENNARD_JONES_INFORMATION::LJ_PME_Direct_Force_With_Atom_Energy_And_Virial(
     int atom_numbers_, int local_atom_numbers, int solvent_numbers, int ghost_numbers,
-    const float* crd, const float* charge, float* frc, const Boundary boundary, NEIGHBOR_LIST* nl,
-    float pme_beta, int need_atom_energy, float* atom_energy, int need_virial, float* atom_virial,
-    float* atom_direct_pme_energy) {
+    const float* crd, const float* charge, float* frc, const Boundary boundary,
+    const ATOM_GROUP* nl, float pme_beta, int need_atom_energy, float* atom_energy,
+    int need_virial, float* atom_virial, float* atom_direct_pme_energy) {
+    static const char* where =
+        "LENNARD_JONES_INFORMATION::LJ_PME_Direct_Force_With_Atom_Energy_And_Virial";
     if (!is_initialized) return;
-    check(sponge_lj_pme_direct_force(handle, nl->handle, atom_numbers_, local_atom_numbers,
+    sponge_nl* list = list_of(nl, where);
+    if (!list) return;
+    check(sponge_lj_pme_direct_force(handle, list, atom_numbers_, local_atom_numbers,
                                      solvent_numbers, ghost_numbers, crd, charge, frc,
                                      boundary.cell, boundary.rcell, pme_beta, need_atom_energy,
                                      atom_energy, need_virial, atom_virial, atom_direct_pme_energy),
-          "LENNARD_JONES_INFORMATION::LJ_PME_Direct_Force_With_Atom_Energy_And_Virial");
+          where);
 }
 
 void LENNARD_JONES_INFORMATION::Long_Range_Correction(int need_pressure, float* d_virial,
@@ -321,6 +356,57 @@ void LENNARD_JONES_INFORMATION::Clear() {
     if (handle) sponge_lj_clear(handle);
     handle = nullptr;
     is_initialized = 0;
+}
+
+// ------------------------------------------------------- SOLVENT_LENNARD_JONES
+void SOLVENT_LENNARD_JONES::Initial(CONTROLLER* controller, LENNARD_JONES_INFORMATION* lj,
+                                    void* lj_soft, int atom_numbers, int residue_numbers,
+                                    const int* res_start, const float* mass, bool default_enable,
+                                    void* stream_) {
+    (void)lj_soft;
+    if (!lj || !lj->is_initialized) return;
+    // solvent_LJ = true / false; absent -> -1 (the default gating)
+    int key = -1;
+    if (controller && controller->Command_Exist("solvent_LJ"))
+        key = controller->Get_Bool("", "solvent_LJ", false) ? 1 : 0;
+    stream = stream_;
+    lj_info = lj;
+    check(sponge_solvent_lj_initial(&handle, lj->handle, atom_numbers, residue_numbers, res_start,
+                                    mass, key, default_enable ? 1 : 0, stream),
+          "SOLVENT_LENNARD_JONES::Initial");
+    if (!handle) return;
+    int enabled = 0;
+    check(sponge_solvent_lj_info(handle, &enabled, &solvent_numbers, &solvent_start,
+                                 &water_points),
+          "SOLVENT_LENNARD_JONES::Initial");
+    is_initialized = enabled;
+    if (!enabled) solvent_numbers = 0;
+    local_solvent_numbers = solvent_numbers;
+}
+
+void SOLVENT_LENNARD_JONES::LJ_PME_Direct_Force_With_Atom_Energy_And_Virial(
+    int atom_numbers, int residue_numbers, const int* d_res_start, const float* crd,
+    const float* charge, float* frc, const Boundary boundary, const ATOM_GROUP* nl,
+    float pme_beta, int need_atom_energy, float* atom_energy, int need_virial,
+    float* atom_lj_virial, float* atom_direct_pme_energy) {
+    static const char* where =
+        "SOLVENT_LENNARD_JONES::LJ_PME_Direct_Force_With_Atom_Energy_And_Virial";
+    (void)d_res_start;
+    if (!is_initialized) return;
+    sponge_nl* list = list_of(nl, where);
+    if (!list) return;
+    check(sponge_solvent_lj_direct_force(handle, list, atom_numbers, residue_numbers, crd, charge,
+                                         frc, boundary.cell, boundary.rcell, pme_beta,
+                                         need_atom_energy, atom_energy, need_virial,
+                                         atom_lj_virial, atom_direct_pme_energy),
+          where);
+}
+
+void SOLVENT_LENNARD_JONES::Clear() {
+    if (handle) sponge_solvent_lj_clear(handle);
+    handle = nullptr;
+    is_initialized = 0;
+    solvent_numbers = local_solvent_numbers = 0;
 }
 
 // --------------------------------------------------------------- Particle_Mesh
@@ -363,7 +449,26 @@ void Particle_Mesh::Initial(CONTROLLER* controller, int atom_numbers_, const Bou
         fftz = f[2];
         d_direct_atom_energy = sponge_pme_direct_atom_energy(handle);
         is_initialized = 1;
+        for (int k = 0; k < 3; ++k) min_corner[k] = 0.0f;
+        max_corner[0] = boundary.cell[0];
+        max_corner[1] = boundary.cell[2];
+        max_corner[2] = boundary.cell[5];
     }
+}
+
+void Particle_Mesh::Update_Box(const Boundary boundary, LTMatrix3 g, float dt) {
+    if (!is_initialized) return;
+    check(sponge_pme_update_box(handle, boundary.cell, boundary.rcell), "Particle_Mesh::Update_Box");
+    auto move = [&](float* r) {
+        const float x = r[0] + dt * (r[0] * g.a11 + r[1] * g.a21 + r[2] * g.a31);
+        const float y = r[1] + dt * (r[1] * g.a22 + r[2] * g.a32);
+        const float z = r[2] + dt * r[2] * g.a33;
+        r[0] = x;
+        r[1] = y;
+        r[2] = z;
+    };
+    move(min_corner);
+    move(max_corner);
 }
 
 void Particle_Mesh::PME_Reciprocal_Force_With_Energy_And_Virial(

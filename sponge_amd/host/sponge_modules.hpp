@@ -4,7 +4,13 @@
 //
 //   NEIGHBOR_LIST              SPONGE/neighbor_list/neighbor_list.h:7-127
 //   LENNARD_JONES_INFORMATION  SPONGE/Lennard_Jones_force/Lennard_Jones_force.h:87-173
+//   SOLVENT_LENNARD_JONES      SPONGE/Lennard_Jones_force/solvent_LJ.h:9-43
 //   Particle_Mesh              SPONGE/PM_force/PM_force.h:152-209
+//
+// The force calls take the list as the reference's callers pass it,
+// NEIGHBOR_LIST::d_nl (SPONGE/main.cpp:1452-1470): a d_nl made by a
+// NEIGHBOR_LIST of this library names that list, whose cell-ordered form the
+// kernels read; any other pointer is an error.
 //
 // Configuration comes from the same mdin keys, legacy "key = value" or TOML
 // tables flattened to "<table>_<key>" (SPONGE/control.cpp:50-94).  Every
@@ -20,6 +26,18 @@
 #include "../../include/sponge_amd.h"
 
 namespace sponge_amd {
+
+// Lower-triangular 3x3 matrix, the reference's element order (SPONGE/utils/vector.hpp:257-259).
+struct LTMatrix3 {
+    float a11, a21, a22, a31, a32, a33;
+};
+
+// One atom's neighbor row, the reference's layout (SPONGE/common.h:177-182).
+struct ATOM_GROUP {
+    int atom_numbers;
+    int ghost_numbers;
+    int* atom_serial;
+};
 
 // The reference's Boundary: cell and rcell LTMatrix3 (SPONGE/utils/boundary.h:16-20).
 struct Boundary {
@@ -60,6 +78,10 @@ struct NEIGHBOR_LIST {
     bool throw_error_when_overflow = false;
     void* stream = nullptr;
     Boundary boundary0{};  // the box of Initial (a re-initialisation reuses it)
+    // the name of this list for the force calls (neighbor_list.h:42); a device
+    // array of ATOM_GROUP whose rows Device_List() fills.  A re-initialisation
+    // (Check_Overflow) gives a new value, as the reference's does.
+    ATOM_GROUP* d_nl = nullptr;
 
     // NEIGHBOR_LIST::Initial (neighbor_list.cpp:871-972)
     void Initial(CONTROLLER* controller, int atom_numbers, float cutoff, float skin,
@@ -74,9 +96,12 @@ struct NEIGHBOR_LIST {
     // written to the controller's commands), or, with
     // throw_error_when_overflow, the error handler gets spongeErrorOverflow
     void Check_Overflow(CONTROLLER* controller, int steps);
-    // the ATOM_GROUP d_nl device array for modules that read the list directly
-    void* Device_List();
+    // fills d_nl's rows (the reference's half list, max_neighbor_numbers per
+    // atom) for modules that read the list directly; returns d_nl
+    ATOM_GROUP* Device_List();
     void Clear();
+    // the list that d_nl names, or null
+    static NEIGHBOR_LIST* From_Device_List(const ATOM_GROUP* d_nl);
 
    private:
     void Initial_With_Options(const sponge_nl_options& o, int atom_numbers, float cutoff,
@@ -101,17 +126,48 @@ struct LENNARD_JONES_INFORMATION {
     void Initial(CONTROLLER* controller, float cutoff, int atom_numbers, int atom_type_numbers,
                  const float* LJ_A, const float* LJ_B, const int* atom_LJ_type,
                  void* stream = nullptr);
-    // LJ_PME_Direct_Force_With_Atom_Energy_And_Virial (:326-384)
+    // LJ_PME_Direct_Force_With_Atom_Energy_And_Virial (:326-384); nl is the
+    // list's d_nl
     void LJ_PME_Direct_Force_With_Atom_Energy_And_Virial(
         int atom_numbers, int local_atom_numbers, int solvent_numbers, int ghost_numbers,
         const float* crd, const float* charge, float* frc, const Boundary boundary,
-        NEIGHBOR_LIST* nl, float pme_beta, int need_atom_energy, float* atom_energy,
+        const ATOM_GROUP* nl, float pme_beta, int need_atom_energy, float* atom_energy,
         int need_virial, float* atom_virial, float* atom_direct_pme_energy);
     // Long_Range_Correction (:287-308)
     void Long_Range_Correction(int need_pressure, float* d_virial, int need_potential,
                                float* d_potential, float volume);
     // Step_Print (:386-400): LJ_short, LJ_long
     void Step_Print(CONTROLLER* controller);
+    void Clear();
+};
+
+// solvent_LJ (solvent_LJ.h:9-43).  MD_INFORMATION's update groups and masses
+// come in as host arrays: residue_numbers residues of contiguous atoms from
+// res_start[r], and the atoms' masses.
+struct SOLVENT_LENNARD_JONES {
+    sponge_solvent_lj* handle = nullptr;
+    int is_initialized = 0;
+    int solvent_numbers = 0;
+    int solvent_start = 0;  // first residue of the water tail
+    int water_points = 0;
+    int local_solvent_numbers = 0;  // one domain: solvent_numbers
+    LENNARD_JONES_INFORMATION* lj_info = nullptr;
+    void* stream = nullptr;
+
+    // Initial (solvent_LJ.cpp:149-248): the mdin key solvent_LJ, else on when
+    // default_enable and there are at least 10 residues (main.cpp:1303-1307
+    // passes mode >= NVT); lj_soft is accepted and unused (no soft-core tail)
+    void Initial(CONTROLLER* controller, LENNARD_JONES_INFORMATION* lj, void* lj_soft,
+                 int atom_numbers, int residue_numbers, const int* res_start, const float* mass,
+                 bool default_enable, void* stream = nullptr);
+    // LJ_PME_Direct_Force_With_Atom_Energy_And_Virial (solvent_LJ.cpp:256-432);
+    // d_res_start is accepted for the reference's signature (the residues of
+    // Initial are used); nl is the list's d_nl
+    void LJ_PME_Direct_Force_With_Atom_Energy_And_Virial(
+        int atom_numbers, int residue_numbers, const int* d_res_start, const float* crd,
+        const float* charge, float* frc, const Boundary boundary, const ATOM_GROUP* nl,
+        float pme_beta, int need_atom_energy, float* atom_energy, int need_virial,
+        float* atom_lj_virial, float* atom_direct_pme_energy);
     void Clear();
 };
 
@@ -138,6 +194,12 @@ struct Particle_Mesh {
                                                      int need_virial, int need_energy,
                                                      float* d_virial, float* d_potential,
                                                      int step);
+    // Update_Box (PM_force.cpp:1254-1266): BC / virial tables and the
+    // neutralizing factor for the new box; the domain corners (one domain:
+    // the origin and the box diagonal) moved by dt * (r g) as the
+    // reference moves them (:1244-1252)
+    void Update_Box(const Boundary boundary, LTMatrix3 g, float dt);
+    float min_corner[3] = {0, 0, 0}, max_corner[3] = {0, 0, 0};
     // MPI_PME_Excluded_Force_With_Atom_Energy (:1819-1842)
     void MPI_PME_Excluded_Force_With_Atom_Energy(int local_atom_numbers, const int* atom_local,
                                                  const int* atom_local_id, const float* crd,

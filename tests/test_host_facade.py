@@ -63,3 +63,19 @@ def test_facade_reads_mdin_keys(tmp_path):
                     f"-Wl,-rpath,{os.path.dirname(lib)}"], check=True)
     out = subprocess.run([str(exe)], capture_output=True, text=True)
     assert out.returncode == 0 and out.stdout.strip() == "ok", out.stdout + out.stderr
+
+
+def test_facade_force_loop_program_builds(tmp_path):
+    """The GPU facade test's program (tests/facade_cfg1.cpp: the reference's
+    force loop through NEIGHBOR_LIST / LENNARD_JONES_INFORMATION /
+    SOLVENT_LENNARD_JONES / Particle_Mesh with neighbor_list.d_nl) compiles
+    and links against the library; it runs in tests/test_gpu_facade.py."""
+    lib = os.path.join(ROOT, "sponge_amd", "libsponge_amd.so")
+    if not os.path.exists(lib):
+        pytest.skip("library not built")
+    exe = tmp_path / "facade_cfg1"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-D__HIP_PLATFORM_AMD__", "-I", ROOT,
+                    "-I", "/opt/rocm/include", os.path.join(ROOT, "tests", "facade_cfg1.cpp"),
+                    lib, "-L/opt/rocm/lib", "-lamdhip64", f"-Wl,-rpath,{os.path.dirname(lib)}",
+                    "-o", str(exe)], check=True)
+    assert exe.exists()
