@@ -83,6 +83,13 @@ __device__ __forceinline__ float3 disp_pbc(float3 a, float3 b, const LT6& c, con
     return o;
 }
 
+// The periodic image of x nearest the point C (orthorhombic box).
+__device__ __forceinline__ float3 near_image(float3 x, float3 C, const LT6& c, const LT6& r) {
+    return make_float3(x.x - c.a11 * rintf((x.x - C.x) * r.a11),
+                       x.y - c.a22 * rintf((x.y - C.y) * r.a22),
+                       x.z - c.a33 * rintf((x.z - C.z) * r.a33));
+}
+
 // r^2 as the reference's VECTOR dot product ((x*x + y*y) + z*z), no FMA.
 __device__ __forceinline__ float norm2_exact(float3 d) {
 #pragma clang fp contract(off)
@@ -101,6 +108,12 @@ template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, kWave));
     return v;
 }
 

@@ -41,7 +41,6 @@ struct sponge_nl {
     int* d_nranges = nullptr;     // [G]
     int max_ranges = 0;
     int cap = 0;                  // row capacity (both directions)
-    int stage = 0;                // candidate-stream entries staged in LDS by the search
     int* d_list_count = nullptr;  // [N] per sorted atom
     unsigned short* d_list = nullptr;  // [N][cap] positions in the row cell's candidate stream
     int2* d_sexw = nullptr;       // [N] combined exclusion window of each sorted atom

@@ -20,6 +20,7 @@
 //      of by a separate swap-with-last pass.
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "internal.h"
@@ -212,23 +213,6 @@ __global__ void k_nl_exwin(int N, const int* __restrict__ ex_start, const int* _
     }
 }
 
-// The reference's exclusion rule for the unordered pair {a, b}, applied in the
-// half-list row of the lower id lo (Delete_Excluded_Atoms_Serial_In_Neighbor_
-// List, neighbor_list.cpp:736-807): the higher id must lie inside the window
-// [list[first], list[last]] of lo's exclusion list and be one of its entries.
-__device__ __noinline__ bool excluded_pair(int a, int b, const int* __restrict__ ex_start,
-                                           const int* __restrict__ ex_list,
-                                           const int* __restrict__ ex_num) {
-    const int lo = a < b ? a : b, hi = a < b ? b : a;
-    const int n = ex_num[lo];
-    if (n <= 0) return false;
-    const int s0 = ex_start[lo];
-    if (hi < ex_list[s0] || hi > ex_list[s0 + n - 1]) return false;
-    for (int e = s0; e < s0 + n; ++e)
-        if (ex_list[e] == hi) return true;
-    return false;
-}
-
 // Candidate stream of a cell: its stencil runs of consecutive cells, i.e.
 // contiguous ranges [first_k, first_k + len_k) of the sorted array,
 // concatenated.  Built in LDS by the block (loads in parallel, scan by one
@@ -255,15 +239,67 @@ __device__ __forceinline__ int cell_stream(int cell, const int* __restrict__ sta
 }
 
 // One workgroup (8 wavefronts) per cell.  The cell's candidate stream (sorted
-// positions with the global id in w) is staged into LDS once; each wavefront
-// then takes up to kSearchRows i atoms of the cell and sweeps the stream 64
-// candidates per pass, testing every staged candidate against all of its
-// rows (one LDS read feeds kSearchRows independent tests).  The pair test is
-// the reference's Find_Neighbors_Gridly criterion (Get_Displacement, r^2 <
-// cutoff_skin_square) and, inside the row's combined exclusion window, its
-// exclusion rule.  Rows hold both directions as 16-bit stream positions.
-constexpr int kSearchStage = 3584;  // covers the stencil streams of water at 12 A list radius
+// positions with the global id in w) is staged into LDS in chunks of
+// kSearchChunk positions; each wavefront takes up to kSearchRows i atoms of
+// the cell and sweeps every chunk, two candidates per lane per pass, testing
+// each against all of its rows (one LDS read feeds up to kSearchRows
+// independent tests).  Small chunks keep the workgroup's LDS small, so that
+// several workgroups share a CU: the search is bound by latency (dependent
+// compares, ballots and stores), which only more resident wavefronts hide.
+// The pair test is the reference's Find_Neighbors_Gridly criterion
+// (Get_Displacement, r^2 < cutoff_skin_square) and, inside the row's
+// combined exclusion window, its exclusion rule.  Rows hold both directions
+// as 16-bit stream positions, in ascending order.
+//
+// In an orthorhombic box wider than twice (list radius + half a cell) on
+// every axis, the chunk holds each candidate's image nearest the cell's
+// centre C, and the rows' images nearest C are taken too: a candidate within
+// the list radius of a row atom then has that image at its minimum-image
+// displacement, which is a plain difference (3 subtractions and 3 FMAs
+// instead of the reference's 18 operations).  That r^2 differs from the
+// reference's by at most `margin` (rounding of the shifted coordinates and of
+// the two formulas; bound below); a candidate decided by it is decided the
+// same way by the reference, and the few within the margin of the threshold
+// are tested again with the reference arithmetic on the raw coordinates, so
+// the list is the reference's set bit for bit.
+// stream positions staged at a time: 32 KiB of LDS, three workgroups per CU
+// (measured on the 95,000-atom water step: chunks of 512 / 1024 / 2048 and the
+// whole stream, 3,584 entries: 557 / 519 / 490 / 552 us per rebuild)
+constexpr int kSearchChunk = 2048;
 constexpr int kSearchRows = 4;
+constexpr int kSearchThreads = 512;  // workgroup size of the search
+
+// Bound on |r2_ref - r2_shift| for a pair within the list radius: per axis
+// the reference displacement fl(fl(a-b) - fl(s L)) and the shifted one
+// fl(a' - b'), a' = fl(a - fl(k L)), both differ from the exact minimum-image
+// displacement D by at most u (6 M + 7 L + 2 |D|) together (u = 2^-24,
+// M = the largest coordinate magnitude, |D| <= 13 near the threshold), and
+// |d|^2 vs |d'|^2 adds delta (2 sqrt(3 thr) + 3 delta) plus 6 u thr of the
+// two r^2 sums; taken x4.
+__host__ __device__ inline float search_margin(float M, float L, float thr) {
+    const float u = 5.9604645e-08f;
+    const float rt = sqrtf(thr);
+    const float delta = u * (6.f * M + 7.f * L + 2.f * rt + 2.f);
+    return 4.f * (delta * (2.f * sqrtf(3.f) * rt + 3.f * delta) + 6.f * u * thr);
+}
+
+// The reference's exclusion rule for the unordered pair {a, b}, applied in the
+// half-list row of the lower id lo (Delete_Excluded_Atoms_Serial_In_Neighbor_
+// List, neighbor_list.cpp:736-807): the higher id must lie inside the window
+// [list[first], list[last]] of lo's exclusion list and be one of its entries.
+
+__device__ __forceinline__ bool excluded_pair_inl(int a, int b, const int* __restrict__ ex_start,
+                                                  const int* __restrict__ ex_list,
+                                                  const int* __restrict__ ex_num) {
+    const int lo = a < b ? a : b, hi = a < b ? b : a;
+    const int n = ex_num[lo];
+    if (n <= 0) return false;
+    const int s0 = ex_start[lo];
+    if (hi < ex_list[s0] || hi > ex_list[s0 + n - 1]) return false;
+    bool hit = false;
+    for (int e = s0; e < s0 + n; ++e) hit |= ex_list[e] == hi;
+    return hit;
+}
 
 template <bool ORTHO>
 __device__ __forceinline__ void nl_search_cells_body(int sa_block, 
@@ -271,13 +307,14 @@ __device__ __forceinline__ void nl_search_cells_body(int sa_block,
     const int* __restrict__ start, const int2* __restrict__ ranges,
     const int* __restrict__ nranges, int max_ranges, LT6 c, LT6 r, float thr,
     const int* __restrict__ ex_start, const int* __restrict__ ex_list,
-    const int* __restrict__ ex_num, int cap, int stage, int* __restrict__ list_count,
+    const int* __restrict__ ex_num, int cap, int* __restrict__ list_count,
     unsigned short* __restrict__ list, int* __restrict__ g_rfirst, int* __restrict__ g_roff,
-    int* __restrict__ flags) {
+    int* __restrict__ flags, float gl, int Nx, int Ny, int Nz) {
     if (*need == 0) return;
-    extern __shared__ float4 cand[];
+    __shared__ float4 cand[kSearchChunk];
     __shared__ int r_first[kMaxRanges];
     __shared__ int r_off[kMaxRanges + 1];
+    __shared__ unsigned s_maxbits;
     const int cell = sa_block;
     const int nr = cell_stream(cell, start, ranges, nranges, max_ranges, r_first, r_off);
     const int total = r_off[nr];
@@ -292,16 +329,41 @@ __device__ __forceinline__ void nl_search_cells_body(int sa_block,
         if (threadIdx.x == 0) atomicOr(flags, SPONGE_AMD_NL_STREAM_OVERFLOW | SPONGE_AMD_NL_LIST_OVERFLOW);
         return;
     }
-    const int staged = total < stage ? total : stage;
-    stage_stream(nr, r_first, r_off, staged, [&](int t, int q) { cand[t] = spos[q]; });
-    __syncthreads();
+    // cell-centre images: box wide enough on every axis (uniform per block)
+    float3 C = make_float3(0.f, 0.f, 0.f);
+    bool shift = false;
+    if (ORTHO) {
+        const int ix = cell / (Ny * Nz), iy = (cell / Nz) % Ny, iz = cell % Nz;
+        const float lx = ix * gl, hx = ix == Nx - 1 ? c.a11 : (ix + 1) * gl;
+        const float ly = iy * gl, hy = iy == Ny - 1 ? c.a22 : (iy + 1) * gl;
+        const float lz = iz * gl, hz = iz == Nz - 1 ? c.a33 : (iz + 1) * gl;
+        C = make_float3(0.5f * (lx + hx), 0.5f * (ly + hy), 0.5f * (lz + hz));
+        const float reach = sqrtf(thr) + 0.01f;
+        shift = 0.5f * c.a11 > reach + 0.5f * (hx - lx) && 0.5f * c.a22 > reach + 0.5f * (hy - ly) &&
+                0.5f * c.a33 > reach + 0.5f * (hz - lz);
+    }
+    const float Lmax = fmaxf(c.a11, fmaxf(c.a22, c.a33));
+    // the largest coordinate magnitude of the cell's atoms (part of the
+    // margin of every chunk)
+    float mrow = 0.f;
+    if (shift) {
+        for (int p = a0 + (int)threadIdx.x; p < a1; p += blockDim.x) {
+            const float4 v = spos[p];
+            mrow = fmaxf(mrow, fmaxf(fabsf(v.x), fmaxf(fabsf(v.y), fabsf(v.z))));
+        }
+    }
+    // running maximum of the coordinate magnitudes staged so far (only
+    // grows, so each chunk's margin covers the chunk)
+    if (threadIdx.x == 0) s_maxbits = 0u;
     const int lane = threadIdx.x & (kWave - 1);
     // the wavefront index as a scalar: row bases and counters stay in SGPRs
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int nwave = blockDim.x / kWave;
     const int rows = a1 - a0;
-    for (int k0 = 0; wave + k0 * nwave < rows; k0 += kSearchRows) {
-        float3 pi[kSearchRows];
+    // every wavefront runs the same row-block and chunk loops (the barriers
+    // of the staging); a wavefront without rows in a block only stages
+    for (int k0 = 0; k0 * nwave < rows; k0 += kSearchRows) {
+        float3 pi[kSearchRows], pc[kSearchRows];
         int gi[kSearchRows], cnt[kSearchRows];
         int2 wi[kSearchRows];
         bool act[kSearchRows];
@@ -312,48 +374,138 @@ __device__ __forceinline__ void nl_search_cells_body(int sa_block,
             const int p = a0 + (act[u] ? row : 0);
             const float4 v = spos[p];
             pi[u] = make_float3(v.x, v.y, v.z);
+            pc[u] = shift ? near_image(pi[u], C, c, r) : pi[u];
             gi[u] = __float_as_int(v.w);
             wi[u] = sexw[p];
             cnt[u] = 0;
         }
-        // one pass of 64 candidates (stream positions t, this lane's at pj4)
-        // against the wavefront's rows
-        auto pass = [&](int t, bool in, float4 pj4) {
-            const float3 pj = make_float3(pj4.x, pj4.y, pj4.z);
-            const int gj = __float_as_int(pj4.w);
-#pragma unroll
-            for (int u = 0; u < kSearchRows; ++u) {
-                if (!act[u]) continue;
-                // Get_Displacement(crd_i, crd_j) as in Find_Neighbors_Gridly
-                // evaluated for every lane (bitwise, no branch per test)
-                const float3 d = disp_pbc<ORTHO>(pi[u], pj, c, r);
-                bool ok = in & (gj != gi[u]) & (norm2_exact(d) < thr);
-                if (ok && gj >= wi[u].x && gj <= wi[u].y)
-                    ok = !excluded_pair(gi[u], gj, ex_start, ex_list, ex_num);
-                const unsigned long long m = __ballot(ok);
-                if (ok) {
-                    const int pos = cnt[u] + lanes_below(m);
-                    if (pos < cap) list[(size_t)(a0 + wave + (k0 + u) * nwave) * cap + pos] = (unsigned short)t;
+        const int na = act[3] ? 4 : act[2] ? 3 : act[1] ? 2 : act[0] ? 1 : 0;
+        for (int c0 = 0; c0 < total; c0 += kSearchChunk) {
+            const int clen = min(kSearchChunk, total - c0);
+            __syncthreads();  // the previous chunk is no longer read
+            // stage stream positions [c0, c0 + clen): one wavefront per run
+            float mx = mrow;
+            for (int k = wave; k < nr; k += nwave) {
+                const int lo = max(r_off[k], c0), hi = min(r_off[k + 1], c0 + clen);
+                for (int t = lo + lane; t < hi; t += kWave) {
+                    const float4 v = spos[r_first[k] + (t - r_off[k])];
+                    if (shift) {
+                        mx = fmaxf(mx, fmaxf(fabsf(v.x), fmaxf(fabsf(v.y), fabsf(v.z))));
+                        const float3 w = near_image(make_float3(v.x, v.y, v.z), C, c, r);
+                        cand[t - c0] = make_float4(w.x, w.y, w.z, v.w);
+                    } else {
+                        cand[t - c0] = v;
+                    }
                 }
-                cnt[u] += __popcll(m);
             }
-        };
-        // staged candidates: LDS only, so no pass waits on the list stores
-        // in flight (the vector memory counter also counts stores)
-        for (int base = 0; base < staged; base += kWave) {
-            const int t = base + lane;
-            const bool in = t < staged;
-            float4 pj4 = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
-            if (in) pj4 = cand[t];
-            pass(t, in, pj4);
-        }
-        // the rest of a stream longer than the stage, from global memory
-        for (int base = staged / kWave * kWave; base < total; base += kWave) {
-            const int t = base + lane;
-            const bool in = t >= staged && t < total;
-            float4 pj4 = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
-            if (in) pj4 = spos[stream_to_sorted(t, nr, r_first, r_off)];
-            pass(t, in, pj4);
+            if (shift) {
+                mx = wave_max(mx);
+                if (lane == 0) atomicMax(&s_maxbits, __float_as_uint(mx));
+            }
+            __syncthreads();
+            float thr_lo = thr, thr_hi = thr;
+            if (shift) {
+                const float m = search_margin(__uint_as_float(s_maxbits), Lmax, thr);
+                thr_lo = thr - m;
+                thr_hi = thr + m;
+            }
+            // the reference test on raw coordinates (stream position t)
+            auto exact = [&](int u, int t, float3 pj) {
+                if (shift) {
+                    const float4 raw = spos[stream_to_sorted(t, nr, r_first, r_off)];
+                    pj = make_float3(raw.x, raw.y, raw.z);
+                }
+                return norm2_exact(disp_pbc<ORTHO>(pi[u], pj, c, r)) < thr;
+            };
+            // Two candidates per lane per pass (stream positions t and
+            // t + 64): the two halves' tests are independent.  The sweep is
+            // compiled per count of active rows (uniform per wavefront), so
+            // empty row slots cost nothing.
+            auto sweep = [&](auto na_tag, auto shift_tag) {
+                constexpr int NA = decltype(na_tag)::value;
+                constexpr bool SH = decltype(shift_tag)::value;
+                constexpr int H = 2;
+                unsigned short* rowp[NA];
+#pragma unroll
+                for (int u = 0; u < NA; ++u) rowp[u] = list + (size_t)(a0 + wave + (k0 + u) * nwave) * cap;
+                for (int base = 0; base < clen; base += H * kWave) {
+                    float3 pj[H];
+                    int gj[H], tt[H];
+                    bool in[H];
+#pragma unroll
+                    for (int h = 0; h < H; ++h) {
+                        const int loc = base + h * kWave + lane;
+                        in[h] = loc < clen;
+                        tt[h] = c0 + loc;
+                        const float4 v = cand[in[h] ? loc : 0];
+                        pj[h] = make_float3(v.x, v.y, v.z);
+                        gj[h] = in[h] ? __float_as_int(v.w) : -1;
+                    }
+                    bool ok[H][NA], amb[H][NA], chk[H][NA];
+                    bool rare = false;
+#pragma unroll
+                    for (int h = 0; h < H; ++h) {
+#pragma unroll
+                        for (int u = 0; u < NA; ++u) {
+                            const bool cand_ok = in[h] & (gj[h] != gi[u]);
+                            if (SH) {
+                                const float dx = pj[h].x - pc[u].x, dy = pj[h].y - pc[u].y,
+                                            dz = pj[h].z - pc[u].z;
+                                const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                                ok[h][u] = cand_ok & (r2 < thr_lo);
+                                amb[h][u] = cand_ok & (r2 >= thr_lo) & (r2 < thr_hi);
+                            } else {
+                                // Get_Displacement(crd_i, crd_j) as in Find_Neighbors_Gridly
+                                ok[h][u] = cand_ok & (norm2_exact(disp_pbc<ORTHO>(pi[u], pj[h], c, r)) < thr);
+                                amb[h][u] = false;
+                            }
+                            chk[h][u] = (ok[h][u] | amb[h][u]) & (gj[h] >= wi[u].x) & (gj[h] <= wi[u].y);
+                            rare |= amb[h][u] | chk[h][u];
+                        }
+                    }
+                    if (__ballot(rare)) {
+#pragma unroll
+                        for (int h = 0; h < H; ++h) {
+#pragma unroll
+                            for (int u = 0; u < NA; ++u) {
+                                if (SH && amb[h][u]) ok[h][u] = exact(u, tt[h], pj[h]);
+                                if (chk[h][u] && ok[h][u])
+                                    ok[h][u] = !excluded_pair_inl(gi[u], gj[h], ex_start, ex_list, ex_num);
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < NA; ++u) {
+#pragma unroll
+                        for (int h = 0; h < H; ++h) {
+                            const unsigned long long m = __ballot(ok[h][u]);
+                            if (ok[h][u]) {
+                                const int pos = cnt[u] + lanes_below(m);
+                                if (pos < cap) rowp[u][pos] = (unsigned short)tt[h];
+                            }
+                            cnt[u] += __popcll(m);
+                        }
+                    }
+                }
+            };
+            static_assert(kSearchRows == 4, "the sweep dispatch covers 1-4 rows");
+            using I1 = std::integral_constant<int, 1>;
+            using I2 = std::integral_constant<int, 2>;
+            using I3 = std::integral_constant<int, 3>;
+            using I4 = std::integral_constant<int, 4>;
+            if (shift) {
+                using S = std::true_type;
+                if (na == 4) sweep(I4{}, S{});
+                else if (na == 3) sweep(I3{}, S{});
+                else if (na == 2) sweep(I2{}, S{});
+                else if (na == 1) sweep(I1{}, S{});
+            } else {
+                using S = std::false_type;
+                if (na == 4) sweep(I4{}, S{});
+                else if (na == 3) sweep(I3{}, S{});
+                else if (na == 2) sweep(I2{}, S{});
+                else if (na == 1) sweep(I1{}, S{});
+            }
         }
         if (lane == 0) {
 #pragma unroll
@@ -368,15 +520,15 @@ __device__ __forceinline__ void nl_search_cells_body(int sa_block,
 }
 
 template <bool ORTHO>
-__global__ void __launch_bounds__(512) k_nl_search_cells(
+__global__ void __launch_bounds__(kSearchThreads) k_nl_search_cells(
     const int* __restrict__ need, const float4* __restrict__ spos, const int2* __restrict__ sexw,
     const int* __restrict__ start, const int2* __restrict__ ranges,
     const int* __restrict__ nranges, int max_ranges, LT6 c, LT6 r, float thr,
     const int* __restrict__ ex_start, const int* __restrict__ ex_list,
-    const int* __restrict__ ex_num, int cap, int stage, int* __restrict__ list_count,
+    const int* __restrict__ ex_num, int cap, int* __restrict__ list_count,
     unsigned short* __restrict__ list, int* __restrict__ g_rfirst, int* __restrict__ g_roff,
-    int* __restrict__ flags) {
-    nl_search_cells_body<ORTHO>(blockIdx.x, need, spos, sexw, start, ranges, nranges, max_ranges, c, r, thr, ex_start, ex_list, ex_num, cap, stage, list_count, list, g_rfirst, g_roff, flags);
+    int* __restrict__ flags, float gl, int Nx, int Ny, int Nz) {
+    nl_search_cells_body<ORTHO>(blockIdx.x, need, spos, sexw, start, ranges, nranges, max_ranges, c, r, thr, ex_start, ex_list, ex_num, cap, list_count, list, g_rfirst, g_roff, flags, gl, Nx, Ny, Nz);
 }
 
 
@@ -573,7 +725,6 @@ extern "C" int sponge_nl_initial(sponge_nl** out, int atom_numbers, float cutoff
     const int N = nl->N, G = nl->G;
     // rows hold both directions: twice the reference's per-atom half capacity
     nl->cap = 2 * nl->opt.max_neighbor_numbers;
-    nl->stage = stage_entries(sizeof(float4), kSearchStage);
     if (nl->max_ranges > kMaxRanges) {
         delete nl;
         return SPONGE_AMD_ERROR_OVERFLOW;
@@ -707,19 +858,20 @@ int sa::nl_update_impl(sponge_nl* nl, const int* atom_local, int local_atom_numb
     SA_LAUNCH_CHECK();
     {
         ScopedTimer timer("nl_search", s);
-        const size_t sh = sizeof(float4) * nl->stage;
         if (nl->ortho)
-            SA_LAUNCH(512, (&nl_search_cells_body<true>), (k_nl_search_cells<true>), dim3(nl->G), dim3(512), sh, s, nl->d_need,
+            SA_LAUNCH(kSearchThreads, (&nl_search_cells_body<true>), (k_nl_search_cells<true>), dim3(nl->G), dim3(kSearchThreads), 0, s, nl->d_need,
                                nl->d_spos, nl->d_sexw, nl->d_cell_start, nl->d_ranges,
                                nl->d_nranges, nl->max_ranges, c, r, nl->thr, excluded_list_start,
-                               excluded_list, excluded_numbers, nl->cap, nl->stage, nl->d_list_count,
-                               nl->d_list, nl->d_rfirst, nl->d_roff, nl->d_flags);
+                               excluded_list, excluded_numbers, nl->cap, nl->d_list_count,
+                               nl->d_list, nl->d_rfirst, nl->d_roff, nl->d_flags,
+                               nl->grid_length, nl->Nx, nl->Ny, nl->Nz);
         else
-            SA_LAUNCH(512, (&nl_search_cells_body<false>), (k_nl_search_cells<false>), dim3(nl->G), dim3(512), sh, s, nl->d_need,
+            SA_LAUNCH(kSearchThreads, (&nl_search_cells_body<false>), (k_nl_search_cells<false>), dim3(nl->G), dim3(kSearchThreads), 0, s, nl->d_need,
                                nl->d_spos, nl->d_sexw, nl->d_cell_start, nl->d_ranges,
                                nl->d_nranges, nl->max_ranges, c, r, nl->thr, excluded_list_start,
-                               excluded_list, excluded_numbers, nl->cap, nl->stage, nl->d_list_count,
-                               nl->d_list, nl->d_rfirst, nl->d_roff, nl->d_flags);
+                               excluded_list, excluded_numbers, nl->cap, nl->d_list_count,
+                               nl->d_list, nl->d_rfirst, nl->d_roff, nl->d_flags,
+                               nl->grid_length, nl->Nx, nl->Ny, nl->Nz);
         SA_LAUNCH_CHECK();
     }
     nl->built = 1;

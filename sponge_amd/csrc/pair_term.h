@@ -19,13 +19,6 @@ constexpr float kTwoOverSqrtPi = 1.1283791670218446f;  // Lennard_Jones_force.h:
 constexpr int kPairStage = 3584;
 constexpr int kMaxLdsTypes = 64;  // the square (A, B) table lives in LDS: up to 64 LJ types
 
-// The periodic image of x nearest the point C (orthorhombic box).
-__device__ __forceinline__ float3 near_image(float3 x, float3 C, const LT6& c, const LT6& r) {
-    return make_float3(x.x - c.a11 * rintf((x.x - C.x) * r.a11),
-                       x.y - c.a22 * rintf((x.y - C.y) * r.a22),
-                       x.z - c.a33 * rintf((x.z - C.z) * r.a33));
-}
-
 // Partner data of a row entry: from the block's LDS stage (every position of
 // the stream staged) or, for the rare stream longer than the stage, from the
 // sorted arrays in HBM.  The two sources are separate code paths chosen once
