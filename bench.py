@@ -332,6 +332,66 @@ def batched_form(torch, abi, S, steps=500, steps_per_launch=10, n_boxes=48):
     return out
 
 
+def lambda_windows_form(torch, abi, S, steps=500, steps_per_launch=10):
+    """The lambda-window alternative of the batched form: 16 soft-core windows
+    (alpha 0.5, p 1, sigma 3 A) decoupling one 24-atom LJ / charge solute from
+    2000 waters (charges first, then LJ), each an NVE system of the step
+    driver in soft-core mode, all
+    advanced by one batched graph launch per steps_per_launch steps, against
+    the same windows run one after another (each replaying its own graph)."""
+    systems, lambdas, st = S.lambda_windows()
+    ms, scs = [], []
+    for s, lam in zip(systems, lambdas):
+        m = abi.MDSystem(s)
+        sc = abi.LJSoftCore(s.n_atoms, st.AA, st.AB, st.BA, st.BB, st.typeA, st.typeB, s.cutoff,
+                            alpha=st.alpha, p=st.p, sigma=st.sigma, sigma_min=0.0,
+                            **{"lambda": float(lam)})
+        m.set_soft_core(sc)
+        m.equilibrate(100, temperature=300.0, every=10)
+        ms.append(m)
+        scs.append(sc)
+    b = abi.Batch(ms)
+    b.run(1)
+    b.autotune(3)
+    b.run(2 * steps_per_launch, steps_per_launch)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.run(steps, steps_per_launch)
+    b.synchronize()
+    t_batch = time.perf_counter() - t0
+    for m in ms:
+        m.run(4, use_graph=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for m in ms:
+        m.run(steps, use_graph=True)
+    for m in ms:
+        m.energies()
+    t_seq = time.perf_counter() - t0
+    # NVE energies of every window after the runs (finite, per window)
+    pots = []
+    for m in ms:
+        m.run(1, energy_interval=1)
+        pots.append(m.energies()["potential"])
+    ms_step = t_batch * 1e3 / steps
+    dt_ps = systems[0].dt_ps
+    out = dict(workload=f"{len(systems)} soft-core lambda windows (alpha 0.5, p 1, sigma 3 A), "
+                        f"one 24-atom LJ / charge solute in {systems[0].n_waters} waters "
+                        f"({systems[0].n_atoms} atoms) decoupled in two stages (charges over "
+                        f"the first 8 windows, LJ over the last 8), cutoff "
+                        f"{systems[0].cutoff:g} A, PME ~1 A, dt 2 fs, SETTLE",
+               systems=len(systems), atoms_per_window=systems[0].n_atoms, steps=steps,
+               steps_per_launch=steps_per_launch, ms_per_step_all_windows=ms_step,
+               ns_day_per_window=86.4 * dt_ps / (ms_step * 1e-3),
+               ms_per_step_sequential=t_seq * 1e3 / steps,
+               speedup_vs_sequential=t_seq / t_batch,
+               potentials_finite=bool(all(np.isfinite(p) for p in pots)))
+    b.clear()
+    for m in ms:
+        m.clear()
+    return out
+
+
 def plain_path(s, torch, abi, lib, steps, warmup, breakdown_steps, eager=False):
     """The plain MD step of one system: fixed equilibration of the seeded
     lattice (outside every timed region), FFT tiling autotune, W warmup
@@ -513,6 +573,7 @@ def main():
         del r2
     if not args.no_batched:
         out["batched"] = batched_form(torch, abi, S)
+        out["lambda_windows"] = lambda_windows_form(torch, abi, S)
     if not args.no_cpu_baseline:
         import dataclasses
         s_now = dataclasses.replace(s, crd=crd)

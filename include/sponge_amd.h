@@ -601,6 +601,17 @@ int sponge_md_autotune(sponge_md* md, int reps, int* tiles);
 #define SPONGE_AMD_MD_OVERLAP_MIN_ATOMS 12000
 int sponge_md_set_reciprocal_overlap(sponge_md* md, int mode);
 
+/* Soft-core mode of a lambda window: the step's LJ + PME direct term is the
+ * soft-core module sc (LJ_Soft_Core_PME_Direct_Force_With_Atom_Energy_And_
+ * Virial, SPONGE/main.cpp:1460-1465, over the driver's list) instead of the
+ * plain LJ module, and energy steps take its Long_Range_Correction
+ * (:1479-1482) and LJ_soft_short; the reference initialises one of the two
+ * modules (main.cpp:1179-1180, the one whose input exists).  The charges of
+ * the system are the window's (lambda-mixed) charges.  sc must have the
+ * system's atom count; it stays owned by the caller; NULL returns to the plain
+ * module.  The batched driver runs soft-core systems too. */
+int sponge_md_set_soft_core(sponge_md* md, sponge_ljsc* sc);
+
 /* Learned multiple-time-step schedule (no counterpart in the reference; the
  * reference's own way to skip reciprocal steps is PME update_interval,
  * PM_force.cpp:1082, 1135-1140): with rsur set, the steps with step % period

@@ -110,6 +110,7 @@ SIGNATURES = {
     "sponge_md_run": (_I, [_P, _I, _I, _I]),
     "sponge_md_autotune": (_I, [_P, _I, C.POINTER(_I)]),
     "sponge_md_set_reciprocal_overlap": (_I, [_P, _I]),
+    "sponge_md_set_soft_core": (_I, [_P, _P]),
     "sponge_md_set_surrogate": (_I, [_P, _P, _I, _I, _F]),
     "sponge_md_surrogate_stats": (_I, [_P, C.POINTER(C.c_longlong), C.POINTER(C.c_double)]),
     "sponge_md_collect_frames": (_I, [_P, _P, _I, _I]),
@@ -734,6 +735,14 @@ class MDSystem:
         beside the pair work; 0: one stream; -1: from the atom count."""
         _check(self.lib.sponge_md_set_reciprocal_overlap(self.h, int(mode)),
                "sponge_md_set_reciprocal_overlap")
+
+    def set_soft_core(self, sc):
+        """Lambda window: the step's LJ + PME direct term is the soft-core
+        module sc (an LJSoftCore over this system's atoms; the system's
+        charges are the window's); None returns to the plain LJ module."""
+        _check(self.lib.sponge_md_set_soft_core(self.h, sc.h if sc is not None else None),
+               "sponge_md_set_soft_core")
+        self._sc = sc  # the module must outlive its use by the driver
 
     def set_surrogate(self, rsur, period=4, check_interval=8, bound=0.01):
         _check(self.lib.sponge_md_set_surrogate(self.h, rsur.h if rsur is not None else None,

@@ -198,6 +198,15 @@ void pme_set_stream(sponge_pme* p, hipStream_t s);
 int pme_spread_waves(const sponge_pme* p);
 void pme_set_spread_waves(sponge_pme* p, int waves);
 int rsur_atoms(const sponge_rsur* h);
+// soft-core module driven by the MD step (step-driver form of the direct
+// force: frc assigned, the list's rebuild flag cleared)
+int ljsc_direct_force_md(sponge_ljsc* h, sponge_nl* nl, const float* crd, const float* charge,
+                         float* frc, const float* cell, const float* rcell, float pme_beta,
+                         int need_atom_energy, float* atom_energy, float* atom_direct_pme_energy,
+                         int* reset_flag);
+void ljsc_set_stream(sponge_ljsc* h, hipStream_t s);
+int ljsc_atoms(const sponge_ljsc* h);
+float ljsc_long_range_factor(const sponge_ljsc* h);
 float rsur_cutoff(const sponge_rsur* h);
 
 // MD-driver hooks of the batched driver

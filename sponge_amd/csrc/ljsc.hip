@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "internal.h"
+#include "batch_launch.h"
 #include "timing.h"
 
 using namespace sa;
@@ -49,18 +50,27 @@ namespace {
 constexpr int kScStage = 3584;
 constexpr float kTwoOverSqrtPiF = 1.1283791670955126f;
 
-__global__ void k_ljsc_pack(int N, const int* __restrict__ sidx, const float4* __restrict__ spos,
+__device__ __forceinline__ void ljsc_pack_body(int sa_block, int N, const int* __restrict__ sidx,
+                            const float4* __restrict__ spos,
                             const float* __restrict__ crd, const float* __restrict__ charge,
                             const float* __restrict__ qba, const int* __restrict__ tA,
                             const int* __restrict__ tB, float4* __restrict__ pos, int* __restrict__ tp,
                             float* __restrict__ pqba, int* __restrict__ gid) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = sa_block * blockDim.x + threadIdx.x;
     if (p >= N) return;
     const int i = sidx[p];
     pos[p] = make_float4(crd[3 * i], crd[3 * i + 1], crd[3 * i + 2], charge[i]);
     tp[p] = tA[i] | (tB[i] << 16);
     pqba[p] = qba ? qba[i] : 0.f;
     gid[p] = __float_as_int(spos[p].w);
+}
+
+__global__ void k_ljsc_pack(int N, const int* __restrict__ sidx, const float4* __restrict__ spos,
+                            const float* __restrict__ crd, const float* __restrict__ charge,
+                            const float* __restrict__ qba, const int* __restrict__ tA,
+                            const int* __restrict__ tB, float4* __restrict__ pos, int* __restrict__ tp,
+                            float* __restrict__ pqba, int* __restrict__ gid) {
+    ljsc_pack_body(blockIdx.x, N, sidx, spos, crd, charge, qba, tA, tB, pos, tp, pqba, gid);
 }
 
 struct ScParams {
@@ -147,7 +157,7 @@ __device__ __forceinline__ ScTerm sc_pair(float r2, float qi, float qj, float qb
 }
 
 template <bool ORTHO, bool F, bool E, bool V, bool DU>
-__global__ void __launch_bounds__(512) k_ljsc_cells(
+__device__ __forceinline__ void ljsc_cells_body(int sa_block,
     const int* __restrict__ start, const int* __restrict__ nranges, int max_ranges,
     const int* __restrict__ g_rfirst, const int* __restrict__ g_roff,
     const int* __restrict__ list_count, const unsigned short* __restrict__ list, int cap,
@@ -164,7 +174,7 @@ __global__ void __launch_bounds__(512) k_ljsc_cells(
     float2* tb = ta + TA * TA;
     __shared__ int r_first[kMaxRanges];
     __shared__ int r_off[kMaxRanges + 1];
-    const int cell = blockIdx.x;
+    const int cell = sa_block;
     const int a0 = start[cell], a1 = start[cell + 1];
     if (a0 == a1) return;
     const int nr = nranges[cell];
@@ -269,22 +279,49 @@ __global__ void __launch_bounds__(512) k_ljsc_cells(
     }
 }
 
+template <bool ORTHO, bool F, bool E, bool V, bool DU>
+__global__ void __launch_bounds__(512) k_ljsc_cells(
+    const int* __restrict__ start, const int* __restrict__ nranges, int max_ranges,
+    const int* __restrict__ g_rfirst, const int* __restrict__ g_roff,
+    const int* __restrict__ list_count, const unsigned short* __restrict__ list, int cap,
+    const float4* __restrict__ pos, const int* __restrict__ ptp, const float* __restrict__ pqba,
+    const int* __restrict__ pgid, const float2* __restrict__ ABa, int TA,
+    const float2* __restrict__ ABb, int TB, LT6 c, LT6 r, float rc2_thr, ScParams sp, int stage,
+    float4* __restrict__ facc, float2* __restrict__ eacc, float* __restrict__ vacc,
+    double* __restrict__ du) {
+    ljsc_cells_body<ORTHO, F, E, V, DU>(blockIdx.x, start, nranges, max_ranges, g_rfirst, g_roff,
+                                         list_count, list, cap, pos, ptp, pqba, pgid, ABa, TA, ABb,
+                                         TB, c, r, rc2_thr, sp, stage, facc, eacc, vacc, du);
+}
+
 // Scatter of the rows: frc +=; atom_energy += LJ + Coulomb; the module's LJ
 // energy and the direct-Coulomb energy overwritten (the reference clears
 // them first, LJ_soft_core.cpp:776-785); atom_virial +=.
+// assign: frc = instead of += (the step driver's first force term);
+// reset_flag (nullable) is cleared by the first thread (the step driver's
+// list-rebuild flag, once the rows of this step have been read).
 template <bool E, bool V>
-__global__ void k_ljsc_finalize(int N, const int* __restrict__ sidx, const float4* __restrict__ facc,
+__device__ __forceinline__ void ljsc_finalize_body(int sa_block, int N, const int* __restrict__ sidx,
+                                const float4* __restrict__ facc,
                                 const float2* __restrict__ eacc, const float* __restrict__ vacc,
                                 float* __restrict__ frc, float* __restrict__ atom_energy,
                                 float* __restrict__ lj_energy, float* __restrict__ direct_energy,
-                                float* __restrict__ atom_virial) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+                                float* __restrict__ atom_virial, int assign,
+                                int* __restrict__ reset_flag) {
+    const int p = sa_block * blockDim.x + threadIdx.x;
+    if (p == 0 && reset_flag) *reset_flag = 0;
     if (p >= N) return;
     const int i = sidx[p];
     const float4 f = facc[p];
-    frc[3 * i] += f.x;
-    frc[3 * i + 1] += f.y;
-    frc[3 * i + 2] += f.z;
+    if (assign) {
+        frc[3 * i] = f.x;
+        frc[3 * i + 1] = f.y;
+        frc[3 * i + 2] = f.z;
+    } else {
+        frc[3 * i] += f.x;
+        frc[3 * i + 1] += f.y;
+        frc[3 * i + 2] += f.z;
+    }
     if (E) {
         const float2 e = eacc[p];
         if (atom_energy) atom_energy[i] += e.x + e.y;
@@ -293,6 +330,17 @@ __global__ void k_ljsc_finalize(int N, const int* __restrict__ sidx, const float
     }
     if (V && atom_virial)
         for (int k = 0; k < 6; ++k) atom_virial[6 * i + k] += vacc[6 * (size_t)p + k];
+}
+
+template <bool E, bool V>
+__global__ void k_ljsc_finalize(int N, const int* __restrict__ sidx, const float4* __restrict__ facc,
+                                const float2* __restrict__ eacc, const float* __restrict__ vacc,
+                                float* __restrict__ frc, float* __restrict__ atom_energy,
+                                float* __restrict__ lj_energy, float* __restrict__ direct_energy,
+                                float* __restrict__ atom_virial, int assign,
+                                int* __restrict__ reset_flag) {
+    ljsc_finalize_body<E, V>(blockIdx.x, N, sidx, facc, eacc, vacc, frc, atom_energy, lj_energy,
+                             direct_energy, atom_virial, assign, reset_flag);
 }
 
 __global__ void k_ljsc_add_scalar(float* p, float v) { p[0] += v; }
@@ -344,7 +392,8 @@ int launch(sponge_ljsc* h, sponge_nl* nl, LT6 c, LT6 r, float beta, bool F, bool
     const size_t sh = (size_t)h->stage * (sizeof(float4) + 3 * sizeof(int)) +
                       sizeof(float2) * ((size_t)h->TA * h->TA + (size_t)h->TB * h->TB);
 #define SA_SC(f, e, v, d)                                                                          \
-    hipLaunchKernelGGL((k_ljsc_cells<ORTHO, f, e, v, d>), dim3(nl->G), dim3(512), sh, h->stream,    \
+    SA_LAUNCH(512, (&ljsc_cells_body<ORTHO, f, e, v, d>), (k_ljsc_cells<ORTHO, f, e, v, d>),          \
+              dim3(nl->G), dim3(512), sh, h->stream,                                              \
                        nl->d_cell_start, nl->d_nranges, nl->max_ranges, nl->d_rfirst, nl->d_roff,   \
                        nl->d_list_count, nl->d_list, nl->cap, h->d_pos, h->d_tp, h->d_qba,          \
                        h->d_gid, h->d_ABa, h->TA, h->d_ABb, h->TB, c, r, h->rc2_thr, sp, h->stage, \
@@ -363,8 +412,8 @@ int launch(sponge_ljsc* h, sponge_nl* nl, LT6 c, LT6 r, float beta, bool F, bool
 int prepare(sponge_ljsc* h, sponge_nl* nl, const float* crd, const float* charge,
             const float* charge_BA) {
     if (!nl->built || nl->N != h->N) return SPONGE_AMD_ERROR_MISSING_COMMAND;
-    hipLaunchKernelGGL(k_ljsc_pack, dim3(grid_for(h->N, 256)), dim3(256), 0, h->stream, h->N,
-                       nl->d_sidx, nl->d_spos, crd, charge, charge_BA, h->d_typeA, h->d_typeB,
+    SA_LAUNCH(256, (&ljsc_pack_body), (k_ljsc_pack), dim3(grid_for(h->N, 256)), dim3(256), 0,
+              h->stream, h->N, nl->d_sidx, nl->d_spos, crd, charge, charge_BA, h->d_typeA, h->d_typeB,
                        h->d_pos, h->d_tp, h->d_qba, h->d_gid);
     SA_LAUNCH_CHECK();
     return 0;
@@ -496,9 +545,9 @@ int sponge_ljsc_direct_force(sponge_ljsc* h, sponge_nl* nl, const float* crd, co
     if (rc) return rc;
     const dim3 g(grid_for(h->N, 256)), b(256);
 #define SA_FIN(e, v)                                                                              \
-    hipLaunchKernelGGL((k_ljsc_finalize<e, v>), g, b, 0, h->stream, h->N, nl->d_sidx, h->d_facc, \
-                       h->d_eacc, h->d_vacc, frc, atom_energy, h->d_LJ_energy_atom,               \
-                       atom_direct_pme_energy, atom_virial)
+    SA_LAUNCH(256, (&ljsc_finalize_body<e, v>), (k_ljsc_finalize<e, v>), g, b, 0, h->stream, h->N, \
+              nl->d_sidx, h->d_facc, h->d_eacc, h->d_vacc, frc, atom_energy, h->d_LJ_energy_atom, \
+              atom_direct_pme_energy, atom_virial, 0, (int*)nullptr)
     if (E && V) SA_FIN(true, true);
     else if (E) SA_FIN(true, false);
     else if (V) SA_FIN(false, true);
@@ -572,3 +621,38 @@ int sponge_ljsc_clear(sponge_ljsc* h) {
 }
 
 }  // extern "C"
+
+// The step driver's form of the direct-force call: frc assigned (the step's
+// first force term), the list's rebuild flag cleared once the rows were read,
+// per-atom energies only on energy steps.
+int sa::ljsc_direct_force_md(sponge_ljsc* h, sponge_nl* nl, const float* crd, const float* charge,
+                             float* frc, const float* cell, const float* rcell, float pme_beta,
+                             int need_atom_energy, float* atom_energy,
+                             float* atom_direct_pme_energy, int* reset_flag) {
+    int rc = prepare(h, nl, crd, charge, nullptr);
+    if (rc) return rc;
+    const LT6 c = lt_from(cell), r = lt_from(rcell);
+    const bool E = need_atom_energy != 0;
+    {
+        ScopedTimer timer("ljsc_pairs", h->stream);
+        rc = lt_is_orthorhombic(c) ? launch<true>(h, nl, c, r, pme_beta, true, E, false, false)
+                                   : launch<false>(h, nl, c, r, pme_beta, true, E, false, false);
+    }
+    if (rc) return rc;
+    const dim3 g(grid_for(h->N, 256)), b(256);
+    if (E)
+        SA_LAUNCH(256, (&ljsc_finalize_body<true, false>), (k_ljsc_finalize<true, false>), g, b, 0,
+                  h->stream, h->N, nl->d_sidx, h->d_facc, h->d_eacc, h->d_vacc, frc, atom_energy,
+                  h->d_LJ_energy_atom, atom_direct_pme_energy, (float*)nullptr, 1, reset_flag);
+    else
+        SA_LAUNCH(256, (&ljsc_finalize_body<false, false>), (k_ljsc_finalize<false, false>), g, b,
+                  0, h->stream, h->N, nl->d_sidx, h->d_facc, h->d_eacc, h->d_vacc, frc,
+                  atom_energy, h->d_LJ_energy_atom, atom_direct_pme_energy, (float*)nullptr, 1,
+                  reset_flag);
+    SA_LAUNCH_CHECK();
+    return 0;
+}
+
+void sa::ljsc_set_stream(sponge_ljsc* h, hipStream_t s) { h->stream = s; }
+int sa::ljsc_atoms(const sponge_ljsc* h) { return h->N; }
+float sa::ljsc_long_range_factor(const sponge_ljsc* h) { return h->long_range_factor; }

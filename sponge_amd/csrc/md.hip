@@ -35,6 +35,7 @@ struct sponge_md {
     sponge_nl* nl = nullptr;
     sponge_lj* lj = nullptr;
     sponge_pme* pme = nullptr;
+    sponge_ljsc* sc = nullptr;  // soft-core module of a lambda window (not owned), or none
     float* d_crd = nullptr;
     float* d_vel = nullptr;
     float* d_frc = nullptr;
@@ -480,9 +481,16 @@ int md_step(sponge_md* m, long long step, int need_energy, StepKind* kind_out = 
     // clears the list's rebuild flag once the list has been used this step.
     // The terms are summed in another order than Main_Calculate_Force's, which
     // changes only the rounding of the total force.
-    rc = lj_direct_force_md(m->lj, m->nl, m->d_crd, m->d_charge, m->d_frc, cell, rcell,
-                            m->pme->beta, need_energy, m->d_atom_energy,
-                            m->pme->d_direct_atom_energy, m->nl->d_need);
+    if (m->sc) {
+        ljsc_set_stream(m->sc, s);
+        rc = ljsc_direct_force_md(m->sc, m->nl, m->d_crd, m->d_charge, m->d_frc, cell, rcell,
+                                  m->pme->beta, need_energy, m->d_atom_energy,
+                                  m->pme->d_direct_atom_energy, m->nl->d_need);
+    } else {
+        rc = lj_direct_force_md(m->lj, m->nl, m->d_crd, m->d_charge, m->d_frc, cell, rcell,
+                                m->pme->beta, need_energy, m->d_atom_energy,
+                                m->pme->d_direct_atom_energy, m->nl->d_need);
+    }
     if (rc) return rc;
     m->need_cleared = true;
     if (!ex_side) {
@@ -493,7 +501,10 @@ int md_step(sponge_md* m, long long step, int need_energy, StepKind* kind_out = 
         if (rc) return rc;
     }
     if (need_energy) {
-        rc = sponge_lj_long_range_correction(m->lj, 0, nullptr, 1, m->d_potential, m->volume);
+        rc = m->sc ? sponge_ljsc_long_range_correction(m->sc, 0, nullptr, 1, m->d_potential,
+                                                       m->volume)
+                   : sponge_lj_long_range_correction(m->lj, 0, nullptr, 1, m->d_potential,
+                                                     m->volume);
         if (rc) return rc;
     }
     const int N3 = 3 * N;
@@ -937,9 +948,19 @@ extern "C" int sponge_md_autotune(sponge_md* m, int reps, int* tiles) {
     return sponge_pme_autotune(m->pme, m->d_crd, m->d_charge, reps, tiles);
 }
 
+extern "C" int sponge_md_set_soft_core(sponge_md* m, sponge_ljsc* sc) {
+    if (!m) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    if (sc && (ljsc_atoms(sc) != m->N || m->rsur)) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    m->sc = sc;
+    capture_destroy(m->capture);  // the captured step calls the other module
+    m->capture_len = 0;
+    return 0;
+}
+
 extern "C" int sponge_md_set_surrogate(sponge_md* m, sponge_rsur* rsur, int period,
                                        int check_interval, float bound) {
     if (!m) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
+    if (rsur && m->sc) return SPONGE_AMD_ERROR_NOT_IMPLEMENTED;
     capture_destroy(m->capture);
     m->capture_len = 0;
     m->anchor_valid = false;
@@ -1036,7 +1057,13 @@ extern "C" int sponge_md_energies(sponge_md* m, double* out) {
     SA_CHECK(hipMemcpyAsync(&pot, m->d_potential, sizeof(float), hipMemcpyDeviceToHost, s));
     SA_CHECK(hipStreamSynchronize(s));
     float lj_short = 0, lj_long = 0, pm[5] = {0, 0, 0, 0, 0};
-    rc = sponge_lj_step_print(m->lj, &lj_short, &lj_long);
+    if (m->sc) {
+        ljsc_set_stream(m->sc, s);
+        rc = sponge_ljsc_step_print(m->sc, &lj_short);
+        lj_long = ljsc_long_range_factor(m->sc) / m->volume;
+    } else {
+        rc = sponge_lj_step_print(m->lj, &lj_short, &lj_long);
+    }
     if (!rc) rc = sponge_pme_step_print(m->pme, &pm[0], &pm[1], &pm[2], &pm[3], &pm[4]);
     if (rc) return rc;
     out[0] = (double)pot + ek[1];

@@ -345,3 +345,106 @@ def batched_boxes(n_boxes=48, lo=500, hi=6000, seed=20260304, cutoff=9.0):
     rng = np.random.default_rng(seed)
     sizes = np.exp(rng.uniform(math.log(lo), math.log(hi), n_boxes)).astype(int)
     return [water_box(int(n), cutoff=cutoff, seed=seed + 1 + k) for k, n in enumerate(sizes)]
+
+
+M_SOLUTE = 12.011  # amu per solute atom (free LJ / charge sites)
+
+
+@dataclasses.dataclass
+class SoftCoreState:
+    """The A / B states of a lambda-window set (LJ_SOFT_CORE's inputs,
+    SPONGE/Lennard_Jones_force/LJ_soft_core.cpp:463-560): per-atom types of
+    each state, both tables in the in-memory convention, charge_B - charge_A,
+    and the soft-core options."""
+    typeA: np.ndarray
+    typeB: np.ndarray
+    AA: np.ndarray
+    AB: np.ndarray
+    BA: np.ndarray
+    BB: np.ndarray
+    charge_A: np.ndarray
+    charge_BA: np.ndarray
+    alpha: float = 0.5
+    p: float = 1.0
+    sigma: float = 3.0
+
+    def charges(self, lam: float) -> np.ndarray:
+        """Charges at charge coupling lam: q_A + lam (q_B - q_A)."""
+        return (self.charge_A.astype(np.float64) + lam * self.charge_BA).astype(np.float32)
+
+
+def lambda_windows(n_windows=16, n_waters=2000, n_solute=24, cutoff=9.0, skin=2.0,
+                   seed=20260305, temperature=300.0):
+    """The lambda-window workload: one n_solute-atom LJ / charge solute
+    (seeded: |q| <= 0.5 e with zero net charge, sigma 2.5-4.0 A, epsilon
+    0.05-0.30 kcal/mol, Lorentz-Berthelot with the water oxygen) at the centre
+    of a box of n_waters rigid waters at 1.0 g/cm^3, decoupled over n_windows
+    windows at evenly spaced lambda in [0, 1]: the B state turns every solute
+    atom into a dummy (no LJ, no charge); soft core alpha 0.5, p 1, sigma 3 A.
+    The decoupling follows the usual two-stage path: over lambda in [0, 1/2]
+    the charges are switched off (charges q_A (1 - 2 lambda)) with the LJ
+    coupled (lambda_lj = 0), over [1/2, 1] the LJ (lambda_lj = 2 lambda - 1)
+    with the charges off.  Switching both at once lets a water hydrogen (no
+    LJ) reach a charged solute atom whose repulsion is already softened,
+    where the reference's pair term for pairs without LJ in either state,
+    plain Coulomb (LJ_soft_core.cpp:252), diverges.  Returns (systems,
+    lambda_lj, state): systems[k] carries the charges of window k and
+    lambda_lj[k] is its soft-core module's lambda_lj."""
+    rng = np.random.default_rng(seed)
+    n_sites = n_waters + n_solute
+    L = (n_sites / WATER_NUMBER_DENSITY) ** (1.0 / 3.0)
+    m = int(math.ceil(n_sites ** (1.0 / 3.0)))
+    box = np.array([L] * 3, np.float64)
+    g = np.stack(np.meshgrid(np.arange(m), np.arange(m), np.arange(m), indexing="ij"), -1)
+    sites = (g.reshape(-1, 3).astype(np.float64) + 0.5) * (L / m)
+    sites = sites[np.sort(rng.permutation(len(sites))[:n_sites])]
+    # the solute: the sites nearest the box centre
+    d = np.linalg.norm(sites - 0.5 * box, axis=1)
+    order = np.argsort(d, kind="stable")
+    solute_sites = sites[np.sort(order[:n_solute])]
+    water_sites = sites[np.sort(order[n_solute:])]
+    wpos = _place_waters(rng, water_sites, box)
+    wmass = np.array([M_O, M_H, M_H])
+    wvel = _rigid_velocities(rng, wpos, wmass, temperature)
+    nw = len(water_sites)
+    N = 3 * nw + n_solute
+    crd = np.concatenate([wpos.reshape(-1, 3), solute_sites])
+    vel = np.concatenate([wvel.reshape(-1, 3),
+                          rng.normal(size=(n_solute, 3)) * math.sqrt(KB * temperature / M_SOLUTE)])
+    mass = np.concatenate([np.tile(wmass, nw), np.full(n_solute, M_SOLUTE)])
+    vel -= (vel * mass[:, None]).sum(0) / mass.sum()
+    q = rng.uniform(-0.5, 0.5, n_solute)
+    q -= q.mean()
+    q *= min(1.0, 0.5 / np.abs(q).max())
+    qO = np.float32(Q_O * CHARGE_SCALE)
+    qH = np.float32(Q_H * CHARGE_SCALE)
+    charge_A = np.concatenate([np.tile(np.array([qO, qH, qH], np.float32), nw),
+                               (q * CHARGE_SCALE).astype(np.float32)])
+    solute_lj = [(float(rng.uniform(2.5, 4.0)), float(rng.uniform(0.05, 0.30)))
+                 for _ in range(n_solute)]
+    params = [LJ_O, LJ_H] + solute_lj
+    AA, AB = _lj_tables(params)
+    BA, BB = _lj_tables(params + [(0.0, 0.0)])  # + the dummy type
+    typeA = np.concatenate([np.tile(np.array([0, 1, 1], np.int32), nw),
+                            np.arange(2, 2 + n_solute, dtype=np.int32)])
+    typeB = typeA.copy()
+    typeB[3 * nw:] = 2 + n_solute
+    charge_BA = np.zeros(N, np.float32)
+    charge_BA[3 * nw:] = -charge_A[3 * nw:]
+    excl_num = np.concatenate([np.tile(np.array([2, 1, 0], np.int32), nw),
+                               np.zeros(n_solute, np.int32)])
+    excl_start = np.concatenate([[0], np.cumsum(excl_num)[:-1]]).astype(np.int32)
+    base = 3 * np.arange(nw, dtype=np.int32)
+    excl_list = np.stack([base + 1, base + 2, base + 2], 1).reshape(-1).astype(np.int32)
+    state = SoftCoreState(typeA=typeA, typeB=typeB, AA=AA, AB=AB, BA=BA, BB=BB,
+                          charge_A=charge_A, charge_BA=charge_BA)
+    lambdas = np.linspace(0.0, 1.0, n_windows)
+    lam_q = np.minimum(1.0, 2.0 * lambdas)
+    lam_lj = np.maximum(0.0, 2.0 * lambdas - 1.0)
+    systems = [System(name=f"lambda{k}", box=box.astype(np.float32), crd=crd.astype(np.float32),
+                      vel=vel.astype(np.float32), charge=state.charges(float(lam_q[k])),
+                      mass=mass.astype(np.float32), lj_type=typeA, lj_A=AA, lj_B=AB,
+                      excl_start=excl_start, excl_num=excl_num, excl_list=excl_list,
+                      n_waters=nw, cutoff=float(cutoff), skin=float(skin), fft=None, seed=seed)
+               for k in range(n_windows)]
+    return systems, lam_lj, state
