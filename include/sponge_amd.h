@@ -543,7 +543,8 @@ int sponge_md_initial(sponge_md** out, int atom_numbers, const float* crd, const
 /* Advances `steps` steps asynchronously.  Energies are evaluated on steps
  * with step % energy_interval == 0 (0 = never).  use_graph != 0 captures the
  * force-only step into one HIP graph on first use and replays it (default
- * schedule only: automatic list refresh, reciprocal part every step).
+ * schedule only: automatic list refresh, reciprocal part every step); while
+ * at least 8 force-only steps remain, 8 steps per graph launch.
  * After every step with (step + 1) % check_overflow_interval == 0 (the
  * neighbor-list option; <= 0 = never) the list's overflow flags are read
  * (a host synchronisation), as NEIGHBOR_LIST::Check_Overflow does

@@ -59,6 +59,11 @@ struct sponge_md {
     SegmentedCapture capture;      // the captured steps (one graph, or segments between timers)
     int capture_len = 0;           // steps in the captured cycle
     std::vector<StepKind> capture_kinds;  // reciprocal part of each captured step
+    // the same for kGraphCycles cycles in one graph (long runs: fewer graph
+    // launches, no gap between the steps of a launch)
+    SegmentedCapture capture_long;
+    int capture_long_len = 0;
+    std::vector<StepKind> capture_long_kinds;
     bool need_cleared = false;     // the pair kernel of the last step cleared the rebuild flag
     // learned multiple-time-step schedule (surrogate between PME steps)
     sponge_rsur* rsur = nullptr;
@@ -92,6 +97,20 @@ struct sponge_md {
     float prune_delta = SPONGE_AMD_MD_LIST_PRUNE_DELTA;  // inner-row buffer of the pair kernel
     int overflow_flags = 0;  // every overflow flag seen by the driver's checks
 };
+
+namespace {
+
+// cycles of the step schedule captured into one graph for long runs
+constexpr int kGraphCycles = 8;
+
+void drop_graphs(sponge_md* m) {
+    capture_destroy(m->capture);
+    m->capture_len = 0;
+    capture_destroy(m->capture_long);
+    m->capture_long_len = 0;
+}
+
+}  // namespace
 
 namespace {
 
@@ -344,7 +363,7 @@ void triangle_geometry(double dAB, double dAC, double dBC, double mA, double mB,
 }
 
 void md_free(sponge_md* m) {
-    capture_destroy(m->capture);
+    drop_graphs(m);
     sponge_nl_clear(m->nl);
     sponge_lj_clear(m->lj);
     sponge_pme_clear(m->pme);
@@ -616,8 +635,7 @@ int md_check_surrogate(sponge_md* m, long long first, long long last) {
         m->rsur = nullptr;
         m->sur_disabled = 1;
         m->anchor_valid = false;
-        capture_destroy(m->capture);
-        m->capture_len = 0;
+        drop_graphs(m);
     }
     return 0;
 }
@@ -641,8 +659,7 @@ int md_check_overflow(sponge_md* m, long long first, long long last) {
     rc = nl_reinitialize(&m->nl, flags);
     if (!rc) rc = nl_set_prune(m->nl, m->prune_delta);
     if (rc) return rc;
-    capture_destroy(m->capture);
-    m->capture_len = 0;
+    drop_graphs(m);
     m->need_cleared = false;
     m->nl_generation++;
     return 0;
@@ -806,36 +823,46 @@ extern "C" int sponge_md_run(sponge_md* m, int steps, int energy_interval, int u
         for (long long t = st; graph && energy_interval > 0 && t < st + cyc; ++t)
             if (t % energy_interval == 0) graph = false;
         if (graph) {
+            // kGraphCycles cycles per launch while the run has that many left
+            // (and no energy step among them), else one cycle per launch; the
+            // learned schedule keeps one cycle per launch, so that a failed
+            // check switches the surrogate off before the next cycle
+            bool long_run = !m->rsur && steps - k >= kGraphCycles * cyc;
+            for (long long t = st; long_run && energy_interval > 0 && t < st + kGraphCycles * cyc; ++t)
+                if (t % energy_interval == 0) long_run = false;
+            SegmentedCapture& cap = long_run ? m->capture_long : m->capture;
+            int& cap_len = long_run ? m->capture_long_len : m->capture_len;
+            std::vector<StepKind>& kinds = long_run ? m->capture_long_kinds : m->capture_kinds;
+            const int len = long_run ? kGraphCycles * cyc : cyc;
             // timers switched on or off, or another cycle, since the capture
-            if (!m->capture.items.empty() &&
-                (m->capture.timing != g_timing || m->capture_len != cyc))
-                capture_destroy(m->capture);
-            if (m->capture.items.empty()) {
-                const int rc0 = capture_begin(m->capture, m->stream);
+            if (!cap.items.empty() && (cap.timing != g_timing || cap_len != len))
+                capture_destroy(cap);
+            if (cap.items.empty()) {
+                const int rc0 = capture_begin(cap, m->stream);
                 if (rc0) return rc0;
                 int rc = 0;
-                m->capture_kinds.clear();
-                for (int c = 0; c < cyc && !rc; ++c) {
+                kinds.clear();
+                for (int c = 0; c < len && !rc; ++c) {
                     StepKind kd = StepKind::Pme;
                     rc = md_step(m, st + c, 0, &kd);
-                    m->capture_kinds.push_back(kd);
+                    kinds.push_back(kd);
                 }
-                const int rc2 = capture_end(m->capture, m->stream);
+                const int rc2 = capture_end(cap, m->stream);
                 if (rc || rc2) {
-                    capture_destroy(m->capture);
+                    capture_destroy(cap);
                     (void)hipGetLastError();
                     return rc ? rc : rc2;
                 }
-                m->capture_len = cyc;
+                cap_len = len;
             }
-            int rc = capture_launch(m->capture, m->stream);
+            int rc = capture_launch(cap, m->stream);
             if (rc) return rc;
-            for (StepKind kd : m->capture_kinds)
+            for (StepKind kd : kinds)
                 if (kd != StepKind::Pme) m->sur_steps++;
-            m->step += cyc;
-            k += cyc;
-            rc = md_check_overflow(m, st, st + cyc - 1);
-            if (!rc) rc = md_check_surrogate(m, st, st + cyc - 1);
+            m->step += len;
+            k += len;
+            rc = md_check_overflow(m, st, st + len - 1);
+            if (!rc) rc = md_check_surrogate(m, st, st + len - 1);
             if (rc) return rc;
         } else {
             const int need_e = energy_interval > 0 && st % energy_interval == 0;
@@ -920,8 +947,7 @@ extern "C" int sponge_md_set_list_prune(sponge_md* m, float delta) {
     const int rc = nl_set_prune(m->nl, delta);
     if (rc) return rc;
     m->prune_delta = delta;
-    capture_destroy(m->capture);  // the captured step reads the other rows
-    m->capture_len = 0;
+    drop_graphs(m);  // the captured step reads the other rows
     return 0;
 }
 
@@ -934,8 +960,7 @@ extern "C" int sponge_md_request_list_update(sponge_md* m) {
 extern "C" int sponge_md_set_reciprocal_overlap(sponge_md* m, int mode) {
     if (!m || mode < -1 || mode > 1) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
     m->overlap_mode = mode;
-    capture_destroy(m->capture);
-    m->capture_len = 0;
+    drop_graphs(m);
     return 0;
 }
 
@@ -943,8 +968,7 @@ extern "C" int sponge_md_set_reciprocal_overlap(sponge_md* m, int mode) {
 // (sponge_pme_autotune); the captured step is rebuilt with the new tiling.
 extern "C" int sponge_md_autotune(sponge_md* m, int reps, int* tiles) {
     if (!m || reps < 1) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
-    capture_destroy(m->capture);
-    m->capture_len = 0;
+    drop_graphs(m);
     return sponge_pme_autotune(m->pme, m->d_crd, m->d_charge, reps, tiles);
 }
 
@@ -952,8 +976,7 @@ extern "C" int sponge_md_set_soft_core(sponge_md* m, sponge_ljsc* sc) {
     if (!m) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
     if (sc && (ljsc_atoms(sc) != m->N || m->rsur)) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
     m->sc = sc;
-    capture_destroy(m->capture);  // the captured step calls the other module
-    m->capture_len = 0;
+    drop_graphs(m);  // the captured step calls the other module
     return 0;
 }
 
@@ -961,8 +984,7 @@ extern "C" int sponge_md_set_surrogate(sponge_md* m, sponge_rsur* rsur, int peri
                                        int check_interval, float bound) {
     if (!m) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
     if (rsur && m->sc) return SPONGE_AMD_ERROR_NOT_IMPLEMENTED;
-    capture_destroy(m->capture);
-    m->capture_len = 0;
+    drop_graphs(m);
     m->anchor_valid = false;
     if (!rsur) {
         m->rsur = nullptr;
@@ -990,8 +1012,7 @@ extern "C" int sponge_md_set_surrogate(sponge_md* m, sponge_rsur* rsur, int peri
 extern "C" int sponge_md_set_surrogate_mode(sponge_md* m, int mode) {
     if (!m || (mode != 0 && mode != 1)) return SPONGE_AMD_ERROR_VALUE_ERROR_COMMAND;
     m->sur_mode = mode;
-    capture_destroy(m->capture);
-    m->capture_len = 0;
+    drop_graphs(m);
     m->anchor_valid = false;  // the next step is a PME step of the new schedule
     return 0;
 }

@@ -287,5 +287,9 @@ def test_impulse_schedule_conserves_energy(dev, trained):
     md.set_surrogate_mode(1)
     print(f"drift kcal/mol per 1000 steps: plain {d_plain:.2f}, impulse {d_impulse:.2f}, "
           f"anchored {d_anchored:.2f}")
-    assert abs(d_impulse) < max(5.0 * abs(d_plain), 5.0)
+    # a slope fitted to 9 energies over 576 steps carries a few kcal/mol per
+    # 1000 steps of noise (the bench measures 10,000 steps): the bars are
+    # 10 kcal/mol per 1000 steps, or 5x the plain step's, against the
+    # anchored schedule's hundreds
+    assert abs(d_impulse) < max(5.0 * abs(d_plain), 10.0)
     assert abs(d_anchored) > 10.0 * abs(d_impulse)
