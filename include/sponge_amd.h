@@ -659,19 +659,24 @@ int sponge_md_clear(sponge_md* md);
 /* Batched driver                                                              */
 /* ------------------------------------------------------------------------- */
 
-/* Many independent systems (each a sponge_md on its own stream, with its own
- * neighbor list and PME grid) advanced together: one HIP graph forks every
- * system's stream from one origin stream, records steps_per_launch MD steps of
- * each and joins them, so one launch advances all systems and their kernels
- * run concurrently.  No counterpart in the reference (one process runs one
- * system there).  The systems stay owned by the caller. */
+/* Many independent systems (each a sponge_md with its own neighbor list, PME
+ * grid and FFT tiling; plain or soft-core) advanced together: one step of
+ * every system is recorded through the same code that launches it alone, and
+ * each kernel of the step then runs as ONE launch over all systems (each
+ * workgroup finds its system from the prefix of the systems' workgroup
+ * counts); steps_per_launch steps of these launches form one HIP graph.
+ * No counterpart in the reference (one process runs one system there).  The
+ * systems stay owned by the caller. */
 typedef struct sponge_batch sponge_batch;
 
 int sponge_batch_initial(sponge_batch** out, int n_systems, sponge_md* const* systems);
 
 /* sponge_pme_autotune for every distinct grid shape of the batch (the same
- * choice for systems that share a shape); tiles[3 * n_systems] (host, may be
- * NULL) gets each system's choice.  Synchronises. */
+ * choice for systems that share a shape), then, per FFT axis, those choices
+ * against one common value of {2, 4, 8, 16} timed on batched steps (one warm
+ * and `reps` timed launches of 10 steps per candidate: the call advances
+ * every system by 150 + 150 reps steps); tiles[3 * n_systems] (host, may be
+ * NULL) gets each system's final choice.  Synchronises. */
 int sponge_batch_autotune(sponge_batch* b, int reps, int* tiles);
 
 /* Advances every system by `steps` steps: whole launches of the graph of

@@ -13,6 +13,8 @@
 // into one HIP graph and replayed.  No counterpart in the reference (one
 // process runs one system there).
 #include <algorithm>
+#include <array>
+#include <chrono>
 #include <map>
 #include <tuple>
 #include <vector>
@@ -265,10 +267,73 @@ int sponge_batch_autotune(sponge_batch* b, int reps, int* tiles) {
             rc = sponge_pme_set_fft_tiling(pme, t);
             if (rc) return rc;
         }
-        if (tiles)
-            for (int a = 0; a < 3; ++a) tiles[3 * i + a] = t[a];
     }
     batch_free_record(b);  // the tilings are launch arguments: record again
+    // Refinement for the batched launches: a single system's tuning favours
+    // short lines per workgroup (a small grid alone has few lines to spread
+    // over the GPU); in one launch over every system longer ones may win.
+    // Per axis, the per-shape choices are timed against one common value of
+    // {2, 4, 8, 16} on batched steps (steps_per_launch 10: one warm launch,
+    // then `reps` timed ones), and the fastest is kept.
+    const size_t n = b->md.size();
+    std::vector<std::array<int, 3>> cur(n), best(n);
+    for (size_t i = 0; i < n; ++i) {
+        sponge_nl* nl = nullptr;
+        sponge_lj* lj = nullptr;
+        sponge_pme* pme = nullptr;
+        int rc = sponge_md_handles(b->md[i], &nl, &lj, &pme);
+        if (rc) return rc;
+        const auto& v = done[std::make_tuple(pme->fx, pme->fy, pme->fz)];
+        cur[i] = {v[0], v[1], v[2]};
+    }
+    auto apply = [&](const std::vector<std::array<int, 3>>& t) -> int {
+        for (size_t i = 0; i < n; ++i) {
+            sponge_nl* nl = nullptr;
+            sponge_lj* lj = nullptr;
+            sponge_pme* pme = nullptr;
+            int rc = sponge_md_handles(b->md[i], &nl, &lj, &pme);
+            if (!rc) rc = sponge_pme_set_fft_tiling(pme, t[i].data());
+            if (rc) return rc;
+        }
+        batch_free_record(b);
+        return 0;
+    };
+    auto time_steps = [&](double* ms) -> int {
+        constexpr int kSpl = 10;
+        int rc = sponge_batch_run(b, kSpl, kSpl);
+        if (!rc) rc = sponge_batch_synchronize(b);
+        if (rc) return rc;
+        const auto t0 = std::chrono::steady_clock::now();
+        rc = sponge_batch_run(b, reps * kSpl, kSpl);
+        if (!rc) rc = sponge_batch_synchronize(b);
+        *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        return rc;
+    };
+    for (int a = 0; a < 3; ++a) {
+        double best_ms = 0;
+        int rc = apply(cur);
+        if (!rc) rc = time_steps(&best_ms);
+        if (rc) return rc;
+        best = cur;
+        for (int v : {2, 4, 8, 16}) {
+            std::vector<std::array<int, 3>> cand = cur;
+            for (size_t i = 0; i < n; ++i) cand[i][a] = v;
+            double ms = 0;
+            rc = apply(cand);
+            if (!rc) rc = time_steps(&ms);
+            if (rc) return rc;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = cand;
+            }
+        }
+        cur = best;
+    }
+    int rc = apply(cur);
+    if (rc) return rc;
+    if (tiles)
+        for (size_t i = 0; i < n; ++i)
+            for (int a = 0; a < 3; ++a) tiles[3 * i + a] = cur[i][a];
     return 0;
 }
 
