@@ -26,6 +26,7 @@ def check_total(name, gpu, ref32, ref64, tol=1e-6):
     d_gpu = float(np.abs(gpu - ref64).max()) / scale
     d_ref = float(np.abs(ref32 - ref64).max()) / scale
     d_gr = float(np.abs(gpu - ref32).max()) / scale
+    print(f"check_total {name}: |gpu-fp64| {d_gpu:.2e} |ref32-fp64| {d_ref:.2e}")
     assert d_gpu <= max(tol, 1.5 * d_ref), \
         f"{name}: |gpu-fp64| {d_gpu:.3e}, |ref32-fp64| {d_ref:.3e}, |gpu-ref32| {d_gr:.3e}"
     assert d_gr <= tol + d_ref + d_gpu
