@@ -138,8 +138,10 @@ __global__ void __launch_bounds__(256) k_lj_pairs(
 // with solvent_numbers > 0): atoms from solvent_start on are the solvent
 // tail, whose pairs with each other the solvent module evaluates; a solvent
 // row here keeps only its partners below solvent_start.
+// LJ = false: the row atom's type has no LJ with any type (the pair
+// arithmetic is Coulomb only and the partners' types are not read).
 template <bool ORTHO, bool NEED_E, bool NEED_V, bool FAST, bool STAGED, bool SHIFT = false,
-          bool SOLV = false>
+          bool SOLV = false, bool LJ = true>
 __device__ __forceinline__ void lj_row(int p, int lane, int n, const unsigned short* __restrict__ row,
                                        PairSrc src, int nr, const int* r_first, const int* r_off,
                                        const float4* __restrict__ pos, const float2* trow, int gi,
@@ -169,8 +171,8 @@ __device__ __forceinline__ void lj_row(int p, int lane, int n, const unsigned sh
         const int qb = src_index<STAGED>(tb, nr, r_first, r_off);
         const float4 pa = src.pos[qa];
         const float4 pb = src.pos[qb];
-        const float2 aba = trow[src.typ[qa]];
-        const float2 abb = trow[src.typ[qb]];
+        const float2 aba = LJ ? trow[src.typ[qa]] : make_float2(0.f, 0.f);
+        const float2 abb = LJ ? trow[src.typ[qb]] : make_float2(0.f, 0.f);
         float3 da, db;
         float r2a, r2b;
         if (SHIFT) {
@@ -191,8 +193,8 @@ __device__ __forceinline__ void lj_row(int p, int lane, int n, const unsigned sh
             r2a = src.gid[qa] < solvent_start ? r2a : rc2_thr;
             r2b = src.gid[qb] < solvent_start ? r2b : rc2_thr;
         }
-        const PairTerm<NEED_E, NEED_V, FAST> A(da, r2a, pi4.w, pa.w, aba, rc2_thr, beta);
-        const PairTerm<NEED_E, NEED_V, FAST> B(db, r2b, pi4.w, pb.w, abb, rc2_thr, beta);
+        const PairTerm<NEED_E, NEED_V, FAST, LJ> A(da, r2a, pi4.w, pa.w, aba, rc2_thr, beta);
+        const PairTerm<NEED_E, NEED_V, FAST, LJ> B(db, r2b, pi4.w, pb.w, abb, rc2_thr, beta);
         fx += A.lx + B.lx;
         fy += A.ly + B.ly;
         fz += A.lz + B.lz;
@@ -297,6 +299,18 @@ __device__ __forceinline__ void lj_cells_body(int sa_block,
     for (int k = threadIdx.x; k <= nr; k += blockDim.x) r_off[k] = g_roff[(size_t)cell * (max_ranges + 1) + k];
     for (int k = threadIdx.x; k < T * T; k += blockDim.x) cab[k] = AB_sq[k];
     __syncthreads();
+    // types with an LJ coefficient (force-only rows of the others skip the LJ
+    // arithmetic)
+    __shared__ unsigned char tlj[kMaxLdsTypes];
+    if (FAST)
+        for (int a = threadIdx.x; a < T; a += blockDim.x) {
+            bool any = false;
+            for (int b = 0; b < T; ++b) {
+                const float2 v = cab[a * T + b];
+                any |= v.x != 0.f || v.y != 0.f;
+            }
+            tlj[a] = any;
+        }
     const int total = r_off[nr];
     const bool staged = total <= stage;
     if (staged)
@@ -314,10 +328,26 @@ __device__ __forceinline__ void lj_cells_body(int sa_block,
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave, nwave = blockDim.x / kWave;
     for (int p = a0 + wave; p < a1; p += nwave) {
-        const float2* trow = cab + ptype[p] * T;
+        const int ti = ptype[p];
+        const float2* trow = cab + ti * T;
         const int gi = kGid ? pgid[p] : 0;
         const int n = list_count[p];
         const unsigned short* row = list + (size_t)p * cap;
+        if (FAST && !SOLV && !tlj[ti]) {
+            if (staged && shift)
+                lj_row<ORTHO, false, false, true, true, true, false, false>(
+                    p, lane, n, row, PairSrc{cpos, ctyp, cgid}, nr, r_first, r_off, pos, trow, gi,
+                    c, r, rc2_thr, beta, facc, eacc, vacc, C, solvent_start);
+            else if (staged)
+                lj_row<ORTHO, false, false, true, true, false, false, false>(
+                    p, lane, n, row, PairSrc{cpos, ctyp, cgid}, nr, r_first, r_off, pos, trow, gi,
+                    c, r, rc2_thr, beta, facc, eacc, vacc, C, solvent_start);
+            else
+                lj_row<ORTHO, false, false, true, false, false, false, false>(
+                    p, lane, n, row, PairSrc{pos, ptype, pgid}, nr, r_first, r_off, pos, trow, gi,
+                    c, r, rc2_thr, beta, facc, eacc, vacc, C, solvent_start);
+            continue;
+        }
         if (staged && shift)
             lj_row<ORTHO, NEED_E, NEED_V, FAST, true, true, SOLV>(
                 p, lane, n, row, PairSrc{cpos, ctyp, cgid}, nr, r_first, r_off, pos, trow, gi, c,

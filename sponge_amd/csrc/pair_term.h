@@ -52,7 +52,10 @@ __device__ __forceinline__ void ref_powers(float rabs, float& pw3, float& pw6, f
 // Force (and energy / virial terms) of one pair, branch-free: pairs at or
 // beyond the cutoff contribute exact zeros (selects, so the non-finite
 // values of a far or padding pair never reach the sums).
-template <bool NEED_E, bool NEED_V, bool FAST>
+// LJ = false: the row atom's type has no LJ coefficient with any type (a
+// water hydrogen), so the LJ term of every pair of the row is exactly zero
+// and is not evaluated (ab unused).
+template <bool NEED_E, bool NEED_V, bool FAST, bool LJ = true>
 struct PairTerm {
     float lx, ly, lz, elj, ec;
     bool in;
@@ -67,8 +70,8 @@ struct PairTerm {
             rinv = __builtin_amdgcn_rsqf(r2);
             rabs = r2 * rinv;
             const float r2inv = rinv * rinv;
-            pw6 = r2inv * r2inv * r2inv;
-            pw8 = pw6 * r2inv;
+            pw6 = LJ ? r2inv * r2inv * r2inv : 0.f;
+            pw8 = LJ ? pw6 * r2inv : 0.f;
             pw3 = rinv * r2inv;
         } else {
             rabs = sqrtf(r2);
@@ -87,7 +90,7 @@ struct PairTerm {
             erfc_br = erfcf(br);
             bracket = fmaf(br * kTwoOverSqrtPi, expf(-br * br), erfc_br);
         }
-        const float flj = (ab.y - ab.x * pw6) * pw8;
+        const float flj = LJ ? (ab.y - ab.x * pw6) * pw8 : 0.f;
         const float fc = qq * pw3 * bracket;
         const float f = in ? flj - fc : 0.f;
         lx = f * d.x;
@@ -96,7 +99,7 @@ struct PairTerm {
         elj = 0.f;
         ec = 0.f;
         if (NEED_E) {
-            elj = in ? (0.083333333f * ab.x * pw6 - 0.166666667f * ab.y) * pw6 : 0.f;
+            elj = in && LJ ? (0.083333333f * ab.x * pw6 - 0.166666667f * ab.y) * pw6 : 0.f;
             ec = in ? qq * erfc_br / rabs : 0.f;
         }
     }
