@@ -126,8 +126,9 @@ __device__ __forceinline__ float wave_max(float v) {
 // exponential, exp(-x^2), for both erfc(x) and its derivative.  The rcp is the
 // hardware reciprocal (1 ulp).
 constexpr float kErfcxMax = 4.5f;
-__device__ __forceinline__ float erfcx_poly(float x) {
-    const float t = __builtin_amdgcn_rcpf(1.0f + 0.4f * x);
+constexpr float kErfcxT = 0.4f;  // t = 1 / (1 + kErfcxT x)
+// erfcx from t itself (callers that form t = 1 / (1 + kErfcxT x) their own way)
+__device__ __forceinline__ float erfcx_poly_t(float t) {
     float p = 7.614902268e-02f;
     p = fmaf(p, t, -4.040216792e-01f);
     p = fmaf(p, t, 7.048484256e-01f);
@@ -137,6 +138,9 @@ __device__ __forceinline__ float erfcx_poly(float x) {
     p = fmaf(p, t, 2.379624228e-01f);
     p = fmaf(p, t, 2.248110469e-01f);
     return p * t;
+}
+__device__ __forceinline__ float erfcx_poly(float x) {
+    return erfcx_poly_t(__builtin_amdgcn_rcpf(1.0f + kErfcxT * x));
 }
 
 // Lower-triangle packed pair index (Get_LJ_Type, Lennard_Jones_force.h:17-28):

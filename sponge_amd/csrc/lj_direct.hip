@@ -249,6 +249,77 @@ __device__ __forceinline__ void lj_row(int p, int lane, int n, const unsigned sh
     }
 }
 
+// One pair of lj_row_fast: the force factor of PairTerm's FAST arithmetic
+// summed into the row force by fused multiply-adds.  has = false gives the
+// absent entry of a partial pass r^2 = the cutoff threshold (out of range).
+template <bool LJ>
+__device__ __forceinline__ void fast_pair(const float4& pj, float2 ab, const float3& pi, float qi,
+                                          float rc2_thr, float beta, bool has, float& fx,
+                                          float& fy, float& fz) {
+    const float3 d = make_float3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
+    float r2 = fmaf(d.z, d.z, fmaf(d.y, d.y, d.x * d.x));
+    r2 = has ? r2 : rc2_thr;
+    const PairTerm<false, false, true, LJ> t(d, r2, qi, pj.w, ab, rc2_thr, beta);
+    fx = fmaf(t.f, d.x, fx);
+    fy = fmaf(t.f, d.y, fy);
+    fz = fmaf(t.f, d.z, fz);
+}
+
+// Force-only row of a staged cell whose partners sit at their images nearest
+// the cell's centre (lj_row with FAST, STAGED and SHIFT): the same pairs and
+// pair arithmetic, with the passes in which all 128 entries exist run under
+// wave-uniform loop control and without presence tests; only the partial
+// last pass masks its absent entries.
+template <bool LJ>
+__device__ __forceinline__ void lj_row_fast(int p, int lane, int n, const unsigned short* __restrict__ row,
+                                            const float4* cpos, const int* ctyp,
+                                            const float4* __restrict__ pos, const float2* trow,
+                                            const LT6& c, const LT6& r, float rc2_thr, float beta,
+                                            float4* __restrict__ facc, float3 C) {
+    const float4 pi4 = pos[p];
+    const float3 pi = near_image(make_float3(pi4.x, pi4.y, pi4.z), C, c, r);
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    const int npairs = (n + 1) >> 1;  // 32-bit words holding the row
+    const int full = n >> 7;          // passes whose 2 x 64 entries all exist
+    const unsigned* row2 = reinterpret_cast<const unsigned*>(row);
+    // unsigned word indices: a 32-bit offset from the row's base
+    const unsigned last = npairs > 0 ? npairs - 1 : 0;
+    unsigned cur = row2[(unsigned)lane < last ? (unsigned)lane : last];
+    int w = lane;
+    for (int k = 0; k < full; ++k, w += kWave) {
+        const unsigned wn = (unsigned)(w + kWave);
+        unsigned nxt = row2[wn < last ? wn : last];
+        const int ta = (int)(cur & 0xffffu), tb = (int)(cur >> 16);
+        const float4 pa = cpos[ta];
+        const float4 pb = cpos[tb];
+        const float2 aba = LJ ? trow[ctyp[ta]] : make_float2(0.f, 0.f);
+        const float2 abb = LJ ? trow[ctyp[tb]] : make_float2(0.f, 0.f);
+        fast_pair<LJ>(pa, aba, pi, pi4.w, rc2_thr, beta, true, fx, fy, fz);
+        fast_pair<LJ>(pb, abb, pi, pi4.w, rc2_thr, beta, true, fx, fy, fz);
+        // the loaded word is opaque from here: the compiler cannot fold the
+        // prefetch back into a load at the top of the next pass
+        asm volatile("" : "+v"(nxt));
+        cur = nxt;
+    }
+    if (full * 2 * kWave < n) {
+        // cur is the row's word w, or its last word for lanes past the end
+        const bool has_a = w < npairs;
+        const bool has_b = 2 * w + 1 < n;
+        const int ta = (int)(cur & 0xffffu);
+        const int tb = has_b ? (int)(cur >> 16) : ta;
+        const float4 pa = cpos[ta];
+        const float4 pb = cpos[tb];
+        const float2 aba = LJ ? trow[ctyp[ta]] : make_float2(0.f, 0.f);
+        const float2 abb = LJ ? trow[ctyp[tb]] : make_float2(0.f, 0.f);
+        fast_pair<LJ>(pa, aba, pi, pi4.w, rc2_thr, beta, has_a, fx, fy, fz);
+        fast_pair<LJ>(pb, abb, pi, pi4.w, rc2_thr, beta, has_b, fx, fy, fz);
+    }
+    fx = wave_sum(fx);
+    fy = wave_sum(fy);
+    fz = wave_sum(fz);
+    if (lane == 0) facc[p] = make_float4(fx, fy, fz, 0.f);
+}
+
 template <bool ORTHO, bool NEED_E, bool NEED_V, bool FAST, bool SOLV = false>
 __device__ __forceinline__ void lj_cells_body(int sa_block, 
     const int* __restrict__ start, const int* __restrict__ nranges, int max_ranges,
@@ -333,12 +404,17 @@ __device__ __forceinline__ void lj_cells_body(int sa_block,
         const int gi = kGid ? pgid[p] : 0;
         const int n = list_count[p];
         const unsigned short* row = list + (size_t)p * cap;
+        if (FAST && !SOLV && staged && shift) {
+            if (tlj[ti])
+                lj_row_fast<true>(p, lane, n, row, cpos, ctyp, pos, trow, c, r, rc2_thr, beta,
+                                  facc, C);
+            else
+                lj_row_fast<false>(p, lane, n, row, cpos, ctyp, pos, trow, c, r, rc2_thr, beta,
+                                   facc, C);
+            continue;
+        }
         if (FAST && !SOLV && !tlj[ti]) {
-            if (staged && shift)
-                lj_row<ORTHO, false, false, true, true, true, false, false>(
-                    p, lane, n, row, PairSrc{cpos, ctyp, cgid}, nr, r_first, r_off, pos, trow, gi,
-                    c, r, rc2_thr, beta, facc, eacc, vacc, C, solvent_start);
-            else if (staged)
+            if (staged)
                 lj_row<ORTHO, false, false, true, true, false, false, false>(
                     p, lane, n, row, PairSrc{cpos, ctyp, cgid}, nr, r_first, r_off, pos, trow, gi,
                     c, r, rc2_thr, beta, facc, eacc, vacc, C, solvent_start);

@@ -376,7 +376,12 @@ __device__ __forceinline__ void nl_search_cells_body(int sa_block,
             pi[u] = make_float3(v.x, v.y, v.z);
             pc[u] = shift ? near_image(pi[u], C, c, r) : pi[u];
             gi[u] = __float_as_int(v.w);
-            wi[u] = sexw[p];
+            // the exclusion window widened to hold the row atom itself: the
+            // row's own stream entry takes the window's rare path, which
+            // drops it by id (the sweep tests no ids otherwise)
+            const int2 w = sexw[p];
+            wi[u] = w.x > w.y ? make_int2(gi[u], gi[u])
+                              : make_int2(min(w.x, gi[u]), max(w.y, gi[u]));
             cnt[u] = 0;
         }
         const int na = act[3] ? 4 : act[2] ? 3 : act[1] ? 2 : act[0] ? 1 : 0;
@@ -428,6 +433,18 @@ __device__ __forceinline__ void nl_search_cells_body(int sa_block,
                 unsigned short* rowp[NA];
 #pragma unroll
                 for (int u = 0; u < NA; ++u) rowp[u] = list + (size_t)(a0 + wave + (k0 + u) * nwave) * cap;
+                // a row's hits of one half-pass appended in stream order (mask
+                // m of the lanes holding ok): one uniform capacity test unless
+                // the row is about to overflow
+                auto append = [&](int u, int h, unsigned long long m, bool okv, int t) {
+                    const int after = cnt[u] + (int)__popcll(m);
+                    if (okv) {
+                        const unsigned pos = __builtin_amdgcn_mbcnt_hi(
+                            (unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, (unsigned)cnt[u]));
+                        if (after <= cap || pos < (unsigned)cap) rowp[u][pos] = (unsigned short)t;
+                    }
+                    cnt[u] = after;
+                };
                 for (int base = 0; base < clen; base += H * kWave) {
                     float3 pj[H];
                     int gj[H], tt[H];
@@ -438,8 +455,11 @@ __device__ __forceinline__ void nl_search_cells_body(int sa_block,
                         in[h] = loc < clen;
                         tt[h] = c0 + loc;
                         const float4 v = cand[in[h] ? loc : 0];
-                        pj[h] = make_float3(v.x, v.y, v.z);
-                        gj[h] = in[h] ? __float_as_int(v.w) : -1;
+                        // SH: a lane past the chunk tests a far point (r2
+                        // ~1e36, beyond every threshold; no test of in)
+                        pj[h] = SH && !in[h] ? make_float3(1e18f, 1e18f, 1e18f)
+                                             : make_float3(v.x, v.y, v.z);
+                        gj[h] = __float_as_int(v.w);
                     }
                     bool ok[H][NA], amb[H][NA], chk[H][NA];
                     bool rare = false;
@@ -447,19 +467,21 @@ __device__ __forceinline__ void nl_search_cells_body(int sa_block,
                     for (int h = 0; h < H; ++h) {
 #pragma unroll
                         for (int u = 0; u < NA; ++u) {
-                            const bool cand_ok = in[h] & (gj[h] != gi[u]);
+                            bool near;
                             if (SH) {
                                 const float dx = pj[h].x - pc[u].x, dy = pj[h].y - pc[u].y,
                                             dz = pj[h].z - pc[u].z;
                                 const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                                ok[h][u] = cand_ok & (r2 < thr_lo);
-                                amb[h][u] = cand_ok & (r2 >= thr_lo) & (r2 < thr_hi);
+                                ok[h][u] = r2 < thr_lo;
+                                near = r2 < thr_hi;
+                                amb[h][u] = near & !ok[h][u];
                             } else {
                                 // Get_Displacement(crd_i, crd_j) as in Find_Neighbors_Gridly
-                                ok[h][u] = cand_ok & (norm2_exact(disp_pbc<ORTHO>(pi[u], pj[h], c, r)) < thr);
+                                ok[h][u] = in[h] & (norm2_exact(disp_pbc<ORTHO>(pi[u], pj[h], c, r)) < thr);
+                                near = ok[h][u];
                                 amb[h][u] = false;
                             }
-                            chk[h][u] = (ok[h][u] | amb[h][u]) & (gj[h] >= wi[u].x) & (gj[h] <= wi[u].y);
+                            chk[h][u] = near & (gj[h] >= wi[u].x) & (gj[h] <= wi[u].y);
                             rare |= amb[h][u] | chk[h][u];
                         }
                     }
@@ -470,21 +492,21 @@ __device__ __forceinline__ void nl_search_cells_body(int sa_block,
                             for (int u = 0; u < NA; ++u) {
                                 if (SH && amb[h][u]) ok[h][u] = exact(u, tt[h], pj[h]);
                                 if (chk[h][u] && ok[h][u])
-                                    ok[h][u] = !excluded_pair_inl(gi[u], gj[h], ex_start, ex_list, ex_num);
+                                    ok[h][u] = gj[h] != gi[u] &&
+                                               !excluded_pair_inl(gi[u], gj[h], ex_start, ex_list, ex_num);
                             }
                         }
-                    }
 #pragma unroll
-                    for (int u = 0; u < NA; ++u) {
+                        for (int u = 0; u < NA; ++u)
 #pragma unroll
-                        for (int h = 0; h < H; ++h) {
-                            const unsigned long long m = __ballot(ok[h][u]);
-                            if (ok[h][u]) {
-                                const int pos = cnt[u] + lanes_below(m);
-                                if (pos < cap) rowp[u][pos] = (unsigned short)tt[h];
-                            }
-                            cnt[u] += __popcll(m);
-                        }
+                            for (int h = 0; h < H; ++h) append(u, h, __ballot(ok[h][u]), ok[h][u], tt[h]);
+                    } else {
+                        // no exclusion or rounding case in this pass: the
+                        // compares' masks are final
+#pragma unroll
+                        for (int u = 0; u < NA; ++u)
+#pragma unroll
+                            for (int h = 0; h < H; ++h) append(u, h, __ballot(ok[h][u]), ok[h][u], tt[h]);
                     }
                 }
             };

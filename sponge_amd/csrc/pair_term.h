@@ -49,6 +49,20 @@ __device__ __forceinline__ void ref_powers(float rabs, float& pw3, float& pw6, f
     pw8 = (float)(ri6 * ri2);
 }
 
+// erfc(beta r) and the force bracket erfc(beta r) + 2 beta r / sqrt(pi)
+// exp(-beta^2 r^2) from r^2 and r (FAST arithmetic): erfc as exp(-x^2)
+// erfcx(x), with beta folded into the constants (hoisted out of a row's
+// loop), so a pair costs one exp2 of r^2 times a constant and one rcp.
+__device__ __forceinline__ void coul_fast(float r2, float rabs, float beta, float& erfc_br,
+                                          float& bracket) {
+    constexpr float kLog2e = 1.4426950408889634f;
+    const float t = __builtin_amdgcn_rcpf(fmaf(rabs, kErfcxT * beta, 1.0f));
+    const float ex = __builtin_amdgcn_exp2f(r2 * (-kLog2e * beta * beta));
+    const float ecx = erfcx_poly_t(t);
+    erfc_br = ex * ecx;
+    bracket = ex * fmaf(rabs, kTwoOverSqrtPi * beta, ecx);
+}
+
 // Force (and energy / virial terms) of one pair, branch-free: pairs at or
 // beyond the cutoff contribute exact zeros (selects, so the non-finite
 // values of a far or padding pair never reach the sums).
@@ -57,7 +71,7 @@ __device__ __forceinline__ void ref_powers(float rabs, float& pw3, float& pw6, f
 // and is not evaluated (ab unused).
 template <bool NEED_E, bool NEED_V, bool FAST, bool LJ = true>
 struct PairTerm {
-    float lx, ly, lz, elj, ec;
+    float f, lx, ly, lz, elj, ec;  // f: the force factor (force = f d)
     bool in;
     __device__ __forceinline__ PairTerm(const float3& d, float r2, float qi, float qj, float2 ab,
                                         float rc2_thr, float beta) {
@@ -79,20 +93,17 @@ struct PairTerm {
             ref_powers(rabs, pw3, pw6, pw8);
         }
         const float qq = qi * qj;
-        const float br = beta * rabs;
         float erfc_br, bracket;
         if (FAST) {
-            const float ex = __expf(-br * br);
-            const float ecx = erfcx_poly(br);
-            erfc_br = ex * ecx;
-            bracket = ex * fmaf(br, kTwoOverSqrtPi, ecx);
+            coul_fast(r2, rabs, beta, erfc_br, bracket);
         } else {
+            const float br = beta * rabs;
             erfc_br = erfcf(br);
             bracket = fmaf(br * kTwoOverSqrtPi, expf(-br * br), erfc_br);
         }
         const float flj = LJ ? (ab.y - ab.x * pw6) * pw8 : 0.f;
         const float fc = qq * pw3 * bracket;
-        const float f = in ? flj - fc : 0.f;
+        f = in ? flj - fc : 0.f;
         lx = f * d.x;
         ly = f * d.y;
         lz = f * d.z;

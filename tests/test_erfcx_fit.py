@@ -1,5 +1,5 @@
 """The erfcx polynomial of the pair kernel's force-only path
-(sponge_amd/csrc/common.h, erfcx_poly) against scipy.special.erfcx, and its
+(sponge_amd/csrc/common.h, erfcx_poly_t) against scipy.special.erfcx, and its
 generator (tests/golden/make_erfcx_fit.py) reproducing a fit of the same
 accuracy."""
 import os
@@ -16,13 +16,14 @@ import make_erfcx_fit as G  # noqa: E402
 
 
 def committed_coefficients():
-    """The literals of erfcx_poly in Horner order (highest power first)."""
+    """The literals of erfcx_poly_t in Horner order (highest power first), and
+    the t scale kErfcxT of t = 1 / (1 + kErfcxT x)."""
     src = open(os.path.join(ROOT, "sponge_amd", "csrc", "common.h")).read()
-    body = src[src.index("float erfcx_poly(float x)"):]
+    body = src[src.index("float erfcx_poly_t(float t)"):]
     body = body[:body.index("return p * t;")]
     vals = re.findall(r"([-+]?\d\.\d+e[-+]\d+)f", body)
     assert len(vals) == G.DEGREE + 1, vals
-    scale = re.search(r"1\.0f \+ ([0-9.]+)f \* x", body)
+    scale = re.search(r"constexpr float kErfcxT = ([0-9.]+)f;", src)
     assert scale and float(scale.group(1)) == G.T_SCALE
     return [float(v) for v in vals]
 
