@@ -76,9 +76,17 @@ __device__ __forceinline__ int batch_locate(const int* __restrict__ start, int n
     return lo;
 }
 
+// Wavefronts per SIMD that a body's batched kernel is compiled for: 8 caps
+// it at 64 VGPRs (a body whose speed depends on full occupancy specialises
+// this); 1 leaves the register budget to the compiler.
+template <auto F>
+struct BodyWaves {
+    static constexpr int value = 1;
+};
+
 template <int LB, auto F>
-__global__ void __launch_bounds__(LB) k_batched(const int* __restrict__ start, int n,
-                                                const char* __restrict__ blobs, int stride) {
+__global__ void __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(BodyWaves<F>::value)))
+k_batched(const int* __restrict__ start, int n, const char* __restrict__ blobs, int stride) {
     using Tr = BodyTraits<decltype(F)>;
     int local = 0;
     const int sys = batch_locate(start, n, (int)blockIdx.x, local);
