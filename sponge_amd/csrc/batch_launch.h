@@ -17,6 +17,7 @@
 
 #include <cstddef>
 #include <cstring>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -49,11 +50,14 @@ struct TupleGet<0> {
     }
 };
 
+// A body's parameters are stored by value in its tuple; a body that takes a
+// structure by const reference reads it in place (from the kernel arguments,
+// or from the batched launch's argument blob), never through a private copy.
 template <typename F>
 struct BodyTraits;
 template <typename... P>
 struct BodyTraits<void (*)(int, P...)> {
-    using Tuple = DevTuple<P...>;
+    using Tuple = DevTuple<std::decay_t<P>...>;
     static constexpr int arity = sizeof...(P);
 };
 

@@ -192,7 +192,7 @@ constexpr int kMaxFftLen = 512;  // largest grid dimension (checked at init)
 // z pass, real -> half spectrum: block = (y block, x)
 __device__ __forceinline__ void fft_z_r2c_body(int sa_block, const float* __restrict__ Q,
                                                    float2* __restrict__ FQ, int fx, int fy, int fz,
-                                                   int fzc, int TL, sa_radix_plan pl,
+                                                   int fzc, int TL, const sa_radix_plan& pl,
                                                    const float2* __restrict__ tw) {
     extern __shared__ float2 sm[];
     __shared__ float2 stw[kMaxFftLen];
@@ -233,7 +233,7 @@ __global__ void __launch_bounds__(256) k_fft_z_r2c(const float* __restrict__ Q,
 // y pass over contiguous lines [kz][x][0..fy)
 template <int SIGN>
 __device__ __forceinline__ void fft_y_body(int sa_block, float2* __restrict__ FQ, int nlines, int fy, int TL,
-                                               sa_radix_plan pl, const float2* __restrict__ tw) {
+                                               const sa_radix_plan& pl, const float2* __restrict__ tw) {
     extern __shared__ float2 sm[];
     __shared__ float2 stw[kMaxFftLen];
     const int l0 = sa_block * TL;
@@ -265,7 +265,7 @@ __global__ void __launch_bounds__(256) k_fft_y(float2* __restrict__ FQ, int nlin
 template <bool ACC>
 __device__ __forceinline__ void fft_x_conv_body(int sa_block, 
     float2* __restrict__ FQ, const float* __restrict__ BCt, const float* __restrict__ vBCt,
-    long long nfft, int fx, int fy, int fz, int fzc, int TL, sa_radix_plan pl,
+    long long nfft, int fx, int fy, int fz, int fzc, int TL, const sa_radix_plan& pl,
     const float2* __restrict__ tw, int need_virial, double* __restrict__ acc) {
     extern __shared__ float2 sm[];
     __shared__ double red[7][4];
@@ -345,7 +345,7 @@ __global__ void __launch_bounds__(256) k_fft_x_conv(
 // z pass, half spectrum -> real (unnormalised C2R): block = (y block, x)
 __device__ __forceinline__ void fft_z_c2r_body(int sa_block, const float2* __restrict__ FQ,
                                                    float* __restrict__ out, int fx, int fy, int fz,
-                                                   int fzc, int TL, sa_radix_plan pl,
+                                                   int fzc, int TL, const sa_radix_plan& pl,
                                                    const float2* __restrict__ tw) {
     extern __shared__ float2 sm[];
     __shared__ float2 stw[kMaxFftLen];
@@ -1200,6 +1200,31 @@ TileGeom pme_tile_geom(const sponge_pme* p) {
 }
 
 }  // namespace
+
+// Register budgets of the batched FFT passes: at 6 wavefronts per SIMD they
+// fit the standalone kernels' ~77 VGPRs (left free, the batched kernels took
+// 100-118 and ran at 4 wavefronts).  The energy / virial convolution keeps
+// the compiler's budget (it would spill).
+template <>
+struct sa::BodyWaves<&fft_z_r2c_body> {
+    static constexpr int value = 6;
+};
+template <>
+struct sa::BodyWaves<&fft_y_body<-1>> {
+    static constexpr int value = 6;
+};
+template <>
+struct sa::BodyWaves<&fft_y_body<1>> {
+    static constexpr int value = 6;
+};
+template <>
+struct sa::BodyWaves<&fft_x_conv_body<false>> {
+    static constexpr int value = 6;
+};
+template <>
+struct sa::BodyWaves<&fft_z_c2r_body> {
+    static constexpr int value = 6;
+};
 
 extern "C" void sponge_pme_default_options(sponge_pme_options* o) {
     o->fftx = o->ffty = o->fftz = -1;
