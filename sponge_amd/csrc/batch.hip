@@ -32,6 +32,8 @@ namespace {
 // one kernel of the recorded step, over every system
 struct BatchPos {
     const void* fn = nullptr;
+    int lane = 0;       // 1: on the side stream
+    bool join = false;  // after the side stream's launches so far
     dim3 block;
     size_t shmem = 0;      // the largest of the systems'
     int total_blocks = 0;  // sum over the systems
@@ -45,6 +47,8 @@ struct BatchPos {
 struct sponge_batch {
     std::vector<sponge_md*> md;
     hipStream_t stream = nullptr;  // the batched launches
+    hipStream_t side = nullptr;    // their reciprocal-space chain
+    hipEvent_t fork = nullptr, join = nullptr;
     std::vector<hipEvent_t> done;  // per system: its earlier work
     hipEvent_t after = nullptr;    // end of a run
     std::vector<BatchPos> pos;
@@ -80,9 +84,12 @@ void batch_free(sponge_batch* b) {
         if (e) (void)hipEventDestroy(e);
     b->done.clear();
     if (b->after) (void)hipEventDestroy(b->after);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->join) (void)hipEventDestroy(b->join);
     if (b->stream) (void)hipStreamDestroy(b->stream);
-    b->after = nullptr;
-    b->stream = nullptr;
+    if (b->side) (void)hipStreamDestroy(b->side);
+    b->after = b->fork = b->join = nullptr;
+    b->stream = b->side = nullptr;
 }
 
 // One step of system m recorded instead of launched.  The system's stream is
@@ -139,7 +146,8 @@ int batch_record(sponge_batch* b) {
             const RecordedLaunch& a = rec[0].launches[p];
             const RecordedLaunch& c = rec[i].launches[p];
             if (a.fn != c.fn || a.block.x != c.block.x || a.block.y != c.block.y ||
-                a.block.z != c.block.z || a.args.size() != c.args.size())
+                a.block.z != c.block.z || a.args.size() != c.args.size() ||
+                a.lane != c.lane || a.join != c.join)
                 return SPONGE_AMD_ERROR_NOT_IMPLEMENTED;
         }
     }
@@ -151,6 +159,8 @@ int batch_record(sponge_batch* b) {
         BatchPos& bp = b->pos[p];
         const RecordedLaunch& r0 = rec[0].launches[p];
         bp.fn = r0.fn;
+        bp.lane = r0.lane;
+        bp.join = r0.join;
         bp.block = r0.block;
         bp.stride = (int)align16(r0.args.size());
         std::vector<int> start(n + 1, 0);
@@ -184,18 +194,40 @@ int batch_record(sponge_batch* b) {
     return 0;
 }
 
-// k batched steps on b->stream (eager, or into the capture in progress)
+// k batched steps on b->stream (eager, or into the capture in progress).  The
+// launches recorded on the side lane (the reciprocal chain that reads only the
+// coordinates) run on b->side, forked at the start of each step and joined
+// before the first launch that reads their results, as the single-system
+// step forks them.
 int batch_issue(sponge_batch* b, int k) {
     const int n = (int)b->md.size();
-    for (int step = 0; step < k; ++step)
+    bool any_side = false;
+    for (const BatchPos& bp : b->pos) any_side |= bp.lane == 1;
+    for (int step = 0; step < k; ++step) {
+        bool open = false;  // side launches of this step not yet joined
+        if (any_side) {
+            SA_CHECK(hipEventRecord(b->fork, b->stream));
+            SA_CHECK(hipStreamWaitEvent(b->side, b->fork, 0));
+        }
         for (const BatchPos& bp : b->pos) {
+            if (bp.join && open) {
+                SA_CHECK(hipEventRecord(b->join, b->side));
+                SA_CHECK(hipStreamWaitEvent(b->stream, b->join, 0));
+                open = false;
+            }
             const int* start = reinterpret_cast<const int*>(b->d_buf + bp.start_off);
             const char* args = b->d_buf + bp.args_off;
             int nn = n, stride = bp.stride;
             void* kargs[] = {(void*)&start, (void*)&nn, (void*)&args, (void*)&stride};
             SA_CHECK(hipLaunchKernel(bp.fn, dim3(bp.total_blocks), bp.block, kargs, bp.shmem,
-                                     b->stream));
+                                     bp.lane == 1 ? b->side : b->stream));
+            open |= bp.lane == 1;
         }
+        if (open) {  // no recorded consumer: joined at the step's end
+            SA_CHECK(hipEventRecord(b->join, b->side));
+            SA_CHECK(hipStreamWaitEvent(b->stream, b->join, 0));
+        }
+    }
     return 0;
 }
 
@@ -229,7 +261,10 @@ int sponge_batch_initial(sponge_batch** out, int n_systems, sponge_md* const* sy
     b->md.assign(systems, systems + n_systems);
     b->done.assign(n_systems, nullptr);
     bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess &&
-              hipEventCreateWithFlags(&b->after, hipEventDisableTiming) == hipSuccess;
+              hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&b->after, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&b->join, hipEventDisableTiming) == hipSuccess;
     for (int i = 0; i < n_systems && ok; ++i)
         ok = hipEventCreateWithFlags(&b->done[i], hipEventDisableTiming) == hipSuccess;
     if (!ok) {

@@ -104,10 +104,17 @@ struct RecordedLaunch {
     dim3 block;
     size_t shmem = 0;
     std::vector<char> args;    // the body's DevTuple
+    int lane = 0;              // 0: the step's stream; 1: its side stream
+    bool join = false;         // waits for the side stream's launches so far
 };
 
+// lane / join_next: the recording code marks the launches that may run on a
+// side stream beside the rest of the step (lane 1), and the first launch that
+// needs their results (join_next, consumed by the next recorded launch)
 struct LaunchRecorder {
     std::vector<RecordedLaunch> launches;
+    int lane = 0;
+    bool join_next = false;
 };
 
 // the recorder of the calling thread (null: launches run)
@@ -124,6 +131,9 @@ void record_launch(dim3 grid, dim3 block, size_t shmem, A&&... a) {
     r.shmem = shmem;
     r.args.resize(sizeof(t));
     std::memcpy(r.args.data(), &t, sizeof(t));
+    r.lane = g_recorder->lane;
+    r.join = g_recorder->join_next;
+    g_recorder->join_next = false;
     g_recorder->launches.push_back(std::move(r));
 }
 

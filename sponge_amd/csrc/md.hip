@@ -492,6 +492,18 @@ int md_step(sponge_md* m, long long step, int need_energy, StepKind* kind_out = 
         if (rc) return rc;
         SA_CHECK(hipEventRecord(m->ev_join, m->recip_stream));
     }
+    // a step recorded for the batched driver marks the same reciprocal chain
+    // for the side stream of the batched step (the driver forks it at the
+    // step's start and joins it before the gather)
+    const bool rec_side = g_recorder && !allow_fork && kind == StepKind::Pme && !m->rsur &&
+                          !m->collect && pme_calculates_reciprocal(m->pme) &&
+                          pme_updates_on(m->pme, (int)step);
+    if (rec_side) {
+        g_recorder->lane = 1;
+        rc = pme_recip_begin(m->pme, m->d_crd, cell, rcell, m->d_charge, 0, need_energy, (int)step);
+        g_recorder->lane = 0;
+        if (rc) return rc;
+    }
     rc = nl_update_impl(m->nl, nullptr, N, 0, m->d_crd, cell, rcell, (int)step,
                         SPONGE_AMD_NL_CONDITIONAL, m->d_excl_start, m->d_excl_list, m->d_excl_num,
                         m->need_cleared);
@@ -532,6 +544,11 @@ int md_step(sponge_md* m, long long step, int need_energy, StepKind* kind_out = 
         rc = pme_recip_end(m->pme, rcell, m->d_charge, m->d_frc, 0, need_energy, nullptr,
                            need_energy ? m->d_potential : nullptr, (int)step,
                            ex_side ? m->d_exfrc : nullptr);
+        if (rc) return rc;
+    } else if (rec_side) {
+        g_recorder->join_next = true;
+        rc = pme_recip_end(m->pme, rcell, m->d_charge, m->d_frc, 0, need_energy, nullptr,
+                           need_energy ? m->d_potential : nullptr, (int)step);
         if (rc) return rc;
     } else if (kind == StepKind::Pme && m->rsur && !m->collect) {
         // exact step of the learned schedule: the anchor of the next steps
