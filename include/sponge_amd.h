@@ -665,6 +665,9 @@ int sponge_md_clear(sponge_md* md);
  * each kernel of the step then runs as ONE launch over all systems (each
  * workgroup finds its system from the prefix of the systems' workgroup
  * counts); steps_per_launch steps of these launches form one HIP graph.
+ * The reciprocal chain of each step (PME bin, spread, FFT convolution) runs
+ * on a second stream of the batch, forked at the step's start and joined
+ * before the gather, as in sponge_md_set_reciprocal_overlap.
  * No counterpart in the reference (one process runs one system there).  The
  * systems stay owned by the caller. */
 typedef struct sponge_batch sponge_batch;
