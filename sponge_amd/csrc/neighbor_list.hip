@@ -240,10 +240,10 @@ __device__ __forceinline__ int cell_stream(int cell, const int* __restrict__ sta
 
 // One workgroup (8 wavefronts) per cell.  The cell's candidate stream (sorted
 // positions with the global id in w) is staged into LDS in chunks of
-// kSearchChunk positions; each wavefront takes up to kSearchRows i atoms of
-// the cell and sweeps every chunk, two candidates per lane per pass, testing
-// each against all of its rows (one LDS read feeds up to kSearchRows
-// independent tests).  Small chunks keep the workgroup's LDS small, so that
+// kSearchChunk positions; each wavefront takes up to ROWS i atoms of the cell
+// (kSearchRows = 4; 2 in the batched driver's launch) and sweeps every chunk,
+// two candidates per lane per pass, testing each against all of its rows (one
+// LDS read feeds up to ROWS independent tests).  Small chunks keep the workgroup's LDS small, so that
 // several workgroups share a CU: the search is bound by latency (dependent
 // compares, ballots and stores), which only more resident wavefronts hide.
 // The pair test is the reference's Find_Neighbors_Gridly criterion
@@ -301,7 +301,7 @@ __device__ __forceinline__ bool excluded_pair_inl(int a, int b, const int* __res
     return hit;
 }
 
-template <bool ORTHO>
+template <bool ORTHO, int ROWS = kSearchRows>
 __device__ __forceinline__ void nl_search_cells_body(int sa_block, 
     const int* __restrict__ need, const float4* __restrict__ spos, const int2* __restrict__ sexw,
     const int* __restrict__ start, const int2* __restrict__ ranges,
@@ -362,13 +362,13 @@ __device__ __forceinline__ void nl_search_cells_body(int sa_block,
     const int rows = a1 - a0;
     // every wavefront runs the same row-block and chunk loops (the barriers
     // of the staging); a wavefront without rows in a block only stages
-    for (int k0 = 0; k0 * nwave < rows; k0 += kSearchRows) {
-        float3 pi[kSearchRows], pc[kSearchRows];
-        int gi[kSearchRows], cnt[kSearchRows];
-        int2 wi[kSearchRows];
-        bool act[kSearchRows];
+    for (int k0 = 0; k0 * nwave < rows; k0 += ROWS) {
+        float3 pi[ROWS], pc[ROWS];
+        int gi[ROWS], cnt[ROWS];
+        int2 wi[ROWS];
+        bool act[ROWS];
 #pragma unroll
-        for (int u = 0; u < kSearchRows; ++u) {
+        for (int u = 0; u < ROWS; ++u) {
             const int row = wave + (k0 + u) * nwave;
             act[u] = row < rows;
             const int p = a0 + (act[u] ? row : 0);
@@ -384,7 +384,9 @@ __device__ __forceinline__ void nl_search_cells_body(int sa_block,
                               : make_int2(min(w.x, gi[u]), max(w.y, gi[u]));
             cnt[u] = 0;
         }
-        const int na = act[3] ? 4 : act[2] ? 3 : act[1] ? 2 : act[0] ? 1 : 0;
+        int na = 0;
+#pragma unroll
+        for (int u = 0; u < ROWS; ++u) na += act[u] ? 1 : 0;
         for (int c0 = 0; c0 < total; c0 += kSearchChunk) {
             const int clen = min(kSearchChunk, total - c0);
             __syncthreads();  // the previous chunk is no longer read
@@ -510,28 +512,28 @@ __device__ __forceinline__ void nl_search_cells_body(int sa_block,
                     }
                 }
             };
-            static_assert(kSearchRows == 4, "the sweep dispatch covers 1-4 rows");
+            static_assert(ROWS == 4 || ROWS == 2, "the sweep dispatch covers 2 or 4 rows");
             using I1 = std::integral_constant<int, 1>;
             using I2 = std::integral_constant<int, 2>;
             using I3 = std::integral_constant<int, 3>;
             using I4 = std::integral_constant<int, 4>;
-            if (shift) {
-                using S = std::true_type;
-                if (na == 4) sweep(I4{}, S{});
-                else if (na == 3) sweep(I3{}, S{});
-                else if (na == 2) sweep(I2{}, S{});
-                else if (na == 1) sweep(I1{}, S{});
-            } else {
-                using S = std::false_type;
-                if (na == 4) sweep(I4{}, S{});
-                else if (na == 3) sweep(I3{}, S{});
-                else if (na == 2) sweep(I2{}, S{});
-                else if (na == 1) sweep(I1{}, S{});
-            }
+            auto dispatch = [&](auto shift_tag) {
+                if constexpr (ROWS == 4) {
+                    if (na == 4) sweep(I4{}, shift_tag);
+                    else if (na == 3) sweep(I3{}, shift_tag);
+                    else if (na == 2) sweep(I2{}, shift_tag);
+                    else if (na == 1) sweep(I1{}, shift_tag);
+                } else {
+                    if (na == 2) sweep(I2{}, shift_tag);
+                    else if (na == 1) sweep(I1{}, shift_tag);
+                }
+            };
+            if (shift) dispatch(std::true_type{});
+            else dispatch(std::false_type{});
         }
         if (lane == 0) {
 #pragma unroll
-            for (int u = 0; u < kSearchRows; ++u) {
+            for (int u = 0; u < ROWS; ++u) {
                 if (!act[u]) continue;
                 const int p = a0 + wave + (k0 + u) * nwave;
                 list_count[p] = cnt[u] < cap ? cnt[u] : cap;
@@ -697,6 +699,12 @@ void nl_free(sponge_nl* nl) {
 }
 
 }  // namespace
+
+// the batched search's register budget: its LDS allows 6 wavefronts per SIMD
+template <>
+struct sa::BodyWaves<&nl_search_cells_body<true, 2>> {
+    static constexpr int value = 6;
+};
 
 extern "C" void sponge_nl_default_options(sponge_nl_options* o) {
     o->refresh_interval = 0;
@@ -880,20 +888,33 @@ int sa::nl_update_impl(sponge_nl* nl, const int* atom_local, int local_atom_numb
     SA_LAUNCH_CHECK();
     {
         ScopedTimer timer("nl_search", s);
-        if (nl->ortho)
-            SA_LAUNCH(kSearchThreads, (&nl_search_cells_body<true>), (k_nl_search_cells<true>), dim3(nl->G), dim3(kSearchThreads), 0, s, nl->d_need,
-                               nl->d_spos, nl->d_sexw, nl->d_cell_start, nl->d_ranges,
-                               nl->d_nranges, nl->max_ranges, c, r, nl->thr, excluded_list_start,
-                               excluded_list, excluded_numbers, nl->cap, nl->d_list_count,
-                               nl->d_list, nl->d_rfirst, nl->d_roff, nl->d_flags,
-                               nl->grid_length, nl->Nx, nl->Ny, nl->Nz);
-        else
+        if (nl->ortho) {
+            // the batched search at two rows per wavefront: fewer scalar
+            // registers, so its batched kernel runs at 6 wavefronts per SIMD
+            // (at 4 rows it needed 108 VGPRs and ran at 4)
+            if (g_recorder) {
+                SA_LAUNCH(kSearchThreads, (&nl_search_cells_body<true, 2>), (k_nl_search_cells<true>), dim3(nl->G), dim3(kSearchThreads), 0, s, nl->d_need,
+                                   nl->d_spos, nl->d_sexw, nl->d_cell_start, nl->d_ranges,
+                                   nl->d_nranges, nl->max_ranges, c, r, nl->thr, excluded_list_start,
+                                   excluded_list, excluded_numbers, nl->cap, nl->d_list_count,
+                                   nl->d_list, nl->d_rfirst, nl->d_roff, nl->d_flags,
+                                   nl->grid_length, nl->Nx, nl->Ny, nl->Nz);
+            } else {
+                SA_LAUNCH(kSearchThreads, (&nl_search_cells_body<true>), (k_nl_search_cells<true>), dim3(nl->G), dim3(kSearchThreads), 0, s, nl->d_need,
+                                   nl->d_spos, nl->d_sexw, nl->d_cell_start, nl->d_ranges,
+                                   nl->d_nranges, nl->max_ranges, c, r, nl->thr, excluded_list_start,
+                                   excluded_list, excluded_numbers, nl->cap, nl->d_list_count,
+                                   nl->d_list, nl->d_rfirst, nl->d_roff, nl->d_flags,
+                                   nl->grid_length, nl->Nx, nl->Ny, nl->Nz);
+            }
+        } else {
             SA_LAUNCH(kSearchThreads, (&nl_search_cells_body<false>), (k_nl_search_cells<false>), dim3(nl->G), dim3(kSearchThreads), 0, s, nl->d_need,
                                nl->d_spos, nl->d_sexw, nl->d_cell_start, nl->d_ranges,
                                nl->d_nranges, nl->max_ranges, c, r, nl->thr, excluded_list_start,
                                excluded_list, excluded_numbers, nl->cap, nl->d_list_count,
                                nl->d_list, nl->d_rfirst, nl->d_roff, nl->d_flags,
                                nl->grid_length, nl->Nx, nl->Ny, nl->Nz);
+        }
         SA_LAUNCH_CHECK();
     }
     nl->built = 1;
