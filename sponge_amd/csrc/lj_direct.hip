@@ -267,9 +267,9 @@ __device__ __forceinline__ void fast_pair(const float4& pj, float2 ab, const flo
 
 // Force-only row of a staged cell whose partners sit at their images nearest
 // the cell's centre (lj_row with FAST, STAGED and SHIFT): the same pairs and
-// pair arithmetic, with the passes in which all 128 entries exist run under
-// wave-uniform loop control and without presence tests; only the partial
-// last pass masks its absent entries.
+// pair arithmetic, with the passes in which all 128 entries exist (the row's
+// first n / 128, the same count in every lane) run without presence tests;
+// only the partial last pass masks its absent entries.
 template <bool LJ>
 __device__ __forceinline__ void lj_row_fast(int p, int lane, int n, const unsigned short* __restrict__ row,
                                             const float4* cpos, const int* ctyp,
