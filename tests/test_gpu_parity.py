@@ -76,10 +76,12 @@ def test_neighbor_list_bit_exact_large_box(dev, variant):
 
 @pytest.mark.parametrize("variant", ["lattice", "perturbed"])
 def test_lj_direct_force_only_large_box(dev, variant):
-    """Force-only pair forces in a box of more than twice (cutoff + cell)
-    (24,000 atoms, 62.1 A), where the pair kernel stages partners as their
-    images nearest the cell centre; some atoms given as images a box length
-    away: per-atom forces against the oracle's reference arithmetic."""
+    """Force-only pair forces through the module call in a 24,000-atom, 62.1 A
+    box, some atoms given as images a box length away: per-atom forces
+    against the oracle's reference arithmetic.  (The module call cannot bound
+    how far atoms moved since the list was built, so it keeps the periodic
+    displacement; the step driver's cell-centre images are covered by
+    test_gpu_fast_rows.py and test_gpu_md.py.)"""
     import torch
     from oracle import oracle as O
     from sponge_amd import abi
