@@ -3,7 +3,7 @@
 // steps between full PME evaluations.  It has no counterpart in the
 // reference; it sits beside PM_force (SPONGE/PM_force/PM_force.cpp), whose
 // PME_Reciprocal_Force_With_Energy_And_Virial (:1070-1169) stays the
-// fallback and the judge of every sampled step.
+// fallback and the reference force every sampled step is checked against.
 //
 // Features of atom i (partners j on the neighbor list with r < Rf; d = r_j -
 // r_i, u = r / Rf, fc = (1 - u^2)^2, w_j = q_j fc, x = 2u - 1, T_k Chebyshev):
