@@ -21,6 +21,7 @@
 
 #include "internal.h"
 #include "batch_launch.h"
+#include "pair_term.h"
 #include "timing.h"
 
 using namespace sa;
@@ -160,6 +161,102 @@ __device__ __forceinline__ ScTerm sc_pair(float r2, float qi, float qj, float qb
     return t;
 }
 
+// Cell geometry for the force-only kernel's cell-centre images: reach =
+// cutoff + the largest distance a row atom may have moved since the list was
+// built (< 0: no bound known, periodic displacements kept).
+struct ShiftGeom {
+    float gl;
+    int Nx, Ny, Nz;
+    float reach;
+};
+
+// One pair of the force-only soft-core rows at cell-centre images (plain
+// displacement): the plain kernel's FAST arithmetic with the state-weighted
+// coefficients (1 - lambda) A_A + lambda A_B for pairs whose LJ is present in
+// both states or in neither, the reference's soft-core pair function for the
+// others (a branch taken only by wavefronts that hold one).  has = false: an
+// absent entry of a partial pass (r^2 = the cutoff threshold, out of range).
+__device__ __forceinline__ void sc_pair_fast(const float4& pj, int tpj, const float3& pi, float qi,
+                                             const float2* rowA, const float2* rowB, float lam,
+                                             float lam_, float rc2_thr, const ScParams& sp,
+                                             bool has, float& fx, float& fy, float& fz) {
+    const float3 d = make_float3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
+    float r2 = fmaf(d.z, d.z, fmaf(d.y, d.y, d.x * d.x));
+    r2 = has ? r2 : rc2_thr;
+    const bool in = r2 < rc2_thr;
+    const float2 a = rowA[tpj & 0xffff], b = rowB[tpj >> 16];
+    const bool plain = a.x * b.x != 0.f || a.x + b.x == 0.f;
+    const float rinv = __builtin_amdgcn_rsqf(r2);
+    const float rabs = r2 * rinv;
+    const float r2inv = rinv * rinv;
+    const float pw6 = r2inv * r2inv * r2inv;
+    const float pw8 = pw6 * r2inv;
+    const float pw3 = rinv * r2inv;
+    const float cA = fmaf(lam, b.x, lam_ * a.x), cB = fmaf(lam, b.y, lam_ * a.y);
+    float erfc_br, bracket;
+    coul_fast(r2, rabs, sp.beta, erfc_br, bracket);
+    float f = (cB - cA * pw6) * pw8 - qi * pj.w * pw3 * bracket;
+    const bool soft = in && !plain;
+    if (__ballot(soft)) {
+        // (charge_BA enters only dU/dlambda)
+        if (soft) f = sc_pair<false, false>(r2, qi, pj.w, 0.f, 0.f, a, b, sp).f;
+    }
+    f = in ? f : 0.f;
+    fx = fmaf(f, d.x, fx);
+    fy = fmaf(f, d.y, fy);
+    fz = fmaf(f, d.z, fz);
+}
+
+// Force-only row of a staged cell at cell-centre images: two entries per
+// lane per pass from one 32-bit word (prefetched a pass ahead), the passes in
+// which all 128 entries exist without presence tests, the partial last pass
+// masked.
+__device__ __forceinline__ void sc_row_fast(int p, int lane, int n, const unsigned short* __restrict__ row,
+                                            const float4* cpos, const int* ctp,
+                                            const float4* __restrict__ pos, const float2* rowA,
+                                            const float2* rowB, const LT6& c, const LT6& r,
+                                            float rc2_thr, const ScParams& sp,
+                                            float4* __restrict__ facc, float3 C) {
+    const float lam = sp.lambda, lam_ = 1.0f - sp.lambda;
+    const float4 pi4 = pos[p];
+    const float3 pi = near_image(make_float3(pi4.x, pi4.y, pi4.z), C, c, r);
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    const int npairs = (n + 1) >> 1;
+    const int full = n >> 7;
+    const unsigned* row2 = reinterpret_cast<const unsigned*>(row);
+    const unsigned last = npairs > 0 ? npairs - 1 : 0;
+    unsigned cur = row2[(unsigned)lane < last ? (unsigned)lane : last];
+    int w = lane;
+    for (int k = 0; k < full; ++k, w += kWave) {
+        const unsigned wn = (unsigned)(w + kWave);
+        unsigned nxt = row2[wn < last ? wn : last];
+        const int ta = (int)(cur & 0xffffu), tb = (int)(cur >> 16);
+        sc_pair_fast(cpos[ta], ctp[ta], pi, pi4.w, rowA, rowB, lam, lam_, rc2_thr, sp, true, fx,
+                     fy, fz);
+        sc_pair_fast(cpos[tb], ctp[tb], pi, pi4.w, rowA, rowB, lam, lam_, rc2_thr, sp, true, fx,
+                     fy, fz);
+        // the loaded word is opaque from here: the compiler cannot fold the
+        // prefetch back into a load at the top of the next pass
+        asm volatile("" : "+v"(nxt));
+        cur = nxt;
+    }
+    if (full * 2 * kWave < n) {
+        // cur is the row's word w, or its last word for lanes past the end
+        const bool has_a = w < npairs;
+        const bool has_b = 2 * w + 1 < n;
+        const int ta = (int)(cur & 0xffffu);
+        const int tb = has_b ? (int)(cur >> 16) : ta;
+        sc_pair_fast(cpos[ta], ctp[ta], pi, pi4.w, rowA, rowB, lam, lam_, rc2_thr, sp, has_a, fx,
+                     fy, fz);
+        sc_pair_fast(cpos[tb], ctp[tb], pi, pi4.w, rowA, rowB, lam, lam_, rc2_thr, sp, has_b, fx,
+                     fy, fz);
+    }
+    fx = wave_sum(fx);
+    fy = wave_sum(fy);
+    fz = wave_sum(fz);
+    if (lane == 0) facc[p] = make_float4(fx, fy, fz, 0.f);
+}
+
 template <bool ORTHO, bool F, bool E, bool V, bool DU, bool FAST = false>
 __device__ __forceinline__ void ljsc_cells_body(int sa_block,
     const int* __restrict__ start, const int* __restrict__ nranges, int max_ranges,
@@ -169,7 +266,7 @@ __device__ __forceinline__ void ljsc_cells_body(int sa_block,
     const int* __restrict__ pgid, const float2* __restrict__ ABa, int TA,
     const float2* __restrict__ ABb, int TB, LT6 c, LT6 r, float rc2_thr, ScParams sp, int stage,
     float4* __restrict__ facc, float2* __restrict__ eacc, float* __restrict__ vacc,
-    double* __restrict__ du) {
+    double* __restrict__ du, ShiftGeom sg) {
     extern __shared__ float4 cpos[];
     // force-only steps stage positions and types only (20 B per entry, so
     // two workgroups share a CU)
@@ -191,9 +288,30 @@ __device__ __forceinline__ void ljsc_cells_body(int sa_block,
     __syncthreads();
     const int total = r_off[nr];
     const bool staged = total <= stage;
+    // force-only steps in an orthorhombic box wider than twice (half the cell
+    // + reach) on every axis: partners staged at their images nearest the
+    // cell's centre C (the plain pair kernel's construction)
+    float3 C = make_float3(0.f, 0.f, 0.f);
+    bool shift = false;
+    if (FAST && ORTHO && sg.reach >= 0.f) {
+        const int ix = cell / (sg.Ny * sg.Nz), iy = (cell / sg.Nz) % sg.Ny, iz = cell % sg.Nz;
+        const float gl = sg.gl;
+        const float lx = ix * gl, hx = ix == sg.Nx - 1 ? c.a11 : (ix + 1) * gl;
+        const float ly = iy * gl, hy = iy == sg.Ny - 1 ? c.a22 : (iy + 1) * gl;
+        const float lz = iz * gl, hz = iz == sg.Nz - 1 ? c.a33 : (iz + 1) * gl;
+        C = make_float3(0.5f * (lx + hx), 0.5f * (ly + hy), 0.5f * (lz + hz));
+        shift = staged && 0.5f * c.a11 > sg.reach + 0.5f * (hx - lx) + 0.01f &&
+                0.5f * c.a22 > sg.reach + 0.5f * (hy - ly) + 0.01f &&
+                0.5f * c.a33 > sg.reach + 0.5f * (hz - lz) + 0.01f;
+    }
     if (staged)
         stage_stream(nr, r_first, r_off, total, [&](int t, int q) {
-            cpos[t] = pos[q];
+            float4 v = pos[q];
+            if (shift) {
+                const float3 im = near_image(make_float3(v.x, v.y, v.z), C, c, r);
+                v = make_float4(im.x, im.y, im.z, v.w);
+            }
+            cpos[t] = v;
             ctp[t] = ptp[q];
             if (!FAST) {
                 cqba[t] = pqba[q];
@@ -211,6 +329,15 @@ __device__ __forceinline__ void ljsc_cells_body(int sa_block,
         // soft-core pairs the reference's pair function in a branch taken
         // only by the wavefronts that hold one
         const float lam = sp.lambda, lam_ = 1.0f - sp.lambda;
+        if (shift) {
+            for (int p = a0 + wave; p < a1; p += nwave) {
+                const int tpi = ptp[p];
+                sc_row_fast(p, lane, list_count[p], list + (size_t)p * cap, cpos, ctp, pos,
+                            ta + (tpi & 0xffff) * TA, tb + (tpi >> 16) * TB, c, r, rc2_thr, sp,
+                            facc, C);
+            }
+            return;
+        }
         for (int p = a0 + wave; p < a1; p += nwave) {
             const float4 pi4 = pos[p];
             const float3 pi = make_float3(pi4.x, pi4.y, pi4.z);
@@ -349,8 +476,24 @@ __device__ __forceinline__ void ljsc_cells_body(int sa_block,
     }
 }
 
+}  // namespace
+
+// The force-only kernels run at 8 wavefronts per SIMD (at most 64 VGPRs), as
+// the plain pair kernel does.
+template <>
+struct sa::BodyWaves<&ljsc_cells_body<true, true, false, false, false, true>> {
+    static constexpr int value = 8;
+};
+template <>
+struct sa::BodyWaves<&ljsc_cells_body<false, true, false, false, false, true>> {
+    static constexpr int value = 8;
+};
+
+namespace {
+
 template <bool ORTHO, bool F, bool E, bool V, bool DU, bool FAST = false>
-__global__ void __launch_bounds__(1024) k_ljsc_cells(
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(FAST ? 8 : 1)))
+k_ljsc_cells(
     const int* __restrict__ start, const int* __restrict__ nranges, int max_ranges,
     const int* __restrict__ g_rfirst, const int* __restrict__ g_roff,
     const int* __restrict__ list_count, const unsigned short* __restrict__ list, int cap,
@@ -358,10 +501,10 @@ __global__ void __launch_bounds__(1024) k_ljsc_cells(
     const int* __restrict__ pgid, const float2* __restrict__ ABa, int TA,
     const float2* __restrict__ ABb, int TB, LT6 c, LT6 r, float rc2_thr, ScParams sp, int stage,
     float4* __restrict__ facc, float2* __restrict__ eacc, float* __restrict__ vacc,
-    double* __restrict__ du) {
+    double* __restrict__ du, ShiftGeom sg) {
     ljsc_cells_body<ORTHO, F, E, V, DU, FAST>(blockIdx.x, start, nranges, max_ranges, g_rfirst, g_roff,
                                          list_count, list, cap, pos, ptp, pqba, pgid, ABa, TA, ABb,
-                                         TB, c, r, rc2_thr, sp, stage, facc, eacc, vacc, du);
+                                         TB, c, r, rc2_thr, sp, stage, facc, eacc, vacc, du, sg);
 }
 
 // Scatter of the rows: frc +=; atom_energy += LJ + Coulomb; the module's LJ
@@ -457,8 +600,9 @@ void ljsc_free(sponge_ljsc* h) {
 
 template <bool ORTHO>
 int launch(sponge_ljsc* h, sponge_nl* nl, LT6 c, LT6 r, float beta, bool F, bool E, bool V,
-           bool DU) {
+           bool DU, float reach = -1.f) {
     const ScParams sp{h->lambda, h->alpha, h->p, h->sigma6, h->sigma6_min, beta};
+    const ShiftGeom sg{nl->grid_length, nl->Nx, nl->Ny, nl->Nz, reach};
     const bool fast = F && !E && !V && !DU && beta * h->cutoff <= kErfcxMax;
     const int stage = fast ? h->stage_fast : h->stage;
     const size_t sh = (size_t)stage * (sizeof(float4) + (fast ? 1 : 3) * sizeof(int)) +
@@ -469,7 +613,7 @@ int launch(sponge_ljsc* h, sponge_nl* nl, LT6 c, LT6 r, float beta, bool F, bool
                        nl->d_cell_start, nl->d_nranges, nl->max_ranges, nl->d_rfirst, nl->d_roff,   \
                        nl->d_list_count, nl->d_list, nl->cap, h->d_pos, h->d_tp, h->d_qba,          \
                        h->d_gid, h->d_ABa, h->TA, h->d_ABb, h->TB, c, r, h->rc2_thr, sp, stage, \
-                       h->d_facc, h->d_eacc, h->d_vacc, h->d_du)
+                       h->d_facc, h->d_eacc, h->d_vacc, h->d_du, sg)
     if (DU) SA_SC(false, false, false, true, false);
     else if (E && V) SA_SC(true, true, true, false, false);
     else if (E) SA_SC(true, true, false, false, false);
@@ -711,10 +855,15 @@ int sa::ljsc_direct_force_md(sponge_ljsc* h, sponge_nl* nl, const float* crd, co
     if (rc) return rc;
     const LT6 c = lt_from(cell), r = lt_from(rcell);
     const bool E = need_atom_energy != 0;
+    // the automatic refresh rebuilds once any atom moved more than
+    // skin * skin_permit: the force-only kernel may use cell-centre images
+    const float reach = nl->opt.refresh_interval <= 0 ? h->cutoff + nl->skin * nl->opt.skin_permit
+                                                      : -1.f;
     {
         ScopedTimer timer("ljsc_pairs", h->stream);
-        rc = lt_is_orthorhombic(c) ? launch<true>(h, nl, c, r, pme_beta, true, E, false, false)
-                                   : launch<false>(h, nl, c, r, pme_beta, true, E, false, false);
+        rc = lt_is_orthorhombic(c)
+                 ? launch<true>(h, nl, c, r, pme_beta, true, E, false, false, reach)
+                 : launch<false>(h, nl, c, r, pme_beta, true, E, false, false, reach);
     }
     if (rc) return rc;
     const dim3 g(grid_for(h->N, 256)), b(256);
