@@ -469,21 +469,21 @@ __device__ __forceinline__ void nl_search_cells_body(int sa_block,
                     for (int h = 0; h < H; ++h) {
 #pragma unroll
                         for (int u = 0; u < NA; ++u) {
-                            bool near;
                             if (SH) {
                                 const float dx = pj[h].x - pc[u].x, dy = pj[h].y - pc[u].y,
                                             dz = pj[h].z - pc[u].z;
                                 const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
                                 ok[h][u] = r2 < thr_lo;
-                                near = r2 < thr_hi;
-                                amb[h][u] = near & !ok[h][u];
+                                amb[h][u] = (r2 < thr_hi) & !ok[h][u];
                             } else {
                                 // Get_Displacement(crd_i, crd_j) as in Find_Neighbors_Gridly
                                 ok[h][u] = in[h] & (norm2_exact(disp_pbc<ORTHO>(pi[u], pj[h], c, r)) < thr);
-                                near = ok[h][u];
                                 amb[h][u] = false;
                             }
-                            chk[h][u] = near & (gj[h] >= wi[u].x) & (gj[h] <= wi[u].y);
+                            // inside the row's window (one unsigned compare;
+                            // the rare path applies the exclusion rule only
+                            // to the candidates found within the list radius)
+                            chk[h][u] = (unsigned)(gj[h] - wi[u].x) <= (unsigned)(wi[u].y - wi[u].x);
                             rare |= amb[h][u] | chk[h][u];
                         }
                     }
