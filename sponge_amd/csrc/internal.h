@@ -169,7 +169,7 @@ int stage_entries(size_t bytes_per_entry, int want, size_t reserve = 0);
 int lj_direct_force_md(sponge_lj* lj, sponge_nl* nl, const float* crd, const float* charge,
                        float* frc, const float* cell, const float* rcell, float pme_beta,
                        int need_atom_energy, float* atom_energy, float* atom_direct_pme_energy,
-                       int* reset_flag);
+                       int* reset_flag, bool* deferred = nullptr);
 int nl_update_impl(sponge_nl* nl, const int* atom_local, int local_atom_numbers, int ghost_numbers,
                    const float* crd, const float* cell, const float* rcell, int step, int update,
                    const int* excluded_list_start, const int* excluded_list,
@@ -185,10 +185,19 @@ void rsur_set_stream(sponge_rsur* h, hipStream_t s);
 // energy) on the module's stream, and that stream
 int pme_recip_begin(sponge_pme* p, const float* crd, const float* cell, const float* rcell,
                     const float* charge, int need_virial, int need_energy, int step);
-// extra (nullable, [N][3]): forces the gather adds with the reciprocal ones
+// extra (nullable, [N][3]): forces the gather adds with the reciprocal ones.
+// base (nullable): the gather assigns frc[i] = base[rank[i]].xyz + reciprocal
+// + extra instead of adding to frc (base: the pair kernel's per-row forces
+// in sorted order, rank: the list's atom -> sorted position map), and clears
+// *reset (nullable) as the pair path's finalize pass would
+struct GatherBase {
+    const float4* base = nullptr;
+    const int* rank = nullptr;
+    int* reset = nullptr;
+};
 int pme_recip_end(sponge_pme* p, const float* rcell, const float* charge, float* frc,
                   int need_virial, int need_energy, float* d_virial, float* d_potential, int step,
-                  const float* extra = nullptr);
+                  const float* extra = nullptr, GatherBase gb = GatherBase{});
 hipStream_t pme_stream(const sponge_pme* p);
 bool pme_calculates_reciprocal(const sponge_pme* p);
 bool pme_updates_on(const sponge_pme* p, int step);  // the reciprocal part is evaluated this step

@@ -839,8 +839,9 @@ extern "C" int sponge_lj_long_range_factor(sponge_lj* lj, float* factor) {
 int sa::lj_direct_force_md(sponge_lj* lj, sponge_nl* nl, const float* crd, const float* charge,
                            float* frc, const float* cell, const float* rcell, float pme_beta,
                            int need_atom_energy, float* atom_energy, float* atom_direct_pme_energy,
-                           int* reset_flag) {
+                           int* reset_flag, bool* deferred) {
     if (!nl->built) return SPONGE_AMD_ERROR_MISSING_COMMAND;
+    if (deferred) *deferred = false;
     hipStream_t s = lj->stream;
     const LT6 c = lt_from(cell), r = lt_from(rcell);
     int rc = nl_pack_for_pairs(nl, crd, charge, lj->d_type, lj->d_pos, lj->d_ptype, lj->d_pgid, s);
@@ -872,6 +873,12 @@ int sa::lj_direct_force_md(sponge_lj* lj, sponge_nl* nl, const float* crd, const
     else
         rc = launch_rows<false>(lj, nl, c, r, pme_beta, need_atom_energy, 0, s, true, -1, inner);
     if (rc) return rc;
+    // a force-only step may leave the per-row forces for the caller's next
+    // pass (the PME gather assigns them with the reciprocal ones)
+    if (deferred && !need_atom_energy && !inner) {
+        *deferred = true;
+        return 0;
+    }
     return finalize(lj, lj->N, nl->d_sidx, need_atom_energy, 0, frc, atom_energy, nullptr,
                     atom_direct_pme_energy, s, 1, reset_flag, inner ? nl->d_need_prune : nullptr);
 }

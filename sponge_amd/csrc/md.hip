@@ -512,15 +512,20 @@ int md_step(sponge_md* m, long long step, int need_energy, StepKind* kind_out = 
     // clears the list's rebuild flag once the list has been used this step.
     // The terms are summed in another order than Main_Calculate_Force's, which
     // changes only the rounding of the total force.
+    bool deferred = false;
     if (m->sc) {
         ljsc_set_stream(m->sc, s);
         rc = ljsc_direct_force_md(m->sc, m->nl, m->d_crd, m->d_charge, m->d_frc, cell, rcell,
                                   m->pme->beta, need_energy, m->d_atom_energy,
                                   m->pme->d_direct_atom_energy, m->nl->d_need);
     } else {
+        // with the excluded-pair correction on the side stream the gather is
+        // the first pass to write the force array: it assigns the pair
+        // kernel's per-row forces with the reciprocal ones (no finalize pass)
         rc = lj_direct_force_md(m->lj, m->nl, m->d_crd, m->d_charge, m->d_frc, cell, rcell,
                                 m->pme->beta, need_energy, m->d_atom_energy,
-                                m->pme->d_direct_atom_energy, m->nl->d_need);
+                                m->pme->d_direct_atom_energy, m->nl->d_need,
+                                ex_side ? &deferred : nullptr);
     }
     if (rc) return rc;
     m->need_cleared = true;
@@ -541,9 +546,15 @@ int md_step(sponge_md* m, long long step, int need_energy, StepKind* kind_out = 
     const int N3 = 3 * N;
     if (overlap) {
         SA_CHECK(hipStreamWaitEvent(s, m->ev_join, 0));
+        GatherBase gb;
+        if (deferred) {
+            gb.base = m->lj->d_facc;
+            gb.rank = m->nl->d_rank;
+            gb.reset = m->nl->d_need;
+        }
         rc = pme_recip_end(m->pme, rcell, m->d_charge, m->d_frc, 0, need_energy, nullptr,
                            need_energy ? m->d_potential : nullptr, (int)step,
-                           ex_side ? m->d_exfrc : nullptr);
+                           ex_side ? m->d_exfrc : nullptr, gb);
         if (rc) return rc;
     } else if (rec_side) {
         g_recorder->join_next = true;

@@ -798,8 +798,10 @@ __device__ __forceinline__ void pme_gather_tiles_body(int sa_block,
     TileGeom tg, const int* __restrict__ gcount, const float4* __restrict__ bw,
     const int* __restrict__ bu, const int* __restrict__ bi, const float4* __restrict__ ow,
     const int* __restrict__ ou, const int* __restrict__ oi, const float* __restrict__ phi, LT6 rc,
-    float update_interval, float* __restrict__ frc, const float* __restrict__ extra) {
+    float update_interval, float* __restrict__ frc, const float* __restrict__ extra,
+    const float4* __restrict__ base, const int* __restrict__ base_rank, int* __restrict__ reset) {
     extern __shared__ float tile[];
+    if (reset && sa_block == 0 && threadIdx.x == 0) *reset = 0;
     const int b = brick_for_block(sa_block, nb);
     const int cz = b % nbz, cy = (b / nbz) % nby, cx = b / (nbz * nby);
     const int X0 = cx * bx, Y0 = cy * by, Z0 = cz * bz;
@@ -894,9 +896,21 @@ __device__ __forceinline__ void pme_gather_tiles_body(int sa_block,
             ey = extra[3 * i + 1];
             ez = extra[3 * i + 2];
         }
-        frc[3 * i] = frc[3 * i] + update_interval * ox + ex;
-        frc[3 * i + 1] = frc[3 * i + 1] + update_interval * oy + ey;
-        frc[3 * i + 2] = frc[3 * i + 2] + update_interval * oz + ez;
+        // every atom is in exactly one brick's list (own slots or overflow)
+        float f0x, f0y, f0z;
+        if (base) {
+            const float4 f0 = base[base_rank[i]];
+            f0x = f0.x;
+            f0y = f0.y;
+            f0z = f0.z;
+        } else {
+            f0x = frc[3 * i];
+            f0y = frc[3 * i + 1];
+            f0z = frc[3 * i + 2];
+        }
+        frc[3 * i] = f0x + update_interval * ox + ex;
+        frc[3 * i + 1] = f0y + update_interval * oy + ey;
+        frc[3 * i + 2] = f0z + update_interval * oz + ez;
     }
 }
 
@@ -905,8 +919,9 @@ __global__ void __launch_bounds__(kGatherThreads) k_pme_gather_tiles(
     TileGeom tg, const int* __restrict__ gcount, const float4* __restrict__ bw,
     const int* __restrict__ bu, const int* __restrict__ bi, const float4* __restrict__ ow,
     const int* __restrict__ ou, const int* __restrict__ oi, const float* __restrict__ phi, LT6 rc,
-    float update_interval, float* __restrict__ frc, const float* __restrict__ extra) {
-    pme_gather_tiles_body(blockIdx.x, fx, fy, fz, bx, by, bz, nbx, nby, nbz, nb, bcap, tg, gcount, bw, bu, bi, ow, ou, oi, phi, rc, update_interval, frc, extra);
+    float update_interval, float* __restrict__ frc, const float* __restrict__ extra,
+    const float4* __restrict__ base, const int* __restrict__ base_rank, int* __restrict__ reset) {
+    pme_gather_tiles_body(blockIdx.x, fx, fy, fz, bx, by, bz, nbx, nby, nbz, nb, bcap, tg, gcount, bw, bu, bi, ow, ou, oi, phi, rc, update_interval, frc, extra, base, base_rank, reset);
 }
 
 __global__ void k_pme_charge_sums(int N, const float* __restrict__ q, double* __restrict__ acc) {
@@ -1520,7 +1535,7 @@ int sa::pme_recip_begin(sponge_pme* p, const float* crd, const float* cell, cons
 
 int sa::pme_recip_end(sponge_pme* p, const float* rcell, const float* charge, float* frc,
                       int need_virial, int need_energy, float* d_virial, float* d_potential,
-                      int step, const float* extra) {
+                      int step, const float* extra, GatherBase gb) {
     const LT6 r = lt_from(rcell);
     hipStream_t s = p->stream;
     const int N = p->N;
@@ -1535,7 +1550,7 @@ int sa::pme_recip_end(sponge_pme* p, const float* rcell, const float* charge, fl
                            sizeof(float) * tg.lds_floats(), s, p->fx, p->fy, p->fz, p->bx, p->by,
                            p->bz, p->nbx, p->nby, p->nbz, p->nb, p->bcap, tg, p->d_gcount, p->d_bw,
                            p->d_bu, p->d_bi, p->d_ow, p->d_ou, p->d_oi, p->d_FBCFQ, r,
-                           (float)p->opt.update_interval, frc, extra);
+                           (float)p->opt.update_interval, frc, extra, gb.base, gb.rank, gb.reset);
         SA_LAUNCH_CHECK();
     }
     if (need_energy) {
